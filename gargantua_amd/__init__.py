@@ -1,0 +1,320 @@
+"""gargantua_amd -- MI355X geodesic renderer behind the reference's render API.
+
+Host-side mirror of the reference interface on top of libgargantua.so (the
+C ABI of include/gargantua.h):
+
+  Schwarzschild, Kerr            crates/gr-core/src/{schwarzschild,kerr}.rs
+  RK45Integrator                 crates/gr-core/src/geodesic.rs:155-181
+  Camera                         crates/raytracer/src/curved/camera.rs:9-15
+  AccretionDisk                  crates/raytracer/src/curved/disk.rs:8-31
+  RenderOptions                  crates/raytracer/src/curved/renderer.rs:17-26
+  render, render_with_disk       crates/raytracer/src/curved/renderer.rs:82-101
+  Renderer.trace_states          trace_ray / trace_ray_with_disk (tracer.rs:54-152)
+  Renderer.geodesic_acceleration Metric::geodesic_acceleration (metric.rs:37-52)
+
+Every computation runs in the HIP library; this module only builds the POD
+structs and numpy buffers.  If libgargantua.so is missing it raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _abi
+from . import configs  # noqa: F401  (scene builders for the benchmark configurations)
+from ._abi import (GG_CLASS_DISK, GG_CLASS_ESCAPED, GG_CLASS_HORIZON, GG_CLASS_MAX_STEPS,  # noqa: F401
+                   GG_DISK_NONE, GG_DISK_THIN, GG_DISK_VOLUMETRIC, GG_ERR_REFERENCE, GG_ERR_RTOL_ATOL,
+                   GG_METRIC_KERR, GG_METRIC_SCHWARZSCHILD)
+
+PRECISIONS = {"fp64": _abi.GG_PREC_FP64, "mixed": _abi.GG_PREC_MIXED}
+
+
+class GargantuaError(RuntimeError):
+    """A libgargantua call returned a negative status."""
+
+
+# ---------------------------------------------------------------- metrics --
+
+@dataclass
+class Schwarzschild:
+    """Schwarzschild::new(mass) (schwarzschild.rs:13-18)."""
+    mass: float = 1.0
+
+    def to_c(self) -> _abi.gg_metric:
+        return _abi.gg_metric(kind=GG_METRIC_SCHWARZSCHILD, mass=self.mass, spin=0.0)
+
+    def event_horizon(self) -> float:
+        return 2.0 * self.mass
+
+
+@dataclass
+class Kerr:
+    """Kerr::new(mass, spin); the library clamps |spin| to 0.9999 M (kerr.rs:29-34)."""
+    mass: float = 1.0
+    spin: float = 0.9
+
+    def __post_init__(self):
+        if not self.mass > 0.0:
+            raise ValueError("Kerr: mass must be positive")  # kerr.rs:30 panics
+
+    def to_c(self) -> _abi.gg_metric:
+        return _abi.gg_metric(kind=GG_METRIC_KERR, mass=self.mass, spin=self.spin)
+
+    def event_horizon(self) -> float:
+        a = max(-0.9999 * self.mass, min(0.9999 * self.mass, self.spin))
+        return self.mass + math.sqrt(self.mass * self.mass - a * a)
+
+
+# ------------------------------------------------------- scene components --
+
+@dataclass
+class RK45Integrator:
+    """RK45Integrator defaults (geodesic.rs:170-181) plus the rtol/atol mode."""
+    initial_step: float = 0.1
+    min_step: float = 1e-8
+    max_step: float = 1.0
+    tolerance: float = 1e-6
+    max_radius: float = 100.0
+    horizon_buffer: float = 0.01
+    horizon_rel: float = 0.0
+    rtol: float = 0.0
+    atol: float = 0.0
+    err_mode: int = GG_ERR_REFERENCE
+    max_steps: int = 5000
+
+    def to_c(self) -> _abi.gg_integrator:
+        return _abi.gg_integrator(initial_step=self.initial_step, min_step=self.min_step, max_step=self.max_step,
+                                  tolerance=self.tolerance, max_radius=self.max_radius,
+                                  horizon_buffer=self.horizon_buffer, horizon_rel=self.horizon_rel,
+                                  rtol=self.rtol, atol=self.atol, err_mode=self.err_mode,
+                                  max_steps=self.max_steps)
+
+
+@dataclass
+class Camera:
+    """Camera (camera.rs:9-15): position (t, r, theta, phi), look_at, up, fov_y, aspect."""
+    position: Sequence[float] = (0.0, 30.0, math.pi / 2.0, 0.0)
+    look_at: Sequence[float] = (-1.0, 0.0, 0.0)
+    up: Sequence[float] = (0.0, 0.0, 1.0)
+    fov_y_radians: float = math.radians(60.0)
+    aspect: float = 1.0
+
+    def to_c(self) -> _abi.gg_camera:
+        c = _abi.gg_camera()
+        c.position[:] = [float(x) for x in self.position]
+        c.look_at[:] = [float(x) for x in self.look_at]
+        c.up[:] = [float(x) for x in self.up]
+        c.fov_y_radians = self.fov_y_radians
+        c.aspect = self.aspect
+        return c
+
+
+@dataclass
+class AccretionDisk:
+    """AccretionDisk defaults (disk.rs:21-31); kind selects the disk model."""
+    r_inner: float = 6.0
+    r_outer: float = 20.0
+    scale_height: float = 1.0
+    density_0: float = 1.0
+    temperature_0: float = 1.0
+    absorption: float = 0.1
+    emissivity_q: float = 3.0
+    edge_width: float = 0.0
+    kind: int = GG_DISK_VOLUMETRIC
+
+    def to_c(self) -> _abi.gg_disk:
+        return _abi.gg_disk(kind=self.kind, r_inner=self.r_inner, r_outer=self.r_outer,
+                            scale_height=self.scale_height, density_0=self.density_0,
+                            temperature_0=self.temperature_0, absorption=self.absorption,
+                            emissivity_q=self.emissivity_q, edge_width=self.edge_width)
+
+
+@dataclass
+class RenderOptions:
+    """RenderOptions (renderer.rs:17-26); the sky is the procedural starfield."""
+    samples_per_axis: int = 1
+    show_progress: bool = False
+
+
+@dataclass
+class Scene:
+    """Everything one render call needs (gg_scene)."""
+    metric: object = field(default_factory=Schwarzschild)
+    camera: Camera = field(default_factory=Camera)
+    disk: Optional[AccretionDisk] = None
+    integrator: RK45Integrator = field(default_factory=RK45Integrator)
+    width: int = 32
+    height: int = 24
+    samples_per_axis: int = 1
+    jitter: bool = False
+    seed: int = 0
+
+    def to_c(self) -> _abi.gg_scene:
+        s = _abi.gg_scene()
+        s.metric = self.metric.to_c()
+        s.camera = self.camera.to_c()
+        s.disk = self.disk.to_c() if self.disk is not None else _abi.gg_disk(kind=GG_DISK_NONE)
+        s.integrator = self.integrator.to_c()
+        s.image = _abi.gg_image(width=self.width, height=self.height, samples_per_axis=self.samples_per_axis,
+                                jitter=1 if self.jitter else 0, seed=self.seed)
+        return s
+
+    @property
+    def n_rays(self) -> int:
+        return self.width * self.samples_per_axis * self.height * self.samples_per_axis
+
+
+# --------------------------------------------------------------- renderer --
+
+def _check(lib, rc: int, what: str):
+    if rc != 0:
+        raise GargantuaError(f"{what}: {lib.gg_strerror(rc).decode()} ({rc}): {lib.gg_last_error().decode()}")
+
+
+def alloc_outputs(n: int, *, state: bool = True, rgb_pixels: int = 0) -> dict:
+    """Host numpy buffers for one scene's per-ray outputs."""
+    o = {
+        "cls": np.zeros(n, np.uint8), "r": np.zeros(n), "theta": np.zeros(n), "phi": np.zeros(n),
+        "intensity": np.zeros(n), "attempts": np.zeros(n, np.uint32), "accepted": np.zeros(n, np.uint32),
+    }
+    if state:
+        o["state"] = np.zeros((9, n))
+    if rgb_pixels:
+        o["rgb"] = np.zeros((rgb_pixels, 3), np.uint8)
+    return o
+
+
+def outputs_struct(o: dict) -> _abi.gg_outputs:
+    """gg_outputs pointing at numpy arrays or at (device) integer addresses."""
+    def p(k):
+        v = o.get(k)
+        if v is None:
+            return None
+        if isinstance(v, int):
+            return v
+        assert v.flags["C_CONTIGUOUS"]
+        return v.ctypes.data
+    return _abi.gg_outputs(cls=p("cls"), r=p("r"), theta=p("theta"), phi=p("phi"), intensity=p("intensity"),
+                           attempts=p("attempts"), accepted=p("accepted"), state=p("state"), rgb=p("rgb"))
+
+
+def stats_dict(st: _abi.gg_stats) -> dict:
+    return {k: getattr(st, k) for k, _ in _abi.gg_stats._fields_}
+
+
+class Renderer:
+    """A libgargantua context (device, stream, device buffers)."""
+
+    def __init__(self, device: int = 0, stream: Optional[int] = None):
+        self.lib = _abi.load()
+        h = C.c_void_p()
+        _check(self.lib, self.lib.gg_context_create(device, C.c_void_p(stream) if stream else None, C.byref(h)),
+               "gg_context_create")
+        self.ctx = h
+
+    def close(self):
+        if self.ctx:
+            self.lib.gg_context_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_launch(self, block_threads=0, blocks_per_cu=0, refill_interval=0):
+        L = _abi.gg_launch(block_threads, blocks_per_cu, refill_interval)
+        _check(self.lib, self.lib.gg_set_launch(self.ctx, C.byref(L)), "gg_set_launch")
+
+    def get_launch(self) -> dict:
+        L = _abi.gg_launch()
+        _check(self.lib, self.lib.gg_get_launch(self.ctx, C.byref(L)), "gg_get_launch")
+        return {k: getattr(L, k) for k, _ in _abi.gg_launch._fields_}
+
+    def render(self, scene: Scene, precision: str = "fp64", outputs: Optional[dict] = None, rgb: bool = False,
+               state: bool = True) -> dict:
+        """Trace every ray of `scene`; returns the per-ray arrays and stats."""
+        sc = scene.to_c()
+        o = outputs if outputs is not None else alloc_outputs(
+            scene.n_rays, state=state, rgb_pixels=scene.width * scene.height if rgb else 0)
+        st = _abi.gg_stats()
+        _check(self.lib, self.lib.gg_render(self.ctx, C.byref(sc), PRECISIONS[precision],
+                                            C.byref(outputs_struct(o)), C.byref(st)), "gg_render")
+        o["stats"] = stats_dict(st)
+        return o
+
+    def render_batch(self, scenes: Sequence[Scene], precision: str = "mixed",
+                     outputs: Optional[Sequence[dict]] = None) -> tuple:
+        arr = (_abi.gg_scene * len(scenes))(*[s.to_c() for s in scenes])
+        outs = list(outputs) if outputs is not None else [alloc_outputs(s.n_rays, state=False) for s in scenes]
+        oarr = (_abi.gg_outputs * len(scenes))(*[outputs_struct(o) for o in outs])
+        st = _abi.gg_stats()
+        _check(self.lib, self.lib.gg_render_batch(self.ctx, arr, len(scenes), PRECISIONS[precision], oarr,
+                                                  C.byref(st)), "gg_render_batch")
+        return outs, stats_dict(st)
+
+    def trace_states(self, scene: Scene, states: np.ndarray, precision: str = "fp64") -> dict:
+        """trace_ray / trace_ray_with_disk from given states [n, 9] (t r th ph vt vr vth vph lambda)."""
+        states = np.ascontiguousarray(states, dtype=np.float64)
+        n = states.shape[0]
+        o = alloc_outputs(n)
+        st = _abi.gg_stats()
+        sc = scene.to_c()
+        _check(self.lib, self.lib.gg_trace_states(self.ctx, C.byref(sc), PRECISIONS[precision], n,
+                                                  states.ctypes.data, C.byref(outputs_struct(o)), C.byref(st)),
+               "gg_trace_states")
+        o["stats"] = stats_dict(st)
+        return o
+
+    def geodesic_acceleration(self, metric, pos: np.ndarray, vel: np.ndarray, precision: str = "fp64"):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 4)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 4)
+        acc = np.zeros_like(pos)
+        m = metric.to_c()
+        _check(self.lib, self.lib.gg_geodesic_acceleration(self.ctx, C.byref(m), PRECISIONS[precision], len(pos),
+                                                           pos.ctypes.data, vel.ctypes.data, acc.ctypes.data),
+               "gg_geodesic_acceleration")
+        return acc
+
+
+_default_renderer: Optional[Renderer] = None
+
+
+def _renderer() -> Renderer:
+    global _default_renderer
+    if _default_renderer is None:
+        _default_renderer = Renderer(0)
+    return _default_renderer
+
+
+def _reference_render_scene(metric, camera, disk, width, height, opts) -> Scene:
+    # render_inner hard-codes max_radius 200 and 5000 attempts (renderer.rs:264-265).
+    integ = RK45Integrator(max_radius=200.0, max_steps=5000)
+    return Scene(metric=metric, camera=camera, disk=disk, integrator=integ, width=width, height=height,
+                 samples_per_axis=max(1, opts.samples_per_axis if opts else 1))
+
+
+def render(metric, camera: Camera, width: int, height: int, opts: Optional[RenderOptions] = None) -> np.ndarray:
+    """raytracer::curved::render (renderer.rs:82-90): returns an (height, width, 3) uint8 image."""
+    sc = _reference_render_scene(metric, camera, None, width, height, opts or RenderOptions())
+    o = _renderer().render(sc, "fp64", rgb=True)
+    return o["rgb"].reshape(height, width, 3)
+
+
+def render_with_disk(metric, camera: Camera, disk: AccretionDisk, width: int, height: int,
+                     opts: Optional[RenderOptions] = None) -> np.ndarray:
+    """raytracer::curved::render_with_disk (renderer.rs:92-101)."""
+    sc = _reference_render_scene(metric, camera, disk, width, height, opts or RenderOptions())
+    o = _renderer().render(sc, "fp64", rgb=True)
+    return o["rgb"].reshape(height, width, 3)

@@ -1,0 +1,605 @@
+// gg_api.hip -- host side of libgargantua: the C ABI of include/gargantua.h.
+//
+// Validates every argument before anything reaches a kernel, turns a
+// gg_scene into per-scene device constants (camera basis, metric at the
+// camera, horizon, thresholds) with the host libm so those values match
+// what the reference computes per ray, stages outputs that live in host
+// memory, launches the persistent trace kernel and times it with hipEvents
+// on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gargantua.h"
+#include "gg_internal.hpp"
+
+using namespace gg;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return fail(GG_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// A device buffer owned by the context, grown on demand, never shrunk.
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return GG_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(e == hipErrorOutOfMemory ? GG_ERR_OUT_OF_MEMORY : GG_ERR_DEVICE, "hipMalloc(%zu): %s", bytes,
+                  hipGetErrorString(e));
+    }
+    cap = bytes;
+    return GG_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+}  // namespace
+
+struct gg_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  uint32_t num_cus = 256;
+  gg_launch launch{256, 4, 16};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevBuf scenes, outs, counters, states_in;
+  DevBuf stage[9];       // device staging for host-memory outputs
+  DevBuf rhs_io;
+  std::vector<DevBuf> batch_stage;
+};
+
+namespace {
+
+// ---- host restatements needed for per-scene constants ----
+
+// Kerr::metric_tensor / Schwarzschild::metric_tensor at the camera
+// (kerr.rs:61-94, schwarzschild.rs:26-43).
+void metric_host(int kind, double m, double a, double rs, const double pos[4], double &tt, double &rr,
+                 double &thth, double &phph, double &tph) {
+  double r = pos[1] >= 1e-10 ? pos[1] : 1e-10;
+  double theta = pos[2];
+  if (kind == GG_METRIC_KERR) {
+    double a2 = a * a;
+    double ct = std::cos(theta);
+    double sigma = r * r + a * a * ct * ct;
+    double delta = r * r - 2.0 * m * r + a * a;
+    double st = std::sin(theta);
+    double sin2 = st * st;
+    tt = -(1.0 - 2.0 * m * r / sigma);
+    rr = sigma / delta;
+    thth = sigma;
+    phph = (r * r + a2 + 2.0 * m * a2 * r * sin2 / sigma) * sin2;
+    tph = -2.0 * m * a * r * sin2 / sigma;
+  } else {
+    double f = 1.0 - rs / r;
+    double st = std::sin(theta);
+    tt = -f;
+    rr = 1.0 / f;
+    thth = r * r;
+    phph = r * r * st * st;
+    tph = 0.0;
+  }
+}
+
+void normalize3(double v[3]) {
+  double n = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  v[0] = v[0] / n; v[1] = v[1] / n; v[2] = v[2] / n;
+}
+void cross3(const double a[3], const double b[3], double r[3]) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+bool finite(double x) { return std::isfinite(x); }
+
+int validate_metric(const gg_metric &m) {
+  if (m.kind != GG_METRIC_SCHWARZSCHILD && m.kind != GG_METRIC_KERR)
+    return fail(GG_ERR_INVALID_ARG, "metric.kind %d is not Schwarzschild (0) or Kerr (1)", m.kind);
+  if (!(m.mass > 0.0) || !finite(m.mass)) return fail(GG_ERR_INVALID_ARG, "metric.mass must be positive and finite");
+  if (!finite(m.spin)) return fail(GG_ERR_INVALID_ARG, "metric.spin must be finite");
+  return GG_OK;
+}
+
+int validate_integrator(const gg_integrator &I) {
+  if (!(I.initial_step > 0.0) || !finite(I.initial_step)) return fail(GG_ERR_INVALID_ARG, "initial_step must be > 0");
+  if (!(I.min_step > 0.0) || !finite(I.min_step)) return fail(GG_ERR_INVALID_ARG, "min_step must be > 0");
+  if (!(I.max_step >= I.min_step) || !finite(I.max_step))
+    return fail(GG_ERR_INVALID_ARG, "max_step must be >= min_step (f64::clamp panics otherwise)");
+  if (I.err_mode == GG_ERR_REFERENCE) {
+    if (!(I.tolerance > 0.0) || !finite(I.tolerance)) return fail(GG_ERR_INVALID_ARG, "tolerance must be > 0");
+  } else if (I.err_mode == GG_ERR_RTOL_ATOL) {
+    if (!(I.rtol >= 0.0) || !(I.atol >= 0.0) || !(I.rtol + I.atol > 0.0) || !finite(I.rtol) || !finite(I.atol))
+      return fail(GG_ERR_INVALID_ARG, "rtol, atol must be >= 0 and not both 0");
+  } else {
+    return fail(GG_ERR_INVALID_ARG, "err_mode %d unknown", I.err_mode);
+  }
+  if (std::isnan(I.max_radius)) return fail(GG_ERR_INVALID_ARG, "max_radius is NaN");
+  if (!finite(I.horizon_buffer) || !finite(I.horizon_rel))
+    return fail(GG_ERR_INVALID_ARG, "horizon_buffer/horizon_rel must be finite");
+  return GG_OK;
+}
+
+int validate_disk(const gg_disk &d) {
+  if (d.kind < GG_DISK_NONE || d.kind > GG_DISK_THIN) return fail(GG_ERR_INVALID_ARG, "disk.kind %d unknown", d.kind);
+  if (d.kind == GG_DISK_NONE) return GG_OK;
+  if (!finite(d.r_inner) || !finite(d.r_outer)) return fail(GG_ERR_INVALID_ARG, "disk radii must be finite");
+  if (d.kind == GG_DISK_VOLUMETRIC) {
+    if (!(d.scale_height > 0.0)) return fail(GG_ERR_INVALID_ARG, "disk.scale_height must be > 0");
+    if (!finite(d.density_0) || !finite(d.absorption)) return fail(GG_ERR_INVALID_ARG, "disk density/absorption");
+  } else {
+    if (!finite(d.emissivity_q)) return fail(GG_ERR_INVALID_ARG, "disk.emissivity_q must be finite");
+    if (!(d.edge_width >= 0.0) || !finite(d.edge_width)) return fail(GG_ERR_INVALID_ARG, "disk.edge_width must be >= 0");
+  }
+  return GG_OK;
+}
+
+int validate_scene(const gg_scene *s, bool need_image) {
+  if (!s) return fail(GG_ERR_INVALID_ARG, "scene is NULL");
+  int rc;
+  if ((rc = validate_metric(s->metric))) return rc;
+  if ((rc = validate_integrator(s->integrator))) return rc;
+  if ((rc = validate_disk(s->disk))) return rc;
+  for (int i = 0; i < 4; ++i)
+    if (!finite(s->camera.position[i])) return fail(GG_ERR_INVALID_ARG, "camera.position must be finite");
+  if (need_image) {
+    const gg_image &im = s->image;
+    if (im.width == 0 || im.height == 0) return fail(GG_ERR_INVALID_ARG, "image width/height must be > 0");
+    if (im.samples_per_axis == 0) return fail(GG_ERR_INVALID_ARG, "samples_per_axis must be >= 1");
+    uint64_t sw = (uint64_t)im.width * im.samples_per_axis, sh = (uint64_t)im.height * im.samples_per_axis;
+    if (sw > 65536 || sh > 65536) return fail(GG_ERR_INVALID_ARG, "sub-pixel grid %llux%llu too large",
+                                              (unsigned long long)sw, (unsigned long long)sh);
+    if (!(s->camera.fov_y_radians > 0.0) || !(s->camera.fov_y_radians < M_PI))
+      return fail(GG_ERR_INVALID_ARG, "camera.fov_y_radians must be in (0, pi)");
+    if (!(s->camera.aspect > 0.0) || !finite(s->camera.aspect))
+      return fail(GG_ERR_INVALID_ARG, "camera.aspect must be > 0");
+    double f[3] = {s->camera.look_at[0], s->camera.look_at[1], s->camera.look_at[2]};
+    double u[3] = {s->camera.up[0], s->camera.up[1], s->camera.up[2]};
+    double c[3];
+    cross3(f, u, c);
+    if (!(c[0] * c[0] + c[1] * c[1] + c[2] * c[2] > 0.0))
+      return fail(GG_ERR_INVALID_ARG, "camera.look_at must be non-zero and not parallel to camera.up");
+  }
+  return GG_OK;
+}
+
+// Kerr::new clamp (kerr.rs:29-34).
+double clamp_spin(const gg_metric &m) {
+  if (m.kind != GG_METRIC_KERR) return 0.0;
+  double mx = m.mass * 0.9999;
+  double s = m.spin;
+  if (s < -mx) s = -mx;
+  if (s > mx) s = mx;
+  return s;
+}
+
+void build_dev_scene(const gg_scene &s, DevScene &D) {
+  std::memset(&D, 0, sizeof(D));
+  D.metric_kind = s.metric.kind;
+  D.disk_kind = s.disk.kind;
+  D.err_mode = s.integrator.err_mode;
+  D.max_steps = s.integrator.max_steps;
+  D.mass = s.metric.mass;
+  D.spin = clamp_spin(s.metric);
+  D.rs = 2.0 * s.metric.mass;
+  if (s.metric.kind == GG_METRIC_KERR) {
+    double disc = D.mass * D.mass - D.spin * D.spin;  // kerr.rs:427-436 (disc > 0 after the clamp)
+    D.r_h = D.mass + std::sqrt(disc);
+  } else {
+    D.r_h = D.rs;
+  }
+  D.r_capture = D.r_h * (1.0 + s.integrator.horizon_rel) + s.integrator.horizon_buffer;
+  D.max_radius = s.integrator.max_radius;
+  D.h0 = s.integrator.initial_step;
+  D.min_step = s.integrator.min_step;
+  D.max_step = s.integrator.max_step;
+  D.tol = s.integrator.tolerance;
+  D.rtol = s.integrator.rtol;
+  D.atol = s.integrator.atol;
+  // camera basis (camera.rs:21-26) and spherical frame at the camera (ray.rs:59-65)
+  const gg_camera &c = s.camera;
+  for (int i = 0; i < 4; ++i) D.cam_pos[i] = c.position[i];
+  double fwd[3] = {c.look_at[0], c.look_at[1], c.look_at[2]};
+  normalize3(fwd);
+  double right[3];
+  cross3(fwd, c.up, right);
+  normalize3(right);
+  double up[3];
+  cross3(right, fwd, up);
+  normalize3(up);
+  for (int i = 0; i < 3; ++i) { D.fwd[i] = fwd[i]; D.right[i] = right[i]; D.up[i] = up[i]; }
+  D.half_h = std::tan(c.fov_y_radians / 2.0);
+  D.half_w = D.half_h * c.aspect;
+  double th = c.position[2], ph = c.position[3];
+  double st = std::sin(th), ct = std::cos(th), sp = std::sin(ph), cp = std::cos(ph);
+  D.rhat[0] = st * cp; D.rhat[1] = st * sp; D.rhat[2] = ct;
+  D.thhat[0] = ct * cp; D.thhat[1] = ct * sp; D.thhat[2] = -st;
+  D.phhat[0] = -sp; D.phhat[1] = cp; D.phhat[2] = 0.0;
+  double tph;
+  metric_host(s.metric.kind, D.mass, D.spin, D.rs, c.position, D.g_tt, D.g_rr, D.g_thth, D.g_phph, tph);
+  D.sqrt_g_rr = std::sqrt(D.g_rr);
+  D.sqrt_g_thth = std::sqrt(D.g_thth);
+  D.sqrt_g_phph = std::sqrt(D.g_phph);
+  double g_tt_obs = -D.g_tt;                                 // tracer.rs:93-94
+  D.sqrt_neg_gtt_obs = std::sqrt(g_tt_obs >= 0.0 ? g_tt_obs : 0.0);
+  uint32_t spa = s.image.samples_per_axis ? s.image.samples_per_axis : 1;
+  D.sub_w = s.image.width * spa;
+  D.sub_h = s.image.height * spa;
+  D.jitter = s.image.jitter ? 1 : 0;
+  D.seed = s.image.seed;
+  D.tiled = (D.sub_w % 8 == 0 && D.sub_h % 8 == 0) ? 1 : 0;
+  D.r_in = s.disk.r_inner;
+  D.r_out = s.disk.r_outer;
+  D.scale_height = s.disk.scale_height;
+  D.density_0 = s.disk.density_0;
+  D.absorption = s.disk.absorption;
+  D.q = s.disk.emissivity_q;
+  D.edge_width = s.disk.edge_width;
+  D.fp32_min_r = 10.0 * D.mass;
+}
+
+bool is_device_ptr(const void *p) {
+  if (!p) return false;
+  hipPointerAttribute_t at;
+  hipError_t e = hipPointerGetAttributes(&at, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
+// Resolve one output pointer: device pointers are used in place, host
+// pointers get a staging buffer copied back after the kernel.
+struct OutPlan {
+  void *host = nullptr;   // non-null: copy back to here
+  void *dev = nullptr;
+  size_t bytes = 0;
+};
+
+int plan_out(DevBuf &stage, void *user, size_t bytes, OutPlan &p) {
+  p = OutPlan{};
+  if (!user) return GG_OK;
+  p.bytes = bytes;
+  if (is_device_ptr(user)) {
+    p.dev = user;
+    return GG_OK;
+  }
+  int rc = stage.ensure(bytes);
+  if (rc) return rc;
+  p.dev = stage.p;
+  p.host = user;
+  return GG_OK;
+}
+
+uint32_t grid_for(gg_context *ctx, uint64_t n_rays) {
+  uint64_t waves_needed = (n_rays + 63) / 64;
+  uint64_t per_block = ctx->launch.block_threads / 64;
+  uint64_t blocks_needed = (waves_needed + per_block - 1) / per_block;
+  uint64_t grid = (uint64_t)ctx->num_cus * ctx->launch.blocks_per_cu;
+  if (blocks_needed < grid) grid = blocks_needed;
+  if (grid < 1) grid = 1;
+  return (uint32_t)grid;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gg_strerror(int status) {
+  switch (status) {
+    case GG_OK: return "ok";
+    case GG_ERR_INVALID_ARG: return "invalid argument";
+    case GG_ERR_DEVICE: return "HIP runtime error";
+    case GG_ERR_NO_DEVICE: return "no HIP device";
+    case GG_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case GG_ERR_UNSUPPORTED: return "unsupported";
+    case GG_ERR_INTERNAL: return "internal error";
+    default: return "unknown status";
+  }
+}
+
+const char *gg_last_error(void) { return g_last_error.c_str(); }
+
+int gg_abi_version(void) { return GG_ABI_VERSION; }
+
+int gg_context_create(int device, void *stream, gg_context **out) {
+  if (!out) return fail(GG_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) return fail(GG_ERR_NO_DEVICE, "no HIP device (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= ndev) return fail(GG_ERR_INVALID_ARG, "device %d out of range [0, %d)", device, ndev);
+  HIP_TRY(hipSetDevice(device));
+  gg_context *c = new gg_context();
+  c->device = device;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    c->num_cus = (uint32_t)cus;
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return fail(GG_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    c->own_stream = true;
+  }
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return fail(GG_ERR_DEVICE, "hipEventCreate failed");
+  }
+  int rc = c->counters.ensure(sizeof(Counters));
+  if (rc) { delete c; return rc; }
+  *out = c;
+  return GG_OK;
+}
+
+int gg_context_destroy(gg_context *ctx) {
+  if (!ctx) return GG_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->scenes.release(); ctx->outs.release(); ctx->counters.release(); ctx->states_in.release();
+  ctx->rhs_io.release();
+  for (auto &b : ctx->stage) b.release();
+  for (auto &b : ctx->batch_stage) b.release();
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return GG_OK;
+}
+
+int gg_set_launch(gg_context *ctx, const gg_launch *l) {
+  if (!ctx || !l) return fail(GG_ERR_INVALID_ARG, "NULL argument");
+  gg_launch v = *l;
+  if (v.block_threads == 0) v.block_threads = 256;
+  if (v.blocks_per_cu == 0) v.blocks_per_cu = 4;
+  if (v.refill_interval == 0) v.refill_interval = 16;
+  if (v.block_threads % 64 || v.block_threads > 256)
+    return fail(GG_ERR_INVALID_ARG, "block_threads must be a multiple of 64 in [64, 256]");
+  if (v.blocks_per_cu > 64) return fail(GG_ERR_INVALID_ARG, "blocks_per_cu must be <= 64");
+  ctx->launch = v;
+  return GG_OK;
+}
+
+int gg_get_launch(gg_context *ctx, gg_launch *l) {
+  if (!ctx || !l) return fail(GG_ERR_INVALID_ARG, "NULL argument");
+  *l = ctx->launch;
+  return GG_OK;
+}
+
+int gg_scene_rays(const gg_scene *scene, uint64_t *n_rays) {
+  int rc = validate_scene(scene, true);
+  if (rc) return rc;
+  uint64_t spa = scene->image.samples_per_axis;
+  if (n_rays) *n_rays = (uint64_t)scene->image.width * spa * scene->image.height * spa;
+  return GG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Common body of gg_render / gg_render_batch / gg_trace_states.
+int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int precision, const gg_outputs *outs,
+                const double *states_host, uint64_t n_states, gg_stats *stats) {
+  auto t_start = std::chrono::steady_clock::now();
+  if (!ctx) return fail(GG_ERR_INVALID_ARG, "context is NULL");
+  if (!scenes || n_scenes == 0) return fail(GG_ERR_INVALID_ARG, "no scenes");
+  if (!outs) return fail(GG_ERR_INVALID_ARG, "outputs are NULL");
+  if (precision != GG_PREC_FP64 && precision != GG_PREC_MIXED)
+    return fail(GG_ERR_INVALID_ARG, "precision %d unknown", precision);
+  const bool from_states = (states_host != nullptr);
+  int rc;
+  for (uint32_t i = 0; i < n_scenes; ++i)
+    if ((rc = validate_scene(&scenes[i], !from_states))) return rc;
+  for (uint32_t i = 1; i < n_scenes; ++i)
+    if (scenes[i].metric.kind != scenes[0].metric.kind || scenes[i].disk.kind != scenes[0].disk.kind)
+      return fail(GG_ERR_UNSUPPORTED, "a batch must share metric kind and disk kind (scene %u differs)", i);
+  HIP_TRY(hipSetDevice(ctx->device));
+
+  std::vector<DevScene> ds(n_scenes);
+  std::vector<DevOut> dout(n_scenes);
+  std::vector<std::vector<OutPlan>> plans(n_scenes);
+  uint64_t total = 0;
+  if (ctx->batch_stage.size() < 9ull * n_scenes) ctx->batch_stage.resize(9ull * n_scenes);
+  for (uint32_t i = 0; i < n_scenes; ++i) {
+    build_dev_scene(scenes[i], ds[i]);
+    uint64_t n = from_states ? n_states : (uint64_t)ds[i].sub_w * ds[i].sub_h;
+    ds[i].ray_begin = total;
+    ds[i].out_index = i;
+    total += n;
+    const gg_outputs &o = outs[i];
+    plans[i].resize(9);
+    DevBuf *st = n_scenes == 1 ? ctx->stage : &ctx->batch_stage[9ull * i];
+    bool need_rgb = (o.rgb != nullptr) && !from_states;
+    // rgb shading reads cls, intensity and the velocity planes: stage them
+    // internally when the caller did not ask for them.
+    void *cls_p = o.cls, *int_p = o.intensity, *state_p = o.state;
+    if ((rc = plan_out(st[0], cls_p, n, plans[i][0]))) return rc;
+    if ((rc = plan_out(st[1], o.r, 8 * n, plans[i][1]))) return rc;
+    if ((rc = plan_out(st[2], o.theta, 8 * n, plans[i][2]))) return rc;
+    if ((rc = plan_out(st[3], o.phi, 8 * n, plans[i][3]))) return rc;
+    if ((rc = plan_out(st[4], int_p, 8 * n, plans[i][4]))) return rc;
+    if ((rc = plan_out(st[5], o.attempts, 4 * n, plans[i][5]))) return rc;
+    if ((rc = plan_out(st[6], o.accepted, 4 * n, plans[i][6]))) return rc;
+    if ((rc = plan_out(st[7], state_p, 72 * n, plans[i][7]))) return rc;
+    if (need_rgb) {
+      if (!plans[i][0].dev) { if ((rc = st[0].ensure(n))) return rc; plans[i][0].dev = st[0].p; }
+      if (!plans[i][4].dev) { if ((rc = st[4].ensure(8 * n))) return rc; plans[i][4].dev = st[4].p; }
+      if (!plans[i][7].dev) { if ((rc = st[7].ensure(72 * n))) return rc; plans[i][7].dev = st[7].p; }
+      size_t rgb_bytes = 3ull * scenes[i].image.width * scenes[i].image.height;
+      if ((rc = plan_out(st[8], o.rgb, rgb_bytes, plans[i][8]))) return rc;
+    }
+    DevOut &d = dout[i];
+    d.cls = (uint8_t *)plans[i][0].dev;
+    d.r = (double *)plans[i][1].dev;
+    d.theta = (double *)plans[i][2].dev;
+    d.phi = (double *)plans[i][3].dev;
+    d.intensity = (double *)plans[i][4].dev;
+    d.attempts = (uint32_t *)plans[i][5].dev;
+    d.accepted = (uint32_t *)plans[i][6].dev;
+    d.state = (double *)plans[i][7].dev;
+    d.n_rays = n;
+  }
+  if ((rc = ctx->scenes.ensure(sizeof(DevScene) * n_scenes))) return rc;
+  if ((rc = ctx->outs.ensure(sizeof(DevOut) * n_scenes))) return rc;
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(ctx->scenes.p, ds.data(), sizeof(DevScene) * n_scenes, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(ctx->outs.p, dout.data(), sizeof(DevOut) * n_scenes, hipMemcpyHostToDevice, s));
+  const double *states_dev = nullptr;
+  if (from_states) {
+    if (is_device_ptr(states_host)) {
+      states_dev = states_host;
+    } else {
+      if ((rc = ctx->states_in.ensure(72 * n_states))) return rc;
+      HIP_TRY(hipMemcpyAsync(ctx->states_in.p, states_host, 72 * n_states, hipMemcpyHostToDevice, s));
+      states_dev = (const double *)ctx->states_in.p;
+    }
+  }
+
+  TraceLaunch T;
+  T.scenes = (const DevScene *)ctx->scenes.p;
+  T.outs = (const DevOut *)ctx->outs.p;
+  T.n_scenes = n_scenes;
+  T.n_rays = total;
+  T.states_in = states_dev;
+  T.counters = (Counters *)ctx->counters.p;
+  T.refill_interval = ctx->launch.refill_interval;
+  uint32_t grid = grid_for(ctx, total);
+  uint32_t block = ctx->launch.block_threads;
+  HIP_TRY(launch_reset(T.counters, s));
+  HIP_TRY(hipEventRecord(ctx->ev0, s));
+  if (total > 0) HIP_TRY(launch_trace(scenes[0].metric.kind, scenes[0].disk.kind, precision, T, grid, block, s));
+  HIP_TRY(hipEventRecord(ctx->ev1, s));
+  for (uint32_t i = 0; i < n_scenes; ++i) {
+    if (plans[i][8].dev) {
+      const DevOut *dop = (const DevOut *)ctx->outs.p + i;
+      const DevScene *dsp = (const DevScene *)ctx->scenes.p + i;
+      HIP_TRY(launch_shade(dsp, dop, scenes[i].image.width, scenes[i].image.height,
+                           scenes[i].image.samples_per_axis, (uint8_t *)plans[i][8].dev, s));
+    }
+  }
+  Counters cnt;
+  HIP_TRY(hipMemcpyAsync(&cnt, T.counters, sizeof(Counters), hipMemcpyDeviceToHost, s));
+  for (uint32_t i = 0; i < n_scenes; ++i)
+    for (auto &p : plans[i])
+      if (p.host && p.dev) HIP_TRY(hipMemcpyAsync(p.host, p.dev, p.bytes, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (stats) {
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    stats->kernel_ms = ms;
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    stats->rays = total;
+    stats->attempts = cnt.attempts;
+    stats->rhs_evals = cnt.rhs;
+    stats->rhs_evals_fp32 = cnt.rhs32;
+    stats->attempts_full = cnt.attempts_full;
+    stats->grid_blocks = grid;
+    stats->block_threads = block;
+  }
+  return GG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gg_render(gg_context *ctx, const gg_scene *scene, int precision, const gg_outputs *out, gg_stats *stats) {
+  return render_impl(ctx, scene, 1, precision, out, nullptr, 0, stats);
+}
+
+int gg_render_batch(gg_context *ctx, const gg_scene *scenes, uint32_t n, int precision, const gg_outputs *out,
+                    gg_stats *stats) {
+  return render_impl(ctx, scenes, n, precision, out, nullptr, 0, stats);
+}
+
+int gg_trace_states(gg_context *ctx, const gg_scene *scene, int precision, uint64_t n, const double *states_in,
+                    const gg_outputs *out, gg_stats *stats) {
+  if (!states_in && n > 0) return fail(GG_ERR_INVALID_ARG, "states_in is NULL");
+  if (n == 0) return GG_OK;
+  return render_impl(ctx, scene, 1, precision, out, states_in, n, stats);
+}
+
+int gg_geodesic_acceleration(gg_context *ctx, const gg_metric *metric, int precision, uint64_t n, const double *pos,
+                             const double *vel, double *acc) {
+  if (!ctx || !metric) return fail(GG_ERR_INVALID_ARG, "NULL argument");
+  if (n == 0) return GG_OK;
+  if (!pos || !vel || !acc) return fail(GG_ERR_INVALID_ARG, "NULL pos/vel/acc");
+  if (precision != GG_PREC_FP64 && precision != GG_PREC_MIXED)
+    return fail(GG_ERR_INVALID_ARG, "precision %d unknown", precision);
+  int rc = validate_metric(*metric);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  gg_scene sc;
+  std::memset(&sc, 0, sizeof(sc));
+  sc.metric = *metric;
+  sc.camera.position[1] = 10.0 * metric->mass;
+  sc.camera.position[2] = M_PI / 2.0;
+  sc.camera.look_at[0] = -1.0;
+  sc.camera.up[2] = 1.0;
+  sc.camera.fov_y_radians = 1.0;
+  sc.camera.aspect = 1.0;
+  sc.image.width = sc.image.height = sc.image.samples_per_axis = 1;
+  DevScene D;
+  build_dev_scene(sc, D);
+  size_t bytes = 32 * n;
+  if ((rc = ctx->rhs_io.ensure(3 * bytes + sizeof(DevScene) + 256))) return rc;
+  char *base = (char *)ctx->rhs_io.p;
+  DevScene *dS = (DevScene *)base;
+  double *dp = (double *)(base + 256 * ((sizeof(DevScene) + 255) / 256));
+  double *dv = dp + 4 * n, *da = dv + 4 * n;
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(dS, &D, sizeof(D), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dp, pos, bytes, hipMemcpyDefault, s));
+  HIP_TRY(hipMemcpyAsync(dv, vel, bytes, hipMemcpyDefault, s));
+  HIP_TRY(launch_rhs(metric->kind, precision, dS, n, dp, dv, da, s));
+  HIP_TRY(hipMemcpyAsync(acc, da, bytes, hipMemcpyDefault, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return GG_OK;
+}
+
+int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, uint32_t checkpoint_every,
+            double *loss, double grad[4], double *intensity, gg_stats *stats) {
+  (void)ctx; (void)scene; (void)target; (void)checkpoint_every; (void)loss; (void)grad; (void)intensity;
+  (void)stats;
+  return fail(GG_ERR_UNSUPPORTED, "gg_grad is not built yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,416 @@
+// gg_physics.hpp -- device-side geodesic physics for gfx950.
+//
+// Two right-hand sides per metric:
+//   rhs64_*  FP64, no FMA contraction, each Christoffel symbol with the
+//            expression tree of the reference and the contraction applied in
+//            the reference's dense mu, alpha, beta order with its exact-zero
+//            terms skipped (skipping x - 0*v*v changes no bit).  This is the
+//            parity path: its results differ from the reference arithmetic
+//            only where the device sin/cos/pow round differently from libm.
+//   rhs32_*  FP32, contracted, three reciprocals instead of fifteen
+//            divisions, symmetric pairs merged.  Used by the mixed kernel on
+//            attempts where the step-size controller is saturated (see
+//            gg_trace.hpp), where its rounding cannot change a decision.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gargantua.h"
+
+namespace gg {
+
+struct d4 { double x, y, z, w; };
+struct f4 { float x, y, z, w; };
+
+// Per-scene constants, computed once on the host (with the host libm, so
+// that tan/sin/cos of the camera are the values the reference computes).
+struct DevScene {
+  // metric
+  int32_t metric_kind;
+  int32_t disk_kind;
+  int32_t err_mode;
+  uint32_t max_steps;
+  double mass, spin, rs, r_h;
+  double r_capture;      // r_h * (1 + horizon_rel) + horizon_buffer
+  double max_radius;
+  // integrator
+  double h0, min_step, max_step, tol, rtol, atol;
+  // camera
+  double cam_pos[4];
+  double fwd[3], right[3], up[3];
+  double half_w, half_h;
+  double rhat[3], thhat[3], phhat[3];
+  double g_tt, g_rr, g_thth, g_phph;            // metric at the camera
+  double sqrt_g_rr, sqrt_g_thth, sqrt_g_phph;
+  double sqrt_neg_gtt_obs;                      // tracer.rs:93-94
+  // sub-pixel grid
+  uint32_t sub_w, sub_h;
+  uint32_t jitter, tiled;                        // tiled: 8x8 ray tiles
+  uint64_t seed;
+  // disk
+  double r_in, r_out, scale_height, density_0, absorption, q, edge_width;
+  // mixed precision policy
+  double fp32_min_r;
+  // batch bookkeeping
+  uint64_t ray_begin;     // first global ray index of this scene
+  uint64_t out_index;     // which output set (batch)
+};
+
+// ---------------------------------------------------------------- FP64 ----
+
+__device__ inline double rmax1e10(double r) { return r >= 1e-10 ? r : 1e-10; }  // r.max(1e-10)
+
+// Kerr::metric_tensor (kerr.rs:61-94), Schwarzschild (schwarzschild.rs:26-43).
+struct Gmn { double tt, rr, thth, phph, tph; };
+
+template <int METRIC>
+__device__ inline Gmn metric64(const DevScene &S, double r_in, double theta) {
+#pragma clang fp contract(off)
+  Gmn g;
+  double r = rmax1e10(r_in);
+  if (METRIC == 1) {
+    double m = S.mass, a = S.spin;
+    double a2 = a * a;
+    double ct = cos(theta);
+    double sigma = r * r + a * a * ct * ct;
+    double delta = r * r - 2.0 * m * r + a * a;
+    double st = sin(theta);
+    double sin2 = st * st;
+    g.tt = -(1.0 - 2.0 * m * r / sigma);
+    g.rr = sigma / delta;
+    g.thth = sigma;
+    g.phph = (r * r + a2 + 2.0 * m * a2 * r * sin2 / sigma) * sin2;
+    g.tph = -2.0 * m * a * r * sin2 / sigma;
+  } else {
+    double f = 1.0 - S.rs / r;
+    double st = sin(theta);
+    g.tt = -f;
+    g.rr = 1.0 / f;
+    g.thth = r * r;
+    g.phph = r * r * st * st;
+    g.tph = 0.0;
+  }
+  return g;
+}
+
+// Kerr RHS: Christoffels of kerr.rs:100-424, contracted as metric.rs:37-52.
+__device__ inline d4 rhs64_kerr(const DevScene &S, double r_in, double theta, d4 v) {
+#pragma clang fp contract(off)
+  double r = rmax1e10(r_in);
+  const double m = S.mass, a = S.spin;
+  double a2 = a * a;
+  double cs = cos(theta);
+  double sigma = r * r + a * a * cs * cs;
+  double delta = r * r - 2.0 * m * r + a * a;
+  double st = sin(theta);
+  double ct = cs;  // cos(theta) evaluated twice in the reference; same value
+  double sin2 = st * st;
+  double sigma2 = sigma * sigma;
+  double r2 = r * r;
+  double dsigma_dr = 2.0 * r;
+  double dsigma_dtheta = -2.0 * a2 * st * ct;
+  double ddelta_dr = 2.0 * r - 2.0 * m;
+  double r2pa2 = r2 + a2;
+  double big_a = r2pa2 * r2pa2 - a2 * delta * sin2;
+  double g_inv_tt = -big_a / (sigma * delta);
+  double g_inv_tphi = -2.0 * m * a * r / (sigma * delta);
+  double g_inv_rr = delta / sigma;
+  double g_inv_thth = 1.0 / sigma;
+  double g_inv_phph = (delta - a2 * sin2) / (sigma * delta * sin2);
+  double dg_tt_dr = 2.0 * m / sigma - 2.0 * m * r * dsigma_dr / sigma2;
+  double dg_tt_dth = -2.0 * m * r * dsigma_dtheta / sigma2;
+  double dg_rr_dr = (dsigma_dr * delta - sigma * ddelta_dr) / (delta * delta);
+  double dg_rr_dth = dsigma_dtheta / delta;
+  double dg_thth_dr = dsigma_dr;
+  double dg_thth_dth = dsigma_dtheta;
+  double big_p = 2.0 * m * a2 * r;
+  double dg_phph_dr = 2.0 * r * sin2 + sin2 * sin2 * 2.0 * m * a2 * (sigma - 2.0 * r2) / sigma2;
+  double dg_phph_dth = 2.0 * r2pa2 * st * ct +
+                       big_p * (4.0 * st * st * st * ct * sigma - sin2 * sin2 * dsigma_dtheta) / sigma2;
+  double dg_tphi_dr = -2.0 * m * a * sin2 * (sigma - r * dsigma_dr) / sigma2;
+  double dg_tphi_dth = -2.0 * m * a * r * (2.0 * st * ct * sigma - sin2 * dsigma_dtheta) / sigma2;
+
+  double G001 = 0.5 * (g_inv_tt * dg_tt_dr + g_inv_tphi * dg_tphi_dr);
+  double G002 = 0.5 * (g_inv_tt * dg_tt_dth + g_inv_tphi * dg_tphi_dth);
+  double G013 = 0.5 * (g_inv_tt * dg_tphi_dr + g_inv_tphi * dg_phph_dr);
+  double G023 = 0.5 * (g_inv_tt * dg_tphi_dth + g_inv_tphi * dg_phph_dth);
+  double G100 = -0.5 * g_inv_rr * dg_tt_dr;
+  double G111 = 0.5 * g_inv_rr * dg_rr_dr;
+  double G122 = -0.5 * g_inv_rr * dg_thth_dr;
+  double G133 = -0.5 * g_inv_rr * dg_phph_dr;
+  double G112 = 0.5 * g_inv_rr * dg_rr_dth;
+  double G103 = -0.5 * g_inv_rr * dg_tphi_dr;
+  double G200 = -0.5 * g_inv_thth * dg_tt_dth;
+  double G211 = -0.5 * g_inv_thth * dg_rr_dth;
+  double G222 = 0.5 * g_inv_thth * dg_thth_dth;
+  double G233 = -0.5 * g_inv_thth * dg_phph_dth;
+  double G212 = 0.5 * g_inv_thth * dg_thth_dr;
+  double G203 = -0.5 * g_inv_thth * dg_tphi_dth;
+  double G301 = 0.5 * (g_inv_tphi * dg_tt_dr + g_inv_phph * dg_tphi_dr);
+  double G302 = 0.5 * (g_inv_tphi * dg_tt_dth + g_inv_phph * dg_tphi_dth);
+  double G313 = 0.5 * (g_inv_tphi * dg_tphi_dr + g_inv_phph * dg_phph_dr);
+  double G323 = 0.5 * (g_inv_tphi * dg_tphi_dth + g_inv_phph * dg_phph_dth);
+
+  const double v0 = v.x, v1 = v.y, v2 = v.z, v3 = v.w;
+  d4 acc;
+  double t;
+  // mu = t: (a,b) = 01 02 10 13 20 23 31 32
+  t = 0.0;
+  t = t - G001 * v0 * v1; t = t - G002 * v0 * v2; t = t - G001 * v1 * v0; t = t - G013 * v1 * v3;
+  t = t - G002 * v2 * v0; t = t - G023 * v2 * v3; t = t - G013 * v3 * v1; t = t - G023 * v3 * v2;
+  acc.x = t;
+  // mu = r: 00 03 11 12 21 22 30 33
+  t = 0.0;
+  t = t - G100 * v0 * v0; t = t - G103 * v0 * v3; t = t - G111 * v1 * v1; t = t - G112 * v1 * v2;
+  t = t - G112 * v2 * v1; t = t - G122 * v2 * v2; t = t - G103 * v3 * v0; t = t - G133 * v3 * v3;
+  acc.y = t;
+  // mu = theta: 00 03 11 12 21 22 30 33
+  t = 0.0;
+  t = t - G200 * v0 * v0; t = t - G203 * v0 * v3; t = t - G211 * v1 * v1; t = t - G212 * v1 * v2;
+  t = t - G212 * v2 * v1; t = t - G222 * v2 * v2; t = t - G203 * v3 * v0; t = t - G233 * v3 * v3;
+  acc.z = t;
+  // mu = phi: 01 02 10 13 20 23 31 32
+  t = 0.0;
+  t = t - G301 * v0 * v1; t = t - G302 * v0 * v2; t = t - G301 * v1 * v0; t = t - G313 * v1 * v3;
+  t = t - G302 * v2 * v0; t = t - G323 * v2 * v3; t = t - G313 * v3 * v1; t = t - G323 * v3 * v2;
+  acc.w = t;
+  return acc;
+}
+
+// Schwarzschild RHS override (schwarzschild.rs:99-140).
+__device__ inline d4 rhs64_schw(const DevScene &S, double r_in, double theta, d4 v) {
+#pragma clang fp contract(off)
+  double r = rmax1e10(r_in);
+  double st = sin(theta);
+  double ct = cos(theta);
+  const double rs = S.rs;
+  double f_denom = r * (r - rs);
+  double rmrs = r - rs;
+  double inv_r = 1.0 / r;
+  double g_t_tr = rs / (2.0 * f_denom);
+  double g_r_tt = rs * rmrs / (2.0 * r * r * r);
+  double g_r_rr = -rs / (2.0 * f_denom);
+  double g_r_thth = -rmrs;
+  double g_r_phph = -rmrs * st * st;
+  double g_th_phph = -st * ct;
+  double g_ph_thph = ct / st;
+  const double vt = v.x, vr = v.y, vth = v.z, vph = v.w;
+  d4 acc;
+  acc.x = -2.0 * g_t_tr * vt * vr;
+  acc.y = -g_r_tt * vt * vt - g_r_rr * vr * vr - g_r_thth * vth * vth - g_r_phph * vph * vph;
+  acc.z = -2.0 * inv_r * vr * vth - g_th_phph * vph * vph;
+  acc.w = -2.0 * inv_r * vr * vph - 2.0 * g_ph_thph * vth * vph;
+  return acc;
+}
+
+template <int METRIC>
+__device__ inline d4 rhs64(const DevScene &S, double r, double theta, d4 v) {
+  if (METRIC == 1) return rhs64_kerr(S, r, theta, v);
+  return rhs64_schw(S, r, theta, v);
+}
+
+// ---------------------------------------------------------------- FP32 ----
+
+__device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Kerr geodesic acceleration in FP32: the same Christoffel symbols with the
+// divisions folded into 1/Sigma, 1/Delta, 1/sin^2 and the symmetric pairs of
+// the contraction merged.
+__device__ inline f4 rhs32_kerr(float m, float a, float r, float theta, f4 v) {
+#pragma clang fp contract(fast)
+  float st, ct;
+  __sincosf(theta, &st, &ct);
+  float a2 = a * a;
+  float r2 = r * r;
+  float sin2 = st * st;
+  float sc = st * ct;
+  float sigma = r2 + a2 * ct * ct;
+  float delta = r2 - 2.0f * m * r + a2;
+  float is = frcp(sigma);
+  float id = frcp(delta);
+  float is2 = is * is;
+  float isd = is * id;
+  float dsig_th = -2.0f * a2 * sc;
+  float r2pa2 = r2 + a2;
+  float big_a = r2pa2 * r2pa2 - a2 * delta * sin2;
+  float twom = 2.0f * m;
+  float ginv_tt = -big_a * isd;
+  float ginv_tph = -twom * a * r * isd;
+  float ginv_rr = delta * is;
+  float ginv_thth = is;
+  float ginv_phph = (delta - a2 * sin2) * isd * frcp(sin2);
+  float sig_m2r2 = sigma - 2.0f * r2;
+  float dtt_dr = twom * sig_m2r2 * is2;
+  float dtt_dth = -twom * r * dsig_th * is2;
+  float drr_dr = (2.0f * r * delta - sigma * (2.0f * r - twom)) * id * id;
+  float drr_dth = dsig_th * id;
+  float dthth_dr = 2.0f * r;
+  float dthth_dth = dsig_th;
+  float big_p = twom * a2 * r;
+  float dpp_dr = 2.0f * r * sin2 + sin2 * sin2 * twom * a2 * sig_m2r2 * is2;
+  float dpp_dth = 2.0f * r2pa2 * sc + big_p * (4.0f * sin2 * sc * sigma - sin2 * sin2 * dsig_th) * is2;
+  float dtp_dr = -twom * a * sin2 * sig_m2r2 * is2;
+  float dtp_dth = -twom * a * r * (2.0f * sc * sigma - sin2 * dsig_th) * is2;
+
+  float G001 = 0.5f * (ginv_tt * dtt_dr + ginv_tph * dtp_dr);
+  float G002 = 0.5f * (ginv_tt * dtt_dth + ginv_tph * dtp_dth);
+  float G013 = 0.5f * (ginv_tt * dtp_dr + ginv_tph * dpp_dr);
+  float G023 = 0.5f * (ginv_tt * dtp_dth + ginv_tph * dpp_dth);
+  float hrr = 0.5f * ginv_rr;
+  float G100 = -hrr * dtt_dr, G111 = hrr * drr_dr, G122 = -hrr * dthth_dr, G133 = -hrr * dpp_dr;
+  float G112 = hrr * drr_dth, G103 = -hrr * dtp_dr;
+  float hth = 0.5f * ginv_thth;
+  float G200 = -hth * dtt_dth, G211 = -hth * drr_dth, G222 = hth * dthth_dth, G233 = -hth * dpp_dth;
+  float G212 = hth * dthth_dr, G203 = -hth * dtp_dth;
+  float G301 = 0.5f * (ginv_tph * dtt_dr + ginv_phph * dtp_dr);
+  float G302 = 0.5f * (ginv_tph * dtt_dth + ginv_phph * dtp_dth);
+  float G313 = 0.5f * (ginv_tph * dtp_dr + ginv_phph * dpp_dr);
+  float G323 = 0.5f * (ginv_tph * dtp_dth + ginv_phph * dpp_dth);
+
+  const float v0 = v.x, v1 = v.y, v2 = v.z, v3 = v.w;
+  f4 acc;
+  acc.x = -2.0f * (G001 * v0 * v1 + G002 * v0 * v2 + G013 * v1 * v3 + G023 * v2 * v3);
+  acc.y = -(G100 * v0 * v0 + G111 * v1 * v1 + G122 * v2 * v2 + G133 * v3 * v3 +
+            2.0f * (G112 * v1 * v2 + G103 * v0 * v3));
+  acc.z = -(G200 * v0 * v0 + G211 * v1 * v1 + G222 * v2 * v2 + G233 * v3 * v3 +
+            2.0f * (G212 * v1 * v2 + G203 * v0 * v3));
+  acc.w = -2.0f * (G301 * v0 * v1 + G302 * v0 * v2 + G313 * v1 * v3 + G323 * v2 * v3);
+  return acc;
+}
+
+__device__ inline f4 rhs32_schw(float rs, float r, float theta, f4 v) {
+#pragma clang fp contract(fast)
+  float st, ct;
+  __sincosf(theta, &st, &ct);
+  float ir = frcp(r);
+  float rmrs = r - rs;
+  float ifd = frcp(r * rmrs);
+  float g_t_tr = 0.5f * rs * ifd;
+  float g_r_tt = 0.5f * rs * rmrs * ir * ir * ir;
+  float g_r_rr = -g_t_tr;
+  f4 acc;
+  acc.x = -2.0f * g_t_tr * v.x * v.y;
+  acc.y = -g_r_tt * v.x * v.x - g_r_rr * v.y * v.y + rmrs * (v.z * v.z + st * st * v.w * v.w);
+  acc.z = -2.0f * ir * v.y * v.z + st * ct * v.w * v.w;
+  acc.w = -2.0f * ir * v.y * v.w - 2.0f * (ct * frcp(st)) * v.z * v.w;
+  return acc;
+}
+
+template <int METRIC>
+__device__ inline f4 rhs32(const DevScene &S, float r, float theta, f4 v) {
+  if (METRIC == 1) return rhs32_kerr((float)S.mass, (float)S.spin, r, theta, v);
+  return rhs32_schw((float)S.rs, r, theta, v);
+}
+
+// ------------------------------------------------------ redshift & disk ----
+
+// orbital_four_velocity + circular_orbit_velocity (kerr.rs:439-449,
+// schwarzschild.rs:146-153, metric.rs:82-94), then redshift_factor
+// (tracer.rs:158-183).
+template <int METRIC>
+__device__ inline double redshift64(const DevScene &S, double r, double theta, d4 p) {
+#pragma clang fp contract(off)
+  Gmn g = metric64<METRIC>(S, r, theta);
+  double p_t = g.tt * p.x + g.tph * p.w;
+  double e_obs = -p_t / S.sqrt_neg_gtt_obs;
+  e_obs = e_obs >= 0.0 ? e_obs : 0.0;
+  bool have_orbit = false;
+  double omega = 0.0;
+  const double m = S.mass;
+  if (METRIC == 1) {
+    if (r > 0.0) {
+      double sqrt_m = sqrt(m);
+      omega = sqrt_m / (pow(r, 1.5) + S.spin * sqrt_m);
+      have_orbit = true;
+    }
+  } else {
+    if (r > 3.0 * m) {
+      omega = sqrt(m / (r * r * r));
+      have_orbit = true;
+    }
+  }
+  double e_emit = 0.0;
+  bool orbit_ok = false;
+  if (have_orbit) {
+    // circular_orbit_velocity re-evaluates the metric at the same point.
+    double denom = -(g.tt + 2.0 * g.tph * omega + g.phph * omega * omega);
+    if (denom > 0.0 && isfinite(denom)) {
+      double u_t = 1.0 / sqrt(denom);
+      double u_ph = omega * u_t;
+      double p_phi = g.tph * p.x + g.phph * p.w;
+      e_emit = -(p_t * u_t + p_phi * u_ph);
+      e_emit = e_emit >= 0.0 ? e_emit : 0.0;
+      orbit_ok = true;
+    }
+  }
+  if (!orbit_ok) {
+    double g_tt_emit = -g.tt;
+    e_emit = g_tt_emit > 0.0 ? -p_t / sqrt(g_tt_emit) : 0.0;
+    e_emit = e_emit >= 0.0 ? e_emit : 0.0;
+  }
+  return e_emit > 0.0 ? e_obs / e_emit : 0.0;
+}
+
+// AccretionDisk::density (disk.rs:154-173).
+__device__ inline double disk_density(const DevScene &S, double r, double theta) {
+#pragma clang fp contract(off)
+  double st = sin(theta), ct = cos(theta);
+  double cap_r = r * fabs(st);
+  double z = r * ct;
+  if (cap_r < S.r_in || cap_r > S.r_out) return 0.0;
+  double t = S.r_in / cap_r;
+  double radial = t * t;
+  double vertical = exp(-z * z / (2.0 * S.scale_height * S.scale_height));
+  return S.density_0 * radial * vertical;
+}
+
+// AccretionDisk::emission given the density just computed (disk.rs:179-195).
+__device__ inline double disk_emission(const DevScene &S, double rho, double r, double theta) {
+#pragma clang fp contract(off)
+  if (rho == 0.0) return 0.0;
+  double cap_r = r * fabs(sin(theta));
+  if (cap_r < S.r_in) return 0.0;
+  double temp_proxy = pow(S.r_in / cap_r, 0.75);
+  return rho * rho * temp_proxy;
+}
+
+// SplitMix64 (Vigna): n-th output (n >= 1) for `seed`.
+__device__ inline uint64_t splitmix64(uint64_t seed, uint64_t n) {
+  uint64_t z = seed + n * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Camera::pixel_direction + GeodesicRay::from_camera (camera.rs:20-29,
+// ray.rs:44-91) with the per-scene basis hoisted to the host.
+__device__ inline void camera_ray(const DevScene &S, uint32_t px, uint32_t py, double pos[4], d4 &vel) {
+#pragma clang fp contract(off)
+  double fx = (double)px + 0.5, fy = (double)py + 0.5;
+  if (S.jitter) {
+    uint64_t k = (uint64_t)py * S.sub_w + px;
+    fx = fx + ((double)(splitmix64(S.seed, 2 * k + 1) >> 11) * (1.0 / 9007199254740992.0) - 0.5);
+    fy = fy + ((double)(splitmix64(S.seed, 2 * k + 2) >> 11) * (1.0 / 9007199254740992.0) - 0.5);
+  }
+  double u = fx / (double)S.sub_w * 2.0 - 1.0;
+  double v = 1.0 - fy / (double)S.sub_h * 2.0;
+  double uw = u * S.half_w, vh = v * S.half_h;
+  double d0 = S.fwd[0] + uw * S.right[0] + vh * S.up[0];
+  double d1 = S.fwd[1] + uw * S.right[1] + vh * S.up[1];
+  double d2 = S.fwd[2] + uw * S.right[2] + vh * S.up[2];
+  double n = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  d0 = d0 / n; d1 = d1 / n; d2 = d2 / n;
+  double n_r = d0 * S.rhat[0] + d1 * S.rhat[1] + d2 * S.rhat[2];
+  double n_t = d0 * S.thhat[0] + d1 * S.thhat[1] + d2 * S.thhat[2];
+  double n_p = d0 * S.phhat[0] + d1 * S.phhat[1] + d2 * S.phhat[2];
+  double kr = n_r / S.sqrt_g_rr;
+  double kth = n_t / S.sqrt_g_thth;
+  double kph = n_p / S.sqrt_g_phph;
+  double spatial = S.g_rr * kr * kr + S.g_thth * kth * kth + S.g_phph * kph * kph;
+  double kt = sqrt(-spatial / S.g_tt);
+  for (int i = 0; i < 4; ++i) pos[i] = S.cam_pos[i];
+  vel.x = kt; vel.y = kr; vel.z = kth; vel.w = kph;
+}
+
+}  // namespace gg

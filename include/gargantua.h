@@ -1,0 +1,253 @@
+/*
+ * gargantua.h -- C ABI of the MI355X geodesic renderer (libgargantua.so).
+ *
+ * The boundary replaces the per-pixel null-geodesic hot path of the reference
+ * renderer (Rust crates gr-core + raytracer::curved).  Every struct is plain
+ * old data; every pointer is caller-owned and is never retained after a call
+ * returns.  No C++ or torch type crosses the boundary.
+ *
+ * Reference interfaces each entry point replaces:
+ *   gg_render            raytracer::curved::render / render_with_disk
+ *                        (crates/raytracer/src/curved/renderer.rs:82-101, driver
+ *                        render_inner/render_row :256-345) and the per-ray loops
+ *                        trace_ray / trace_ray_with_disk
+ *                        (crates/raytracer/src/curved/tracer.rs:54-152).
+ *   gg_trace_states      trace_ray / trace_ray_with_disk on caller-supplied
+ *                        GeodesicState values (tracer.rs:54-87; state type
+ *                        crates/gr-core/src/geodesic.rs:13-20).
+ *   gg_geodesic_acceleration
+ *                        Metric::geodesic_acceleration, batched
+ *                        (crates/gr-core/src/metric.rs:37-52, Kerr via
+ *                        kerr.rs:100-424, Schwarzschild override
+ *                        schwarzschild.rs:99-140).
+ *   gg_render_batch      many independent render() calls (one per scene) fused
+ *                        into one persistent launch; no single reference
+ *                        counterpart, the reference calls render() per image.
+ *   gg_grad              no reference counterpart: gradient of an image loss
+ *                        through the thin-disk render (reverse mode, hand
+ *                        written adjoint with checkpoints).
+ *
+ * Errors: every entry point returns GG_OK (0) or a negative gg_status; the
+ * message of the last failure on the calling thread is gg_last_error().
+ * Invalid arguments never reach a kernel.  Where the reference panics
+ * (Kerr::new with mass <= 0, kerr.rs:30) the ABI returns GG_ERR_INVALID_ARG.
+ *
+ * Threading: calls on distinct contexts may run concurrently.  Calls that
+ * share one context are serialised by the caller.
+ */
+#ifndef GARGANTUA_H
+#define GARGANTUA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GG_ABI_VERSION 1
+
+typedef enum gg_status {
+  GG_OK = 0,
+  GG_ERR_INVALID_ARG = -1,
+  GG_ERR_DEVICE = -2,        /* HIP runtime error (message in gg_last_error) */
+  GG_ERR_NO_DEVICE = -3,     /* no HIP device visible                          */
+  GG_ERR_OUT_OF_MEMORY = -4,
+  GG_ERR_UNSUPPORTED = -5,
+  GG_ERR_INTERNAL = -6
+} gg_status;
+
+/* ---- metric (crates/gr-core/src/kerr.rs:17-34, schwarzschild.rs:7-18) ---- */
+typedef enum gg_metric_kind {
+  GG_METRIC_SCHWARZSCHILD = 0,
+  GG_METRIC_KERR = 1
+} gg_metric_kind;
+
+typedef struct gg_metric {
+  int32_t kind;     /* gg_metric_kind                                         */
+  int32_t _pad;
+  double mass;      /* M > 0                                                  */
+  double spin;      /* a; clamped to [-0.9999 M, 0.9999 M] like Kerr::new    */
+} gg_metric;
+
+/* ---- camera (crates/raytracer/src/curved/camera.rs:9-15) ---- */
+typedef struct gg_camera {
+  double position[4];   /* (t, r, theta, phi) in the metric chart             */
+  double look_at[3];    /* camera-local Cartesian frame                       */
+  double up[3];
+  double fov_y_radians;
+  double aspect;
+} gg_camera;
+
+/* ---- disk (crates/raytracer/src/curved/disk.rs:8-31) ---- */
+typedef enum gg_disk_kind {
+  GG_DISK_NONE = 0,        /* trace_ray: sky / horizon only (tracer.rs:54-70)  */
+  GG_DISK_VOLUMETRIC = 1,  /* trace_ray_with_disk (tracer.rs:80-152)           */
+  GG_DISK_THIN = 2         /* thin equatorial disk, I = r^-q g^4, terminate on
+                              crossing (no reference counterpart)              */
+} gg_disk_kind;
+
+typedef struct gg_disk {
+  int32_t kind;            /* gg_disk_kind                                    */
+  int32_t _pad;
+  double r_inner;          /* AccretionDisk::r_inner (default 6)              */
+  double r_outer;          /* AccretionDisk::r_outer (default 20)             */
+  double scale_height;     /* volumetric: Gaussian H (default 1)              */
+  double density_0;        /* volumetric: rho_0 (default 1)                   */
+  double temperature_0;    /* volumetric: T_0 (default 1)                     */
+  double absorption;       /* volumetric: 0.1 in tracer.rs:91                 */
+  double emissivity_q;     /* thin: I = r_c^-q g^4                            */
+  double edge_width;       /* thin: 0 = hard [r_inner, r_outer]; w > 0 = smooth
+                              logistic edges of width w (differentiable mode) */
+} gg_disk;
+
+/* ---- integrator (crates/gr-core/src/geodesic.rs:155-181) ---- */
+typedef enum gg_err_mode {
+  GG_ERR_REFERENCE = 0,    /* accept iff ||dv||_2 < tolerance (geodesic.rs:308-325) */
+  GG_ERR_RTOL_ATOL = 1     /* accept iff RMS(dv_i / (atol + rtol max|v_i|)) < 1    */
+} gg_err_mode;
+
+typedef struct gg_integrator {
+  double initial_step;     /* 0.1                                             */
+  double min_step;         /* 1e-8                                            */
+  double max_step;         /* 1.0                                             */
+  double tolerance;        /* 1e-6 (GG_ERR_REFERENCE)                          */
+  double max_radius;       /* escape: r > max_radius (renderer uses 200)      */
+  double horizon_buffer;   /* capture: r < r_h (1 + horizon_rel) + buffer     */
+  double horizon_rel;      /* 0 in the reference                              */
+  double rtol;             /* GG_ERR_RTOL_ATOL                                 */
+  double atol;
+  int32_t err_mode;        /* gg_err_mode                                     */
+  uint32_t max_steps;      /* attempts per ray, as tracer.rs:61 counts them   */
+} gg_integrator;
+
+/* ---- image ---- */
+typedef struct gg_image {
+  uint32_t width;
+  uint32_t height;
+  uint32_t samples_per_axis; /* RenderOptions::samples_per_axis (>= 1)        */
+  uint32_t jitter;           /* 0: pixel centres (ray.rs:52-53); 1: centre +
+                                U[-0.5,0.5)^2 from SplitMix64(seed, ray)      */
+  uint64_t seed;
+} gg_image;
+
+/* A scene is everything one render() call needs. */
+typedef struct gg_scene {
+  gg_metric metric;
+  gg_camera camera;
+  gg_disk disk;
+  gg_integrator integrator;
+  gg_image image;
+} gg_scene;
+
+typedef enum gg_precision {
+  GG_PREC_FP64 = 0,   /* every operation in FP64, reference expression order  */
+  GG_PREC_MIXED = 1   /* far-field RHS stages in FP32, everything else FP64   */
+} gg_precision;
+
+/* Ray classes (RayOutcome, crates/raytracer/src/curved/outcome.rs:3-22;
+ * StepResult::Singular folds into Horizon as tracer.rs:65 does). */
+typedef enum gg_ray_class {
+  GG_CLASS_HORIZON = 0,
+  GG_CLASS_ESCAPED = 1,
+  GG_CLASS_DISK = 2,
+  GG_CLASS_MAX_STEPS = 3
+} gg_ray_class;
+
+/* Per-ray outputs, structure of arrays.  N = (width*spa) * (height*spa),
+ * ray index = sub_py * (width*spa) + sub_px.  Any pointer may be NULL.
+ * Pointers may address host memory or device memory of the context's device
+ * (gg_render detects which). */
+typedef struct gg_outputs {
+  uint8_t *cls;         /* [N] gg_ray_class                                   */
+  double *r;            /* [N] final r                                        */
+  double *theta;        /* [N] final theta                                    */
+  double *phi;          /* [N] final phi                                      */
+  double *intensity;    /* [N] disk intensity (0 if none)                     */
+  uint32_t *attempts;   /* [N] RK45 step() calls                              */
+  uint32_t *accepted;   /* [N] accepted steps                                 */
+  double *state;        /* [9*N] SoA planes: t r th ph vt vr vth vph lambda   */
+  uint8_t *rgb;         /* [3*width*height] reference shading + sRGB          */
+} gg_outputs;
+
+/* Launch statistics (filled when the pointer is non-NULL). */
+typedef struct gg_stats {
+  double kernel_ms;         /* hipEvent time of the trace kernel(s)           */
+  double total_ms;          /* whole call incl. copies                        */
+  uint64_t rays;
+  uint64_t attempts;        /* sum of step() calls                            */
+  uint64_t rhs_evals;       /* RHS evaluations actually made                  */
+  uint64_t rhs_evals_fp32;  /* of which evaluated in FP32 (mixed)             */
+  uint64_t attempts_full;   /* attempts that reached the error estimate       */
+  uint32_t grid_blocks;
+  uint32_t block_threads;
+} gg_stats;
+
+/* Launch shape of the persistent trace kernel. 0 = library default. */
+typedef struct gg_launch {
+  uint32_t block_threads;     /* 64..1024, multiple of 64                     */
+  uint32_t blocks_per_cu;     /* resident blocks requested per CU             */
+  uint32_t refill_interval;   /* attempts between retired-lane refills        */
+} gg_launch;
+
+typedef struct gg_context gg_context;
+
+const char *gg_strerror(int status);
+const char *gg_last_error(void);
+int gg_abi_version(void);
+
+/* stream: a hipStream_t of the device, or NULL for a library-owned stream. */
+int gg_context_create(int device, void *stream, gg_context **out);
+int gg_context_destroy(gg_context *ctx);
+int gg_set_launch(gg_context *ctx, const gg_launch *launch);
+int gg_get_launch(gg_context *ctx, gg_launch *launch);
+
+/* Rays traced by gg_render for a scene (validates the scene). */
+int gg_scene_rays(const gg_scene *scene, uint64_t *n_rays);
+
+int gg_render(gg_context *ctx, const gg_scene *scene, int precision,
+              const gg_outputs *out, gg_stats *stats);
+
+/* n scenes in one persistent launch; out[i] belongs to scenes[i]. */
+int gg_render_batch(gg_context *ctx, const gg_scene *scenes, uint32_t n,
+                    int precision, const gg_outputs *out, gg_stats *stats);
+
+/* Trace n caller-supplied states (9 doubles each, AoS: t r th ph vt vr vth vph
+ * lambda).  Camera and image fields of the scene are ignored except for the
+ * observer position used by disk redshift (camera.position). */
+int gg_trace_states(gg_context *ctx, const gg_scene *scene, int precision,
+                    uint64_t n, const double *states_in, const gg_outputs *out,
+                    gg_stats *stats);
+
+/* acc[4*i..] = -Gamma^mu_ab v^a v^b at (pos[4*i..], vel[4*i..]); AoS. */
+int gg_geodesic_acceleration(gg_context *ctx, const gg_metric *metric,
+                             int precision, uint64_t n, const double *pos,
+                             const double *vel, double *acc);
+
+/* ---- differentiable thin-disk render (no reference counterpart) ---- */
+typedef struct gg_grad_scene {
+  double mass;
+  double spin;             /* a      (differentiated)                        */
+  double inclination;      /* i, rad (differentiated)                        */
+  double emissivity_q;     /* q      (differentiated)                        */
+  double r_inner;          /* r_in   (differentiated)                        */
+  double r_outer;
+  double edge_width;       /* > 0: logistic edge width                       */
+  double r_camera;
+  double fov_y_radians;
+  gg_integrator integrator;
+  uint32_t width;
+  uint32_t height;
+} gg_grad_scene;
+
+/* Loss L = mean_p (I_p - target_p)^2 over width*height pixel-centre rays.
+ * grad[0..3] = dL/d(spin, inclination, q, r_inner).  intensity (optional,
+ * [N]) receives the forward image.  checkpoint_every = K accepted steps. */
+int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target,
+            uint32_t checkpoint_every, double *loss, double grad[4],
+            double *intensity, gg_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GARGANTUA_H */

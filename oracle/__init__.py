@@ -1,0 +1,222 @@
+"""ctypes front end of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline use it as the checker; the product (gargantua_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from gargantua_amd import _abi
+from gargantua_amd import Scene, alloc_outputs, outputs_struct
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liboracle.so")
+
+_D = C.c_double
+_P = C.c_void_p
+_SIGS = [
+    ("oracle_metric_tensor", C.c_int, [C.POINTER(_abi.gg_metric), _P, _P]),
+    ("oracle_christoffel", C.c_int, [C.POINTER(_abi.gg_metric), _P, _P]),
+    ("oracle_geodesic_acceleration", C.c_int, [C.POINTER(_abi.gg_metric), C.c_uint64, _P, _P, _P]),
+    ("oracle_event_horizon", C.c_int, [C.POINTER(_abi.gg_metric), C.POINTER(_D)]),
+    ("oracle_orbital_four_velocity", C.c_int, [C.POINTER(_abi.gg_metric), _P, _P]),
+    ("oracle_redshift_factor", _D, [C.POINTER(_abi.gg_metric), _P, _P, _D]),
+    ("oracle_rk45_step", C.c_int, [C.POINTER(_abi.gg_metric), C.POINTER(_abi.gg_integrator), _P, C.POINTER(_D),
+                                   C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("oracle_rk4_step", C.c_int, [C.POINTER(_abi.gg_metric), _D, _D, _D, _P]),
+    ("oracle_pixel_direction", None, [C.POINTER(_abi.gg_camera), _D, _D, _P]),
+    ("oracle_from_camera", C.c_int, [C.POINTER(_abi.gg_scene), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    ("oracle_disk_density", _D, [C.POINTER(_abi.gg_disk), _P]),
+    ("oracle_disk_emission", _D, [C.POINTER(_abi.gg_disk), _P]),
+    ("oracle_disk_color", None, [_D, _P]),
+    ("oracle_encode_srgb", None, [_P, _P]),
+    ("oracle_sky", None, [_P, _P]),
+    ("oracle_splitmix64", C.c_uint64, [C.c_uint64, C.c_uint64]),
+    ("oracle_render_rays", C.c_int, [C.POINTER(_abi.gg_scene), C.c_uint64, _P, C.POINTER(_abi.gg_outputs), C.c_int]),
+    ("oracle_render", C.c_int, [C.POINTER(_abi.gg_scene), C.POINTER(_abi.gg_outputs), C.c_int]),
+    ("oracle_trace_states", C.c_int, [C.POINTER(_abi.gg_scene), C.c_uint64, _P, C.POINTER(_abi.gg_outputs),
+                                      C.c_int]),
+    ("oracle_last_rhs", C.c_uint64, []),
+    ("oracle_r_isco", _D, [_D, _D]),
+    ("oracle_grad", C.c_int, [C.POINTER(_abi.gg_grad_scene), _P, C.POINTER(_D), _P, _P, _P, C.c_int]),
+    ("oracle_loss_frozen", C.c_int, [C.POINTER(_abi.gg_grad_scene), C.POINTER(_abi.gg_grad_scene), _P,
+                                     C.POINTER(_D), _P, C.c_int]),
+    ("oracle_grad_scene_to_scene", C.c_int, [C.POINTER(_abi.gg_grad_scene), C.POINTER(_abi.gg_scene)]),
+]
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None:
+        assert a.size == n
+    return a
+
+
+class Oracle:
+    def __init__(self, threads: int = 0):
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        self.lib = C.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            f = getattr(self.lib, name)
+            f.restype = res
+            f.argtypes = args
+        self.threads = threads
+
+    # ---- metric level ----
+    def metric_tensor(self, metric, pos):
+        g = np.zeros(16)
+        p = _f64(pos, 4)
+        self.lib.oracle_metric_tensor(C.byref(metric.to_c()), p.ctypes.data, g.ctypes.data)
+        return g.reshape(4, 4)
+
+    def christoffel(self, metric, pos):
+        g = np.zeros(64)
+        p = _f64(pos, 4)
+        self.lib.oracle_christoffel(C.byref(metric.to_c()), p.ctypes.data, g.ctypes.data)
+        return g.reshape(4, 4, 4)
+
+    def geodesic_acceleration(self, metric, pos, vel):
+        pos = _f64(pos).reshape(-1, 4)
+        vel = _f64(vel).reshape(-1, 4)
+        acc = np.zeros_like(pos)
+        self.lib.oracle_geodesic_acceleration(C.byref(metric.to_c()), len(pos), pos.ctypes.data, vel.ctypes.data,
+                                              acc.ctypes.data)
+        return acc
+
+    def event_horizon(self, metric):
+        rh = C.c_double()
+        ok = self.lib.oracle_event_horizon(C.byref(metric.to_c()), C.byref(rh))
+        return rh.value if ok else None
+
+    def orbital_four_velocity(self, metric, pos):
+        u = np.zeros(4)
+        p = _f64(pos, 4)
+        ok = self.lib.oracle_orbital_four_velocity(C.byref(metric.to_c()), p.ctypes.data, u.ctypes.data)
+        return u if ok else None
+
+    def redshift_factor(self, metric, pos, p, sqrt_neg_gtt_obs):
+        x = _f64(pos, 4)
+        k = _f64(p, 4)
+        return self.lib.oracle_redshift_factor(C.byref(metric.to_c()), x.ctypes.data, k.ctypes.data,
+                                               sqrt_neg_gtt_obs)
+
+    # ---- integrator level ----
+    def rk45_step(self, metric, integ, state, h):
+        s = _f64(state, 9).copy()
+        hh = C.c_double(h)
+        acc = C.c_int()
+        nr = C.c_int()
+        res = self.lib.oracle_rk45_step(C.byref(metric.to_c()), C.byref(integ.to_c()), s.ctypes.data, C.byref(hh),
+                                        C.byref(acc), C.byref(nr))
+        return res, s, hh.value, bool(acc.value), nr.value
+
+    def rk4_step(self, metric, step, state, max_radius=100.0, horizon_buffer=0.01):
+        s = _f64(state, 9).copy()
+        res = self.lib.oracle_rk4_step(C.byref(metric.to_c()), step, max_radius, horizon_buffer, s.ctypes.data)
+        return res, s
+
+    def pixel_direction(self, camera, u, v):
+        d = np.zeros(3)
+        self.lib.oracle_pixel_direction(C.byref(camera.to_c()), u, v, d.ctypes.data)
+        return d
+
+    def from_camera(self, scene: Scene, px, py, width=None, height=None):
+        s = np.zeros(9)
+        sc = scene.to_c()
+        self.lib.oracle_from_camera(C.byref(sc), px, py, width or scene.width * scene.samples_per_axis,
+                                    height or scene.height * scene.samples_per_axis, s.ctypes.data)
+        return s
+
+    # ---- disk & shading ----
+    def disk_density(self, disk, pos):
+        p = _f64(pos, 4)
+        return self.lib.oracle_disk_density(C.byref(disk.to_c()), p.ctypes.data)
+
+    def disk_emission(self, disk, pos):
+        p = _f64(pos, 4)
+        return self.lib.oracle_disk_emission(C.byref(disk.to_c()), p.ctypes.data)
+
+    def disk_color(self, intensity):
+        out = np.zeros(3)
+        self.lib.oracle_disk_color(intensity, out.ctypes.data)
+        return out
+
+    def encode_srgb(self, lin):
+        lin = _f64(lin, 3)
+        out = np.zeros(3, np.uint8)
+        self.lib.oracle_encode_srgb(lin.ctypes.data, out.ctypes.data)
+        return out
+
+    def sky(self, d):
+        d = _f64(d, 3)
+        out = np.zeros(3)
+        self.lib.oracle_sky(d.ctypes.data, out.ctypes.data)
+        return out
+
+    def splitmix64(self, seed, n):
+        return self.lib.oracle_splitmix64(seed, n)
+
+    def r_isco(self, mass, spin):
+        return self.lib.oracle_r_isco(mass, spin)
+
+    # ---- whole paths ----
+    def render(self, scene: Scene, rgb: bool = False, state: bool = True) -> dict:
+        o = alloc_outputs(scene.n_rays, state=state, rgb_pixels=scene.width * scene.height if rgb else 0)
+        sc = scene.to_c()
+        rc = self.lib.oracle_render(C.byref(sc), C.byref(outputs_struct(o)), self.threads)
+        if rc:
+            raise RuntimeError(f"oracle_render failed: {rc}")
+        o["rhs"] = self.lib.oracle_last_rhs()
+        return o
+
+    def render_rays(self, scene: Scene, idx, state: bool = True) -> dict:
+        idx = np.ascontiguousarray(idx, dtype=np.uint64)
+        o = alloc_outputs(len(idx), state=state)
+        sc = scene.to_c()
+        rc = self.lib.oracle_render_rays(C.byref(sc), len(idx), idx.ctypes.data, C.byref(outputs_struct(o)),
+                                         self.threads)
+        if rc:
+            raise RuntimeError(f"oracle_render_rays failed: {rc}")
+        o["rhs"] = self.lib.oracle_last_rhs()
+        return o
+
+    def trace_states(self, scene: Scene, states) -> dict:
+        states = _f64(states).reshape(-1, 9)
+        o = alloc_outputs(len(states))
+        sc = scene.to_c()
+        rc = self.lib.oracle_trace_states(C.byref(sc), len(states), states.ctypes.data, C.byref(outputs_struct(o)),
+                                          self.threads)
+        if rc:
+            raise RuntimeError(f"oracle_trace_states failed: {rc}")
+        return o
+
+    def grad(self, gscene, target, want_dI=False):
+        n = gscene.width * gscene.height
+        target = _f64(target, n)
+        loss = C.c_double()
+        grad = np.zeros(4)
+        inten = np.zeros(n)
+        dI = np.zeros(4 * n) if want_dI else None
+        rc = self.lib.oracle_grad(C.byref(gscene), target.ctypes.data, C.byref(loss), grad.ctypes.data,
+                                  inten.ctypes.data, dI.ctypes.data if want_dI else None, self.threads)
+        if rc:
+            raise RuntimeError(f"oracle_grad failed: {rc}")
+        out = {"loss": loss.value, "grad": grad, "intensity": inten}
+        if want_dI:
+            out["dI"] = dI.reshape(n, 4)
+        return out
+
+    def loss_frozen(self, ref, gscene, target):
+        n = gscene.width * gscene.height
+        target = _f64(target, n)
+        loss = C.c_double()
+        inten = np.zeros(n)
+        rc = self.lib.oracle_loss_frozen(C.byref(ref), C.byref(gscene), target.ctypes.data, C.byref(loss),
+                                         inten.ctypes.data, self.threads)
+        if rc:
+            raise RuntimeError(f"oracle_loss_frozen failed: {rc}")
+        return loss.value, inten
