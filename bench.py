@@ -1,0 +1,175 @@
+"""Benchmark: geodesics/s for Kerr a=0.9 at 1024x1024 on one MI355X.
+
+A step is one render of the whole 1024x1024 jittered Kerr a=0.9 thin-disk
+scene (camera r=100M, inclination 60 deg, FOV 25 deg, rtol 1e-8, atol 1e-10)
+by the mixed FP32/FP64 kernel, with the per-ray outputs written to device
+memory.  The plain FP64 kernel is timed the same way beside it, and the
+mixed result is compared with the FP64 one (class agreement, final-state
+error).  The roofline object reports algorithmic FLOPs of the trace kernel
+over its hipEvent time against the vector (VALU) peak; the CPU baseline is
+the FP64 oracle on a stratified sample of the same rays.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic FLOPs (add, sub, mul, div, sqrt, sin, cos, pow = 1 each; the
+# FP64 kernels contract nothing) counted from the kernel source after common
+# subexpressions: Kerr RHS 141 ops for the Christoffel symbols + 96 for the
+# 32 non-zero contraction terms; Schwarzschild RHS 44; stage combinations,
+# error norm and controller of one attempt that reaches the error estimate
+# 318 (rtol/atol mode).
+F_RHS = {"kerr": 237, "schwarzschild": 44}
+F_COMB = 318
+PEAK_FP64 = 78.6    # TFLOP/s, MI355X vector FP64 (published)
+PEAK_FP32 = 157.3   # TFLOP/s, MI355X vector FP32 (published; needs packed math)
+
+METRIC = "geodesics/s Kerr a=0.9 1024²: mixed FP32 vs FP64 at ≤1e-6 err; adjoint grads/s"
+
+
+def _dev_outputs(torch, n):
+    d = "cuda"
+    return {
+        "cls": torch.empty(n, dtype=torch.uint8, device=d), "r": torch.empty(n, dtype=torch.float64, device=d),
+        "theta": torch.empty(n, dtype=torch.float64, device=d), "phi": torch.empty(n, dtype=torch.float64, device=d),
+        "intensity": torch.empty(n, dtype=torch.float64, device=d),
+        "attempts": torch.empty(n, dtype=torch.int32, device=d), "accepted": torch.empty(n, dtype=torch.int32, device=d),
+    }
+
+
+def _ptrs(o):
+    return {k: v.data_ptr() for k, v in o.items()}
+
+
+def time_precision(torch, rend, scene, precision, steps, warmup, outs):
+    ptrs = _ptrs(outs)
+    for _ in range(warmup):
+        rend.render(scene, precision, outputs=dict(ptrs))
+    torch.cuda.synchronize()
+    kms, stats = [], None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o = rend.render(scene, precision, outputs=dict(ptrs))
+        kms.append(o["stats"]["kernel_ms"])
+        stats = o["stats"]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, kms, stats
+
+
+def flops(stats, metric_kind):
+    f_rhs = F_RHS[metric_kind]
+    f32 = stats["rhs_evals_fp32"] * f_rhs
+    f64 = (stats["rhs_evals"] - stats["rhs_evals_fp32"]) * f_rhs + stats["attempts_full"] * F_COMB
+    return f32, f64
+
+
+def cpu_baseline(scene, stride=8):
+    """FP64 oracle on every `stride`-th row and column of the same rays."""
+    from oracle import Oracle
+    w = scene.width
+    idx = np.array([py * w + px for py in range(0, scene.height, stride) for px in range(0, w, stride)],
+                   dtype=np.uint64)
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    ora = Oracle(threads=cores)
+    t0 = time.perf_counter()
+    ora.render_rays(scene, idx, state=False)
+    dt = time.perf_counter() - t0
+    one = Oracle(threads=1)
+    sub = idx[:: max(1, len(idx) // 1024)]
+    t1 = time.perf_counter()
+    one.render_rays(scene, sub, state=False)
+    dt1 = time.perf_counter() - t1
+    return {"value": len(idx) / dt, "unit": "geodesics/s", "cores": cores, "kind": "port",
+            "sample": f"every {stride}th row and column of the 1024x1024 jittered Kerr a=0.9 scene "
+                      f"({len(idx)} rays, {dt:.1f} s on {cores} threads); 1 thread: {len(sub) / dt1:.1f} geodesics/s "
+                      f"on {len(sub)} rays",
+            "value_1core": len(sub) / dt1}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--size", type=int, default=1024)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(rays are independent; the multi-GPU split is described in the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import gargantua_amd as gg
+    from tests.parity import compare
+
+    scene = gg.configs.kerr_1024(args.size, args.size)
+    n = scene.n_rays
+    rend = gg.Renderer(0, stream=torch.cuda.current_stream().cuda_stream)
+    outs_m = _dev_outputs(torch, n)
+    outs_d = _dev_outputs(torch, n)
+
+    wall_m, kms_m, st_m = time_precision(torch, rend, scene, "mixed", args.steps, args.warmup, outs_m)
+    wall_d, kms_d, st_d = time_precision(torch, rend, scene, "fp64", args.steps, args.warmup, outs_d)
+
+    # accuracy of the mixed kernel against the FP64 kernel (same inputs)
+    host = lambda o: {k: v.cpu().numpy() for k, v in o.items()}
+    hm, hd = host(outs_m), host(outs_d)
+    for h in (hm, hd):
+        h["attempts"] = h["attempts"].astype(np.uint32)
+    cmp = compare(hm, hd, shape=(scene.height, scene.width))
+
+    value = n * args.steps / wall_m
+    ms_step = 1e3 * wall_m / args.steps
+    k_ms = float(np.mean(kms_m))
+    f32, f64 = flops(st_m, "kerr")
+    tflops = (f32 + f64) / (k_ms * 1e-3) / 1e12
+    peak = (f32 + f64) / (f32 / PEAK_FP32 + f64 / PEAK_FP64) if (f32 + f64) else PEAK_FP64
+    d32, d64 = flops(st_d, "kerr")
+    k_ms_d = float(np.mean(kms_d))
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "geodesics/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (camera rays of the scene; no inputs)",
+        "config": {"workload": f"Kerr a=0.9 {args.size}x{args.size} thin disk r_ISCO..30M q=3, camera r=100M "
+                               "i=60deg FOV 25deg, jitter seed 2026, rtol 1e-8 atol 1e-10, escape 1000M, "
+                               "capture 1.01 r_h, 20000 attempts", "precision": "mixed",
+                   "rays": n, "launch": rend.get_launch()},
+        "roofline": {"bound": "valu", "achieved": tflops, "peak": peak, "unit": "TFLOP/s", "frac": tflops / peak,
+                     "traffic": None, "kernel": "trace_kernel<Kerr, thin, mixed>",
+                     "kernel_ms": k_ms, "flop_per_launch": f32 + f64, "fp32_flop_share": f32 / (f32 + f64)},
+        "fp64": {"value": n * args.steps / wall_d, "kernel_ms": k_ms_d,
+                 "roofline_frac": (d32 + d64) / (k_ms_d * 1e-3) / 1e12 / PEAK_FP64,
+                 "mixed_speedup": k_ms_d / k_ms},
+        "mixed_vs_fp64": {k: cmp[k] for k in ("class_agree", "class_agree_outside_band", "err_le_1e-6",
+                                             "err_le_1e-6_outside_band", "psnr_intensity", "state_bit_identical")},
+        "work": {"attempts": st_m["attempts"], "rhs_evals": st_m["rhs_evals"], "rhs_evals_fp32": st_m["rhs_evals_fp32"],
+                 "attempts_per_ray": st_m["attempts"] / n},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(scene)
+    rend.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
