@@ -30,7 +30,7 @@ from ._abi import (GG_CLASS_DISK, GG_CLASS_ESCAPED, GG_CLASS_HORIZON, GG_CLASS_M
                    GG_DISK_NONE, GG_DISK_THIN, GG_DISK_VOLUMETRIC, GG_ERR_REFERENCE, GG_ERR_RTOL_ATOL,
                    GG_METRIC_KERR, GG_METRIC_SCHWARZSCHILD)
 
-PRECISIONS = {"fp64": _abi.GG_PREC_FP64, "mixed": _abi.GG_PREC_MIXED}
+PRECISIONS = {"fp64": _abi.GG_PREC_FP64, "mixed": _abi.GG_PREC_MIXED, "fp64fast": _abi.GG_PREC_FP64_FAST}
 
 
 class GargantuaError(RuntimeError):

@@ -20,6 +20,7 @@ GG_ERR_REFERENCE = 0
 GG_ERR_RTOL_ATOL = 1
 GG_PREC_FP64 = 0
 GG_PREC_MIXED = 1
+GG_PREC_FP64_FAST = 2
 GG_CLASS_HORIZON = 0
 GG_CLASS_ESCAPED = 1
 GG_CLASS_DISK = 2
@@ -104,7 +105,7 @@ EXPORTS = [
 ]
 
 LIB_NAME = "libgargantua.so"
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+LIB_PATH = os.environ.get("GARGANTUA_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 _lib = None
 

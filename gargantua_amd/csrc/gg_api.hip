@@ -270,6 +270,7 @@ void build_dev_scene(const gg_scene &s, DevScene &D) {
   D.q = s.disk.emissivity_q;
   D.edge_width = s.disk.edge_width;
   D.fp32_min_r = 10.0 * D.mass;
+  D.fp32_far_r = 300.0 * D.mass;
 }
 
 bool is_device_ptr(const void *p) {
@@ -422,7 +423,7 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
   if (!ctx) return fail(GG_ERR_INVALID_ARG, "context is NULL");
   if (!scenes || n_scenes == 0) return fail(GG_ERR_INVALID_ARG, "no scenes");
   if (!outs) return fail(GG_ERR_INVALID_ARG, "outputs are NULL");
-  if (precision != GG_PREC_FP64 && precision != GG_PREC_MIXED)
+  if (precision < GG_PREC_FP64 || precision > GG_PREC_FP64_FAST)
     return fail(GG_ERR_INVALID_ARG, "precision %d unknown", precision);
   const bool from_states = (states_host != nullptr);
   int rc;
@@ -562,7 +563,7 @@ int gg_geodesic_acceleration(gg_context *ctx, const gg_metric *metric, int preci
   if (!ctx || !metric) return fail(GG_ERR_INVALID_ARG, "NULL argument");
   if (n == 0) return GG_OK;
   if (!pos || !vel || !acc) return fail(GG_ERR_INVALID_ARG, "NULL pos/vel/acc");
-  if (precision != GG_PREC_FP64 && precision != GG_PREC_MIXED)
+  if (precision < GG_PREC_FP64 || precision > GG_PREC_FP64_FAST)
     return fail(GG_ERR_INVALID_ARG, "precision %d unknown", precision);
   int rc = validate_metric(*metric);
   if (rc) return rc;

@@ -50,8 +50,9 @@ struct DevScene {
   uint64_t seed;
   // disk
   double r_in, r_out, scale_height, density_0, absorption, q, edge_width;
-  // mixed precision policy
-  double fp32_min_r;
+  // mixed precision policy: FP32 stages on outgoing rays beyond fp32_min_r,
+  // on inbound rays beyond fp32_far_r
+  double fp32_min_r, fp32_far_r;
   // batch bookkeeping
   uint64_t ray_begin;     // first global ray index of this scene
   uint64_t out_index;     // which output set (batch)
@@ -210,6 +211,109 @@ __device__ inline d4 rhs64(const DevScene &S, double r, double theta, d4 v) {
   return rhs64_schw(S, r, theta, v);
 }
 
+// ----------------------------------------------------------- fast FP64 ----
+
+// 1/x to ~1 ulp: v_rcp_f64 refined by two Newton steps (x is a positive,
+// normal quantity here: Sigma, Delta or sin^2 theta).
+__device__ inline double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+// Kerr RHS in FP64 with the algebra of rhs32_kerr: the same Christoffel
+// symbols through 1/Sigma, 1/Delta and 1/sin^2 instead of fifteen divisions,
+// FMA contraction and merged symmetric pairs.  Agrees with rhs64_kerr to a
+// few ulp; used by the fast-FP64 and mixed kernels.
+__device__ inline d4 rhs64f_kerr(double m, double a, double r_in, double theta, d4 v) {
+#pragma clang fp contract(fast)
+  double r = rmax1e10(r_in);
+  double st, ct;
+  sincos(theta, &st, &ct);
+  double a2 = a * a;
+  double r2 = r * r;
+  double sin2 = st * st;
+  double sc = st * ct;
+  double sigma = r2 + a2 * ct * ct;
+  double twom = 2.0 * m;
+  double delta = r2 - twom * r + a2;
+  double is = rcp64(sigma);
+  double id = rcp64(delta);
+  double is2 = is * is;
+  double isd = is * id;
+  double dsig_th = -2.0 * a2 * sc;
+  double r2pa2 = r2 + a2;
+  double big_a = r2pa2 * r2pa2 - a2 * delta * sin2;
+  double ginv_tt = -big_a * isd;
+  double ginv_tph = -twom * a * r * isd;
+  double ginv_rr = delta * is;
+  double ginv_thth = is;
+  double ginv_phph = (delta - a2 * sin2) * isd * rcp64(sin2);
+  double sig_m2r2 = sigma - 2.0 * r2;
+  double dtt_dr = twom * sig_m2r2 * is2;
+  double dtt_dth = -twom * r * dsig_th * is2;
+  double drr_dr = (2.0 * r * delta - sigma * (2.0 * r - twom)) * id * id;
+  double drr_dth = dsig_th * id;
+  double dthth_dr = 2.0 * r;
+  double dthth_dth = dsig_th;
+  double big_p = twom * a2 * r;
+  double dpp_dr = 2.0 * r * sin2 + sin2 * sin2 * twom * a2 * sig_m2r2 * is2;
+  double dpp_dth = 2.0 * r2pa2 * sc + big_p * (4.0 * sin2 * sc * sigma - sin2 * sin2 * dsig_th) * is2;
+  double dtp_dr = -twom * a * sin2 * sig_m2r2 * is2;
+  double dtp_dth = -twom * a * r * (2.0 * sc * sigma - sin2 * dsig_th) * is2;
+
+  double G001 = 0.5 * (ginv_tt * dtt_dr + ginv_tph * dtp_dr);
+  double G002 = 0.5 * (ginv_tt * dtt_dth + ginv_tph * dtp_dth);
+  double G013 = 0.5 * (ginv_tt * dtp_dr + ginv_tph * dpp_dr);
+  double G023 = 0.5 * (ginv_tt * dtp_dth + ginv_tph * dpp_dth);
+  double hrr = 0.5 * ginv_rr;
+  double G100 = -hrr * dtt_dr, G111 = hrr * drr_dr, G122 = -hrr * dthth_dr, G133 = -hrr * dpp_dr;
+  double G112 = hrr * drr_dth, G103 = -hrr * dtp_dr;
+  double hth = 0.5 * ginv_thth;
+  double G200 = -hth * dtt_dth, G211 = -hth * drr_dth, G222 = hth * dthth_dth, G233 = -hth * dpp_dth;
+  double G212 = hth * dthth_dr, G203 = -hth * dtp_dth;
+  double G301 = 0.5 * (ginv_tph * dtt_dr + ginv_phph * dtp_dr);
+  double G302 = 0.5 * (ginv_tph * dtt_dth + ginv_phph * dtp_dth);
+  double G313 = 0.5 * (ginv_tph * dtp_dr + ginv_phph * dpp_dr);
+  double G323 = 0.5 * (ginv_tph * dtp_dth + ginv_phph * dpp_dth);
+
+  const double v0 = v.x, v1 = v.y, v2 = v.z, v3 = v.w;
+  d4 acc;
+  acc.x = -2.0 * (G001 * v0 * v1 + G002 * v0 * v2 + G013 * v1 * v3 + G023 * v2 * v3);
+  acc.y = -(G100 * v0 * v0 + G111 * v1 * v1 + G122 * v2 * v2 + G133 * v3 * v3 +
+            2.0 * (G112 * v1 * v2 + G103 * v0 * v3));
+  acc.z = -(G200 * v0 * v0 + G211 * v1 * v1 + G222 * v2 * v2 + G233 * v3 * v3 +
+            2.0 * (G212 * v1 * v2 + G203 * v0 * v3));
+  acc.w = -2.0 * (G301 * v0 * v1 + G302 * v0 * v2 + G313 * v1 * v3 + G323 * v2 * v3);
+  return acc;
+}
+
+__device__ inline d4 rhs64f_schw(double rs, double r_in, double theta, d4 v) {
+#pragma clang fp contract(fast)
+  double r = rmax1e10(r_in);
+  double st, ct;
+  sincos(theta, &st, &ct);
+  double ir = rcp64(r);
+  double rmrs = r - rs;
+  double ifd = rcp64(r * rmrs);
+  double g_t_tr = 0.5 * rs * ifd;
+  double g_r_tt = 0.5 * rs * rmrs * ir * ir * ir;
+  d4 acc;
+  acc.x = -2.0 * g_t_tr * v.x * v.y;
+  acc.y = -g_r_tt * v.x * v.x + g_t_tr * v.y * v.y + rmrs * (v.z * v.z + st * st * v.w * v.w);
+  acc.z = -2.0 * ir * v.y * v.z + st * ct * v.w * v.w;
+  acc.w = -2.0 * ir * v.y * v.w - 2.0 * (ct * rcp64(st)) * v.z * v.w;
+  return acc;
+}
+
+template <int METRIC>
+__device__ inline d4 rhs64f(const DevScene &S, double r, double theta, d4 v) {
+  if (METRIC == 1) return rhs64f_kerr(S.mass, S.spin, r, theta, v);
+  return rhs64f_schw(S.rs, r, theta, v);
+}
+
 // ---------------------------------------------------------------- FP32 ----
 
 __device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -220,7 +324,7 @@ __device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline f4 rhs32_kerr(float m, float a, float r, float theta, f4 v) {
 #pragma clang fp contract(fast)
   float st, ct;
-  __sincosf(theta, &st, &ct);
+  sincosf(theta, &st, &ct);
   float a2 = a * a;
   float r2 = r * r;
   float sin2 = st * st;
@@ -282,7 +386,7 @@ __device__ inline f4 rhs32_kerr(float m, float a, float r, float theta, f4 v) {
 __device__ inline f4 rhs32_schw(float rs, float r, float theta, f4 v) {
 #pragma clang fp contract(fast)
   float st, ct;
-  __sincosf(theta, &st, &ct);
+  sincosf(theta, &st, &ct);
   float ir = frcp(r);
   float rmrs = r - rs;
   float ifd = frcp(r * rmrs);

@@ -47,17 +47,36 @@ constexpr double HALF_PI = PI / 2.0;
 
 enum { RES_CONTINUE = 0, RES_HORIZON = 1, RES_ESCAPED = 2 };
 
+// Stage tableau in constant memory, indexed by the (wavefront-uniform) stage
+// number s = 2..7 of k_s; the values are the reference's double quotients.
+__constant__ double kA[7] = {0.0, 0.0, A2, A3, A4, A5, 1.0};
+__constant__ double kB[7][6] = {
+    {0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}, {0, B21, 0, 0, 0, 0}, {0, B31, B32, 0, 0, 0},
+    {0, B41, B42, B43, 0, 0}, {0, B51, B52, B53, B54, 0}, {0, B61, B62, B63, B64, B65}};
+__constant__ double kC[7] = {0.0, C1, 0.0, C3, C4, C5, C6};
+__constant__ double kD[8] = {0.0, D1, 0.0, D3, D4, D5, D6, D7};
+
 struct Lane {
   double pos[4];
   d4 vel;
   double lambda;
   double h;
-  d4 k1;
-  bool k1_ok;
+  bool k1_ok;        // LDS slot 0 holds k1 = a(pos, vel)
   uint32_t attempts, accepted;
   double intensity, transmission;
   double err_prev;   // error ratio of the previous full attempt
   uint32_t n_rhs, n_rhs32, n_full;
+};
+
+// Per-lane view of the LDS stage store: slots 0..4 hold k1..k5, structure of
+// arrays [slot][component][lane] so each ds_read_b64 of a wavefront touches
+// 64 consecutive doubles.
+struct KStore {
+  double *p;        // &lds[threadIdx.x]
+  uint32_t stride;  // blockDim.x
+  __device__ double &at(int slot, int c) const { return p[(slot * 4 + c) * stride]; }
+  __device__ d4 get(int slot) const { return d4{at(slot, 0), at(slot, 1), at(slot, 2), at(slot, 3)}; }
+  __device__ void put(int slot, d4 k) const { at(slot, 0) = k.x; at(slot, 1) = k.y; at(slot, 2) = k.z; at(slot, 3) = k.w; }
 };
 
 __device__ inline bool valid4(double p0, double p1, double p2, double p3) {
@@ -66,180 +85,229 @@ __device__ inline bool valid4(double p0, double p1, double p2, double p3) {
 
 __device__ inline d4 to64(f4 a) { return d4{(double)a.x, (double)a.y, (double)a.z, (double)a.w}; }
 
-template <int METRIC>
-__device__ inline d4 eval_rhs(const DevScene &S, bool use32, double r, double th, d4 v) {
-  if (use32) {
+// Arithmetic of one attempt.  ARITH_EXACT: FP64 RHS and combinations in the
+// reference's expression order, nothing fused.  ARITH_F32: FP32 RHS, fused
+// combinations.  ARITH_FAST: reciprocal-form FP64 RHS, fused combinations.
+enum { ARITH_EXACT = 0, ARITH_F32 = 1, ARITH_FAST = 2 };
+
+template <int METRIC, int ARITH>
+__device__ inline d4 eval_rhs(const DevScene &S, double r, double th, d4 v) {
+  if (ARITH == ARITH_EXACT) return rhs64<METRIC>(S, r, th, v);
+  if (ARITH == ARITH_F32) {
     f4 vf{(float)v.x, (float)v.y, (float)v.z, (float)v.w};
     return to64(rhs32<METRIC>(S, (float)r, (float)th, vf));
   }
-  return rhs64<METRIC>(S, r, th, v);
+  return rhs64f<METRIC>(S, r, th, v);
 }
 
-// One RK45 attempt.  Returns RES_*; on acceptance sets *acc and leaves the
-// pre-step position/velocity in p0/v0 for the disk hooks.
-template <int METRIC, int PREC>
-__device__ inline int attempt(const DevScene &S, Lane &L, bool &acc, double p0[4], d4 &v0, double &h_used) {
+// a*b + c: unfused (reference rounding) for ARITH_EXACT, one FMA otherwise.
+template <int ARITH>
+__device__ inline double madd(double a, double b, double c) {
 #pragma clang fp contract(off)
+  if (ARITH == ARITH_EXACT) return a * b + c;
+  return __builtin_fma(a, b, c);
+}
+
+// Bound on what FP32 rounding of the six stages can add to the error ratio.
+// One FP32 RHS evaluation errs by a few eps32 times the largest term of each
+// component's contraction; T_c bounds those terms with the Schwarzschild-form
+// symbols (the spin terms are O(a/r) of them beyond 10 M), and the error
+// estimate weighs the stages by |c_i - d_i|, which sum to < 0.2.
+__device__ inline float fp32_noise(const DevScene &S, const double pos[4], d4 v, double h) {
+#pragma clang fp contract(fast)
+  float r = (float)pos[1];
+  float st, ct;
+  sincosf((float)pos[2], &st, &ct);
+  float ir = 1.0f / r;
+  float m_r2 = (float)S.mass * ir * ir;
+  float vt = fabsf((float)v.x), vr = fabsf((float)v.y), vth = fabsf((float)v.z), vph = fabsf((float)v.w);
+  float T0 = 2.0f * m_r2 * vt * (vr + vph * r * (float)fabs(S.spin) * ir);
+  float T1 = m_r2 * (vt * vt + vr * vr) + r * (vth * vth + st * st * vph * vph);
+  float T2 = 2.0f * vr * vth * ir + fabsf(st * ct) * vph * vph;
+  float T3 = 2.0f * vr * vph * ir + 2.0f * fabsf(ct / st) * vth * vph;
+  const float k = 2.0e-7f * (float)h;   // 2 (safety) x 0.2 x 8 eps32
+  if (S.err_mode == 1) {
+    float atol = (float)S.atol, rtol = (float)S.rtol;
+    float n0 = T0 / (atol + rtol * vt), n1 = T1 / (atol + rtol * vr);
+    float n2 = T2 / (atol + rtol * vth), n3 = T3 / (atol + rtol * vph);
+    return k * fmaxf(fmaxf(n0, n1), fmaxf(n2, n3));
+  }
+  return k * sqrtf(T0 * T0 + T1 * T1 + T2 * T2 + T3 * T3) / (float)S.tol;
+}
+
+struct StageOut {
+  d4 nv, k7;
+  double err, thresh;
+  int nr;
+  bool invalid;
+};
+
+// The six stages of one attempt from k1 (LDS slot 0): k2..k7, the 5th-order
+// velocity and the error ratio (geodesic.rs:244-308).
+template <int METRIC, int ARITH>
+__device__ inline StageOut stages(const DevScene &S, const KStore &K, const double p0[4], const d4 vel, double h) {
+#pragma clang fp contract(off)
+  StageOut o;
+  const d4 k1 = K.get(0);
+  d4 s5{C1 * k1.x, C1 * k1.y, C1 * k1.z, C1 * k1.w};   // sum c_i k_i (:294-295)
+  d4 s4{D1 * k1.x, D1 * k1.y, D1 * k1.z, D1 * k1.w};   // sum d_i k_i (:299-305)
+  o.nv = d4{0, 0, 0, 0};
+  o.k7 = d4{0, 0, 0, 0};
+  o.nr = 0;
+  o.invalid = false;
+#pragma unroll 1
+  for (int s = 2; s <= 7; ++s) {
+    double qr, qth;
+    d4 vs;
+    if (s <= 6) {
+      // stage position pos + (h a_s) vel on the forward-Euler line (:248-284)
+      double ha = h * kA[s];
+      double q0 = madd<ARITH>(ha, vel.x, p0[0]), q1 = madd<ARITH>(ha, vel.y, p0[1]);
+      double q2 = madd<ARITH>(ha, vel.z, p0[2]), q3 = madd<ARITH>(ha, vel.w, p0[3]);
+      if (!valid4(q0, q1, q2, q3)) { o.invalid = true; break; }   // :250-291 halve h
+      if (s == 2) {
+        double hb = h * B21;                                        // vel + (h b21) k1 (:249)
+        vs = d4{madd<ARITH>(hb, k1.x, vel.x), madd<ARITH>(hb, k1.y, vel.y), madd<ARITH>(hb, k1.z, vel.z),
+                madd<ARITH>(hb, k1.w, vel.w)};
+      } else {
+        double b = kB[s][1];
+        d4 sum{b * k1.x, b * k1.y, b * k1.z, b * k1.w};
+#pragma unroll 1
+        for (int j = 2; j < s; ++j) {
+          double bj = kB[s][j];
+          d4 kj = K.get(j - 1);
+          sum.x = madd<ARITH>(bj, kj.x, sum.x); sum.y = madd<ARITH>(bj, kj.y, sum.y);
+          sum.z = madd<ARITH>(bj, kj.z, sum.z); sum.w = madd<ARITH>(bj, kj.w, sum.w);
+        }
+        vs = d4{madd<ARITH>(h, sum.x, vel.x), madd<ARITH>(h, sum.y, vel.y), madd<ARITH>(h, sum.z, vel.z),
+                madd<ARITH>(h, sum.w, vel.w)};
+      }
+      qr = q1;
+      qth = q2;
+    } else {
+      qr = madd<ARITH>(h, vel.y, p0[1]);                            // k7 at (pos6, new_vel) (:298)
+      qth = madd<ARITH>(h, vel.z, p0[2]);
+      vs = o.nv;
+    }
+    d4 k = eval_rhs<METRIC, ARITH>(S, qr, qth, vs);
+    ++o.nr;
+    if (s <= 5) K.put(s - 1, k);
+    if (s >= 3) {
+      if (s <= 6) {
+        double c = kC[s];
+        s5.x = madd<ARITH>(c, k.x, s5.x); s5.y = madd<ARITH>(c, k.y, s5.y);
+        s5.z = madd<ARITH>(c, k.z, s5.z); s5.w = madd<ARITH>(c, k.w, s5.w);
+      }
+      double d = kD[s];
+      s4.x = madd<ARITH>(d, k.x, s4.x); s4.y = madd<ARITH>(d, k.y, s4.y);
+      s4.z = madd<ARITH>(d, k.z, s4.z); s4.w = madd<ARITH>(d, k.w, s4.w);
+    }
+    if (s == 6)
+      o.nv = d4{madd<ARITH>(h, s5.x, vel.x), madd<ARITH>(h, s5.y, vel.y), madd<ARITH>(h, s5.z, vel.z),
+                madd<ARITH>(h, s5.w, vel.w)};
+    if (s == 7) o.k7 = k;
+  }
+  if (o.invalid) return o;
+  const d4 nv = o.nv;
+  d4 ve{madd<ARITH>(h, s4.x, vel.x), madd<ARITH>(h, s4.y, vel.y), madd<ARITH>(h, s4.z, vel.z),
+        madd<ARITH>(h, s4.w, vel.w)};
+  double dx = nv.x - ve.x, dy = nv.y - ve.y, dz = nv.z - ve.z, dw = nv.w - ve.w;
+  if (S.err_mode == 1) {
+    double s0 = dx / (S.atol + S.rtol * fmax(fabs(vel.x), fabs(nv.x)));
+    double s1 = dy / (S.atol + S.rtol * fmax(fabs(vel.y), fabs(nv.y)));
+    double s2 = dz / (S.atol + S.rtol * fmax(fabs(vel.z), fabs(nv.z)));
+    double s3 = dw / (S.atol + S.rtol * fmax(fabs(vel.w), fabs(nv.w)));
+    double sq = 0.0;
+    sq = sq + s0 * s0; sq = sq + s1 * s1; sq = sq + s2 * s2; sq = sq + s3 * s3;
+    o.err = sqrt(0.25 * sq);
+    o.thresh = 1.0;
+  } else {
+    // nalgebra Vector4::norm: sqrt((x0^2 + x2^2) + (x1^2 + x3^2))  (:308)
+    double a = dx * dx, b = dy * dy, c = dz * dz, e = dw * dw;
+    a = a + c;
+    b = b + e;
+    o.err = sqrt(a + b);
+    o.thresh = S.tol;
+  }
+  return o;
+}
+
+// One RK45Integrator::step (geodesic.rs:185-328) on lane state L with the
+// stage values in LDS.  PREC 0: exact FP64; 1: mixed (FP32 stages where the
+// controller is saturated, exact FP64 otherwise, so that every decision and
+// every FP64 step match the FP64 kernel); 2: fast FP64.  Returns RES_*;
+// `acc` reports acceptance; p0/v0 keep the pre-step position and velocity.
+template <int METRIC, int PREC>
+__device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool &acc, double p0[4], d4 &v0) {
+#pragma clang fp contract(off)
+  constexpr int BASE = (PREC == 2) ? ARITH_FAST : ARITH_EXACT;
   acc = false;
   const double r = L.pos[1];
   if (r > S.max_radius) return RES_ESCAPED;          // geodesic.rs:194-196
   if (r < S.r_capture) return RES_HORIZON;           // :198-202
   if (r < S.r_h) return RES_HORIZON;                 // :204-206
-  if (!L.k1_ok) {
-    L.k1 = rhs64<METRIC>(S, L.pos[1], L.pos[2], L.vel);
+  const d4 vel = L.vel;
+  for (int i = 0; i < 4; ++i) p0[i] = L.pos[i];
+  v0 = vel;
+  if (!L.k1_ok) {                                     // k1 (:245); otherwise k7 of the last accepted step
+    K.put(0, eval_rhs<METRIC, BASE>(S, p0[1], p0[2], vel));
     L.k1_ok = true;
     L.n_rhs++;
   }
-  const d4 vel = L.vel;
-  const d4 k1 = L.k1;
-  for (int i = 0; i < 4; ++i) p0[i] = L.pos[i];
-  v0 = vel;
+  const double h = L.h;
 
   bool use32 = false;
   if (PREC == 1) {
-    // FP32 only where the controller is pinned at max_step with margin.
-    double noise;
-    if (S.err_mode == 1) {
-      double m0 = fabs(k1.x) / (S.atol + S.rtol * fabs(vel.x));
-      double m1 = fabs(k1.y) / (S.atol + S.rtol * fabs(vel.y));
-      double m2 = fabs(k1.z) / (S.atol + S.rtol * fabs(vel.z));
-      double m3 = fabs(k1.w) / (S.atol + S.rtol * fabs(vel.w));
-      noise = fmax(fmax(m0, m1), fmax(m2, m3));
-    } else {
-      noise = (fabs(k1.x) + fabs(k1.y) + fabs(k1.z) + fabs(k1.w)) / S.tol;
-    }
-    noise *= 1e-6 * L.h;
-    const double th = L.pos[2];
-    use32 = (L.h == S.max_step) && (L.err_prev < 0.3) && (noise < 0.05) && (r >= S.fp32_min_r) &&
-            (th > 1e-3) && (th < PI - 1e-3);
+    // Outgoing photons beyond the photon-orbit region never turn back, so
+    // FP32 rounding on them is not amplified by a later close approach;
+    // inbound rays only take FP32 steps far out.
+    const double th = p0[2];
+    bool far_enough = (vel.y > 0.0) ? (r >= S.fp32_min_r) : (r >= S.fp32_far_r);
+    use32 = far_enough && (h == S.max_step) && (L.err_prev < 0.3) && (th > 1e-3) && (th < PI - 1e-3) &&
+            (fp32_noise(S, p0, vel, h) < 0.02f);
   }
-
-  for (int pass = 0; pass < 2; ++pass) {
-    const double h = L.h;
-    h_used = h;
-    double q0, q1, q2, q3;
-    d4 k2, k3, k4, k5, k6, k7, vs;
-    int nr = 0;
-    // stage 2 (geodesic.rs:248-254)
-    {
-      double ha = h * A2;
-      q0 = p0[0] + ha * vel.x; q1 = p0[1] + ha * vel.y; q2 = p0[2] + ha * vel.z; q3 = p0[3] + ha * vel.w;
-      if (!valid4(q0, q1, q2, q3)) { L.h = h * 0.5; L.n_rhs += nr; return RES_CONTINUE; }
-      double hb = h * B21;
-      vs.x = vel.x + hb * k1.x; vs.y = vel.y + hb * k1.y; vs.z = vel.z + hb * k1.z; vs.w = vel.w + hb * k1.w;
-      k2 = eval_rhs<METRIC>(S, use32, q1, q2, vs); ++nr;
-    }
-    // stage 3 (:257-263)
-    {
-      double ha = h * A3;
-      q0 = p0[0] + ha * vel.x; q1 = p0[1] + ha * vel.y; q2 = p0[2] + ha * vel.z; q3 = p0[3] + ha * vel.w;
-      if (!valid4(q0, q1, q2, q3)) { L.h = h * 0.5; L.n_rhs += nr; return RES_CONTINUE; }
-      vs.x = vel.x + h * (B31 * k1.x + B32 * k2.x);
-      vs.y = vel.y + h * (B31 * k1.y + B32 * k2.y);
-      vs.z = vel.z + h * (B31 * k1.z + B32 * k2.z);
-      vs.w = vel.w + h * (B31 * k1.w + B32 * k2.w);
-      k3 = eval_rhs<METRIC>(S, use32, q1, q2, vs); ++nr;
-    }
-    // stage 4 (:266-272)
-    {
-      double ha = h * A4;
-      q0 = p0[0] + ha * vel.x; q1 = p0[1] + ha * vel.y; q2 = p0[2] + ha * vel.z; q3 = p0[3] + ha * vel.w;
-      if (!valid4(q0, q1, q2, q3)) { L.h = h * 0.5; L.n_rhs += nr; return RES_CONTINUE; }
-      vs.x = vel.x + h * (B41 * k1.x + B42 * k2.x + B43 * k3.x);
-      vs.y = vel.y + h * (B41 * k1.y + B42 * k2.y + B43 * k3.y);
-      vs.z = vel.z + h * (B41 * k1.z + B42 * k2.z + B43 * k3.z);
-      vs.w = vel.w + h * (B41 * k1.w + B42 * k2.w + B43 * k3.w);
-      k4 = eval_rhs<METRIC>(S, use32, q1, q2, vs); ++nr;
-    }
-    // stage 5 (:275-281)
-    {
-      double ha = h * A5;
-      q0 = p0[0] + ha * vel.x; q1 = p0[1] + ha * vel.y; q2 = p0[2] + ha * vel.z; q3 = p0[3] + ha * vel.w;
-      if (!valid4(q0, q1, q2, q3)) { L.h = h * 0.5; L.n_rhs += nr; return RES_CONTINUE; }
-      vs.x = vel.x + h * (B51 * k1.x + B52 * k2.x + B53 * k3.x + B54 * k4.x);
-      vs.y = vel.y + h * (B51 * k1.y + B52 * k2.y + B53 * k3.y + B54 * k4.y);
-      vs.z = vel.z + h * (B51 * k1.z + B52 * k2.z + B53 * k3.z + B54 * k4.z);
-      vs.w = vel.w + h * (B51 * k1.w + B52 * k2.w + B53 * k3.w + B54 * k4.w);
-      k5 = eval_rhs<METRIC>(S, use32, q1, q2, vs); ++nr;
-    }
-    // stage 6 (:284-291): pos6 = pos + h vel is also the accepted position
-    double p6[4];
-    {
-      p6[0] = p0[0] + h * vel.x; p6[1] = p0[1] + h * vel.y; p6[2] = p0[2] + h * vel.z; p6[3] = p0[3] + h * vel.w;
-      if (!valid4(p6[0], p6[1], p6[2], p6[3])) { L.h = h * 0.5; L.n_rhs += nr; return RES_CONTINUE; }
-      vs.x = vel.x + h * (B61 * k1.x + B62 * k2.x + B63 * k3.x + B64 * k4.x + B65 * k5.x);
-      vs.y = vel.y + h * (B61 * k1.y + B62 * k2.y + B63 * k3.y + B64 * k4.y + B65 * k5.y);
-      vs.z = vel.z + h * (B61 * k1.z + B62 * k2.z + B63 * k3.z + B64 * k4.z + B65 * k5.z);
-      vs.w = vel.w + h * (B61 * k1.w + B62 * k2.w + B63 * k3.w + B64 * k4.w + B65 * k5.w);
-      k6 = eval_rhs<METRIC>(S, use32, p6[1], p6[2], vs); ++nr;
-    }
-    // 5th-order velocity (:294-295), k7 at (pos6, new_vel) (:298)
-    d4 nv;
-    nv.x = vel.x + h * (C1 * k1.x + C3 * k3.x + C4 * k4.x + C5 * k5.x + C6 * k6.x);
-    nv.y = vel.y + h * (C1 * k1.y + C3 * k3.y + C4 * k4.y + C5 * k5.y + C6 * k6.y);
-    nv.z = vel.z + h * (C1 * k1.z + C3 * k3.z + C4 * k4.z + C5 * k5.z + C6 * k6.z);
-    nv.w = vel.w + h * (C1 * k1.w + C3 * k3.w + C4 * k4.w + C5 * k5.w + C6 * k6.w);
-    k7 = eval_rhs<METRIC>(S, use32, p6[1], p6[2], nv); ++nr;
-    // 4th-order velocity (:299-305) and the error (:308)
-    d4 ve;
-    ve.x = vel.x + h * (D1 * k1.x + D3 * k3.x + D4 * k4.x + D5 * k5.x + D6 * k6.x + D7 * k7.x);
-    ve.y = vel.y + h * (D1 * k1.y + D3 * k3.y + D4 * k4.y + D5 * k5.y + D6 * k6.y + D7 * k7.y);
-    ve.z = vel.z + h * (D1 * k1.z + D3 * k3.z + D4 * k4.z + D5 * k5.z + D6 * k6.z + D7 * k7.z);
-    ve.w = vel.w + h * (D1 * k1.w + D3 * k3.w + D4 * k4.w + D5 * k5.w + D6 * k6.w + D7 * k7.w);
-    double dx = nv.x - ve.x, dy = nv.y - ve.y, dz = nv.z - ve.z, dw = nv.w - ve.w;
-    double err, thresh;
-    if (S.err_mode == 1) {
-      double s0 = dx / (S.atol + S.rtol * fmax(fabs(vel.x), fabs(nv.x)));
-      double s1 = dy / (S.atol + S.rtol * fmax(fabs(vel.y), fabs(nv.y)));
-      double s2 = dz / (S.atol + S.rtol * fmax(fabs(vel.z), fabs(nv.z)));
-      double s3 = dw / (S.atol + S.rtol * fmax(fabs(vel.w), fabs(nv.w)));
-      double s = 0.0;
-      s = s + s0 * s0; s = s + s1 * s1; s = s + s2 * s2; s = s + s3 * s3;
-      err = sqrt(0.25 * s);
-      thresh = 1.0;
-    } else {
-      // nalgebra Vector4::norm: sqrt((x0^2 + x2^2) + (x1^2 + x3^2))
-      double a = dx * dx, b = dy * dy, c = dz * dz, e = dw * dw;
-      a = a + c;
-      b = b + e;
-      err = sqrt(a + b);
-      thresh = S.tol;
-    }
-    L.n_rhs += nr;
-    if (use32) L.n_rhs32 += nr;
-    double ratio = err / thresh;
-    if (PREC == 1 && use32 && !(ratio < 0.4)) {  // FP32 result too close to a decision: redo in FP64
-      use32 = false;
-      continue;
-    }
-    L.n_full++;
-    L.err_prev = ratio;
-    if (err < thresh || h <= S.min_step) {           // accept (:310-321)
-      L.pos[0] = p6[0]; L.pos[1] = p6[1]; L.pos[2] = p6[2]; L.pos[3] = p6[3];
-      L.vel = nv;
-      L.lambda = L.lambda + h;
-      // normalize_spherical_coords (:337-352)
-      bool refl = false;
-      if (L.pos[2] < 0.0) { L.pos[2] = -L.pos[2]; L.pos[3] = L.pos[3] + PI; refl = true; }
-      if (L.pos[2] > PI) { L.pos[2] = 2.0 * PI - L.pos[2]; L.pos[3] = L.pos[3] + PI; refl = true; }
-      double ph = fmod(L.pos[3], TWO_PI);
-      L.pos[3] = ph < 0.0 ? ph + TWO_PI : ph;
-      L.k1 = k7;
-      L.k1_ok = !refl;
-      if (err > 0.0) {
-        double hn = h * 0.9 * pow(thresh / err, 0.2);
-        if (hn < S.min_step) hn = S.min_step;
-        if (hn > S.max_step) hn = S.max_step;
-        L.h = hn;
-      }
-      acc = true;
-    } else {                                          // reject (:322-325)
-      double hn = h * 0.9 * pow(thresh / err, 0.25);
-      L.h = hn >= S.min_step ? hn : S.min_step;       // f64::max
-    }
-    return RES_CONTINUE;
+  StageOut o;
+  bool done = false;
+  if (PREC == 1 && use32) {
+    o = stages<METRIC, ARITH_F32>(S, K, p0, vel, h);
+    L.n_rhs += o.nr;
+    L.n_rhs32 += o.nr;
+    // FP32 result near a decision (or an invalid stage): redo in FP64
+    done = !o.invalid && (o.err / o.thresh < 0.4);
   }
-  return RES_CONTINUE;  // not reached
+  if (!done) {
+    o = stages<METRIC, BASE>(S, K, p0, vel, h);
+    L.n_rhs += o.nr;
+  }
+  if (o.invalid) { L.h = h * 0.5; return RES_CONTINUE; }
+  const double err = o.err, thresh = o.thresh;
+  L.n_full++;
+  L.err_prev = err / thresh;
+  if (err < thresh || h <= S.min_step) {           // accept (:310-321)
+    L.pos[0] = madd<BASE>(h, vel.x, p0[0]); L.pos[1] = madd<BASE>(h, vel.y, p0[1]);
+    L.pos[2] = madd<BASE>(h, vel.z, p0[2]); L.pos[3] = madd<BASE>(h, vel.w, p0[3]);
+    L.vel = o.nv;
+    L.lambda = L.lambda + h;
+    // normalize_spherical_coords (:337-352)
+    bool refl = false;
+    if (L.pos[2] < 0.0) { L.pos[2] = -L.pos[2]; L.pos[3] = L.pos[3] + PI; refl = true; }
+    if (L.pos[2] > PI) { L.pos[2] = 2.0 * PI - L.pos[2]; L.pos[3] = L.pos[3] + PI; refl = true; }
+    double ph = fmod(L.pos[3], TWO_PI);
+    L.pos[3] = ph < 0.0 ? ph + TWO_PI : ph;
+    K.put(0, o.k7);                                 // FSAL: k7 is the next k1 ...
+    L.k1_ok = !refl;                                // ... unless theta was reflected
+    if (err > 0.0) {
+      double hn = h * 0.9 * pow(thresh / err, 0.2);
+      if (hn < S.min_step) hn = S.min_step;
+      if (hn > S.max_step) hn = S.max_step;
+      L.h = hn;
+    }
+    acc = true;
+  } else {                                          // reject (:322-325)
+    double hn = h * 0.9 * pow(thresh / err, 0.25);
+    L.h = hn >= S.min_step ? hn : S.min_step;       // f64::max
+  }
+  return RES_CONTINUE;
 }
 
 __device__ inline void ray_pixel(const DevScene &S, uint64_t local, uint32_t &px, uint32_t &py) {
@@ -273,12 +341,17 @@ __device__ inline unsigned long long wave_sum(unsigned long long x) {
 // BATCH 0: one scene, its constants wavefront-uniform (scalar registers);
 // BATCH 1: each lane looks up the scene of its ray.
 template <int METRIC, int DISK, int PREC, int SRC, int BATCH>
-__global__ __launch_bounds__(256) void trace_kernel(TraceLaunch T) {
+#ifndef GG_MIN_WAVES
+#define GG_MIN_WAVES 2
+#endif
+__global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T) {
 #pragma clang fp contract(off)
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t n = T.n_rays;
   constexpr bool single = (BATCH == 0);
 
+  extern __shared__ double lds_k[];
+  const KStore K{lds_k + threadIdx.x, blockDim.x};
   Lane L;
   L.n_rhs = 0; L.n_rhs32 = 0; L.n_full = 0;
   uint32_t n_att = 0;
@@ -287,8 +360,10 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceLaunch T) {
   uint32_t si = 0;        // scene of this lane's ray
   uint64_t out_i = 0;     // output slot of this lane's ray
   bool exhausted = false;
-  // Per-lane scene copy: uniform (SGPR) for single-scene launches.
-  DevScene S = T.scenes[0];
+  // Scene constants are read through the pointer where they are used, so
+  // the compiler reloads them (scalar loads for a single scene) instead of
+  // holding ~75 doubles live across the persistent loop.
+  const DevScene *__restrict__ Sp = T.scenes;
 
   uint32_t iter = 0;
   for (;;) {
@@ -308,13 +383,13 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceLaunch T) {
           ray = base + rank;
           if (ray < n) {
             si = single ? 0u : find_scene(T.scenes, T.n_scenes, ray);
-            if (!single) S = T.scenes[si];
-            uint64_t local = ray - S.ray_begin;
+            if (!single) Sp = T.scenes + si;
+            uint64_t local = ray - Sp->ray_begin;
             if (SRC == 0) {
               uint32_t px, py;
-              ray_pixel(S, local, px, py);
-              out_i = (uint64_t)py * S.sub_w + px;
-              camera_ray(S, px, py, L.pos, L.vel);
+              ray_pixel(*Sp, local, px, py);
+              out_i = (uint64_t)py * Sp->sub_w + px;
+              camera_ray(*Sp, px, py, L.pos, L.vel);
               L.lambda = 0.0;
             } else {
               out_i = local;
@@ -323,7 +398,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceLaunch T) {
               L.vel = d4{st[4], st[5], st[6], st[7]};
               L.lambda = st[8];
             }
-            L.h = S.h0;
+            L.h = Sp->h0;
             L.k1_ok = false;
             L.attempts = 0; L.accepted = 0;
             L.intensity = 0.0; L.transmission = 1.0;
@@ -343,15 +418,14 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceLaunch T) {
 
     // ---- one step() call ----
     uint8_t cls = 255;
-    if (L.attempts >= S.max_steps) {
+    if (L.attempts >= Sp->max_steps) {
       cls = GG_CLASS_MAX_STEPS;
     } else {
       bool acc;
       double p0[4];
       d4 v0;
-      double h_used;
       double lam0 = L.lambda;
-      int res = attempt<METRIC, PREC>(S, L, acc, p0, v0, h_used);
+      int res = attempt<METRIC, PREC>(*Sp, L, K, acc, p0, v0);
       L.attempts++;
       n_att++;
       if (acc) L.accepted++;
@@ -362,15 +436,15 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceLaunch T) {
         if (dl > 0.0) {
           double mr = 0.5 * (p0[1] + L.pos[1]);
           double mth = 0.5 * (p0[2] + L.pos[2]);
-          double rho = disk_density(S, mr, mth);
+          double rho = disk_density(*Sp, mr, mth);
           if (rho > 0.0) {
-            double j = disk_emission(S, rho, mr, mth);
+            double j = disk_emission(*Sp, rho, mr, mth);
             d4 pm{0.5 * (v0.x + L.vel.x), 0.5 * (v0.y + L.vel.y), 0.5 * (v0.z + L.vel.z), 0.5 * (v0.w + L.vel.w)};
-            double g = redshift64<METRIC>(S, mr, mth, pm);
+            double g = redshift64<METRIC>(*Sp, mr, mth, pm);
             double g2 = g * g;
             double red = j * (g2 * g2);
             L.intensity = L.intensity + red * dl * L.transmission;
-            L.transmission *= exp(-rho * dl * S.absorption);
+            L.transmission *= exp(-rho * dl * Sp->absorption);
             if (L.transmission < 1e-3) cls = GG_CLASS_DISK;
           }
         }
@@ -385,16 +459,16 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceLaunch T) {
           if (side0 != side1) {
             double s = (HALF_PI - p0[2]) / (L.pos[2] - p0[2]);
             double rc = p0[1] + s * (L.pos[1] - p0[1]);
-            double w = S.edge_width;
-            bool hit = (w > 0.0) ? (rc <= S.r_out + 8.0 * w && rc > 0.0) : (rc >= S.r_in && rc <= S.r_out);
+            double w = Sp->edge_width;
+            bool hit = (w > 0.0) ? (rc <= Sp->r_out + 8.0 * w && rc > 0.0) : (rc >= Sp->r_in && rc <= Sp->r_out);
             if (hit) {
               d4 vc{v0.x + s * (L.vel.x - v0.x), v0.y + s * (L.vel.y - v0.y), v0.z + s * (L.vel.z - v0.z),
                     v0.w + s * (L.vel.w - v0.w)};
-              double g = redshift64<METRIC>(S, rc, HALF_PI, vc);
+              double g = redshift64<METRIC>(*Sp, rc, HALF_PI, vc);
               double g2 = g * g;
-              double I = pow(rc, -S.q) * (g2 * g2);
+              double I = pow(rc, -Sp->q) * (g2 * g2);
               if (w > 0.0) {
-                I = I * (1.0 / (1.0 + exp(-((rc - S.r_in) / w)))) * (1.0 / (1.0 + exp(-((S.r_out - rc) / w))));
+                I = I * (1.0 / (1.0 + exp(-((rc - Sp->r_in) / w)))) * (1.0 / (1.0 + exp(-((Sp->r_out - rc) / w))));
               }
               L.intensity = I;
               cls = GG_CLASS_DISK;
@@ -410,7 +484,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceLaunch T) {
     if (cls == GG_CLASS_MAX_STEPS && DISK == 1 && L.intensity > 0.0) cls = GG_CLASS_DISK;
     if (cls != 255) {
       // retire: structure-of-arrays stores of the final state
-      const DevOut &O = T.outs[single ? 0 : S.out_index];
+      const DevOut &O = T.outs[single ? 0 : Sp->out_index];
       if (O.cls) O.cls[out_i] = cls;
       if (O.r) O.r[out_i] = L.pos[1];
       if (O.theta) O.theta[out_i] = L.pos[2];
@@ -454,12 +528,9 @@ __global__ void rhs_kernel(const DevScene *S, uint64_t n, const double *pos, con
   if (i >= n) return;
   d4 v{vel[4 * i], vel[4 * i + 1], vel[4 * i + 2], vel[4 * i + 3]};
   d4 a;
-  if (PREC == 1) {
-    f4 vf{(float)v.x, (float)v.y, (float)v.z, (float)v.w};
-    a = to64(rhs32<METRIC>(*S, (float)pos[4 * i + 1], (float)pos[4 * i + 2], vf));
-  } else {
-    a = rhs64<METRIC>(*S, pos[4 * i + 1], pos[4 * i + 2], v);
-  }
+  if (PREC == 1) a = eval_rhs<METRIC, ARITH_F32>(*S, pos[4 * i + 1], pos[4 * i + 2], v);
+  else if (PREC == 2) a = eval_rhs<METRIC, ARITH_FAST>(*S, pos[4 * i + 1], pos[4 * i + 2], v);
+  else a = eval_rhs<METRIC, ARITH_EXACT>(*S, pos[4 * i + 1], pos[4 * i + 2], v);
   acc[4 * i] = a.x; acc[4 * i + 1] = a.y; acc[4 * i + 2] = a.z; acc[4 * i + 3] = a.w;
 }
 
@@ -528,20 +599,25 @@ __global__ void shade_kernel(const DevScene *Sp, const DevOut *Op, uint32_t widt
   for (int c = 0; c < 3; ++c) rgb[3 * (uint64_t)i + c] = to_u8(acc[c] * inv);
 }
 
+// k1..k5 of every lane: 5 slots x 4 components x 8 bytes.
+inline size_t lds_bytes(uint32_t block) { return (size_t)block * 5 * 4 * sizeof(double); }
+
 template <int M, int D, int P>
 hipError_t launch3(const TraceLaunch &t, uint32_t grid, uint32_t block, hipStream_t s) {
   if (t.states_in)
-    hipLaunchKernelGGL((trace_kernel<M, D, P, 1, 0>), dim3(grid), dim3(block), 0, s, t);
+    hipLaunchKernelGGL((trace_kernel<M, D, P, 1, 0>), dim3(grid), dim3(block), lds_bytes(block), s, t);
   else if (t.n_scenes > 1)
-    hipLaunchKernelGGL((trace_kernel<M, D, P, 0, 1>), dim3(grid), dim3(block), 0, s, t);
+    hipLaunchKernelGGL((trace_kernel<M, D, P, 0, 1>), dim3(grid), dim3(block), lds_bytes(block), s, t);
   else
-    hipLaunchKernelGGL((trace_kernel<M, D, P, 0, 0>), dim3(grid), dim3(block), 0, s, t);
+    hipLaunchKernelGGL((trace_kernel<M, D, P, 0, 0>), dim3(grid), dim3(block), lds_bytes(block), s, t);
   return hipGetLastError();
 }
 
 template <int M, int D>
 hipError_t launch2(int prec, const TraceLaunch &t, uint32_t grid, uint32_t block, hipStream_t s) {
-  return prec == 1 ? launch3<M, D, 1>(t, grid, block, s) : launch3<M, D, 0>(t, grid, block, s);
+  if (prec == 1) return launch3<M, D, 1>(t, grid, block, s);
+  if (prec == 2) return launch3<M, D, 2>(t, grid, block, s);
+  return launch3<M, D, 0>(t, grid, block, s);
 }
 
 template <int M>
@@ -570,13 +646,13 @@ hipError_t launch_rhs(int metric_kind, int precision, const DevScene *scene, uin
                       const double *vel, double *acc, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   uint32_t grid = (uint32_t)((n + 255) / 256);
+#define GG_RHS_LAUNCH(M, P) hipLaunchKernelGGL((rhs_kernel<M, P>), dim3(grid), dim3(256), 0, stream, scene, n, pos, vel, acc)
   if (metric_kind == 1) {
-    if (precision == 1) hipLaunchKernelGGL((rhs_kernel<1, 1>), dim3(grid), dim3(256), 0, stream, scene, n, pos, vel, acc);
-    else hipLaunchKernelGGL((rhs_kernel<1, 0>), dim3(grid), dim3(256), 0, stream, scene, n, pos, vel, acc);
+    if (precision == 1) GG_RHS_LAUNCH(1, 1); else if (precision == 2) GG_RHS_LAUNCH(1, 2); else GG_RHS_LAUNCH(1, 0);
   } else {
-    if (precision == 1) hipLaunchKernelGGL((rhs_kernel<0, 1>), dim3(grid), dim3(256), 0, stream, scene, n, pos, vel, acc);
-    else hipLaunchKernelGGL((rhs_kernel<0, 0>), dim3(grid), dim3(256), 0, stream, scene, n, pos, vel, acc);
+    if (precision == 1) GG_RHS_LAUNCH(0, 1); else if (precision == 2) GG_RHS_LAUNCH(0, 2); else GG_RHS_LAUNCH(0, 0);
   }
+#undef GG_RHS_LAUNCH
   return hipGetLastError();
 }
 

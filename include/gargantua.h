@@ -141,8 +141,11 @@ typedef struct gg_scene {
 } gg_scene;
 
 typedef enum gg_precision {
-  GG_PREC_FP64 = 0,   /* every operation in FP64, reference expression order  */
-  GG_PREC_MIXED = 1   /* far-field RHS stages in FP32, everything else FP64   */
+  GG_PREC_FP64 = 0,       /* every operation in FP64, reference expression order */
+  GG_PREC_MIXED = 1,      /* RHS stages in FP32 where the step-size controller is
+                             saturated and FP32 rounding cannot change a decision;
+                             fast FP64 elsewhere                                  */
+  GG_PREC_FP64_FAST = 2   /* FP64 with reciprocal-based RHS algebra and FMA      */
 } gg_precision;
 
 /* Ray classes (RayOutcome, crates/raytracer/src/curved/outcome.rs:3-22;

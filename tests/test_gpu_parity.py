@@ -106,6 +106,8 @@ def test_kerr_1024_subsample_matches_oracle(renderer, oracle):
 
 
 def test_mixed_matches_fp64(renderer):
+    """The mixed kernel keeps every decision and every FP64 step of the FP64
+    kernel, so final states agree to 1e-6 on (nearly) every ray."""
     scene = gg.configs.kerr_1024(256, 256)
     a = renderer.render(scene, "fp64")
     b = renderer.render(scene, "mixed")
@@ -113,5 +115,34 @@ def test_mixed_matches_fp64(renderer):
     frac32 = b["stats"]["rhs_evals_fp32"] / max(1, b["stats"]["rhs_evals"])
     print(f"mixed vs fp64: {c}, fp32 share of RHS {frac32:.3f}")
     assert c["class_agree"] >= 0.999
-    assert c["err_le_1e-6"] >= 0.99
+    assert c["class_agree_outside_band"] == 1.0
+    assert c["err_le_1e-6"] >= 0.999
+    assert c["attempts_equal"] >= 0.999
     assert frac32 > 0.3
+
+
+def test_fast_fp64_matches_fp64(renderer):
+    """The fast FP64 kernel rounds differently; the reference controller keeps h
+    unchanged when the error rounds to exactly zero (geodesic.rs:318), so on
+    a few percent of rays the step sequence shifts and the escape crossing
+    moves by up to one step: classes and disk images still agree."""
+    scene = gg.configs.kerr_1024(256, 256)
+    a = renderer.render(scene, "fp64")
+    b = renderer.render(scene, "fp64fast")
+    c = compare(b, a, shape=(256, 256))
+    print(f"fp64fast vs fp64: {c}")
+    assert c["class_agree"] >= 0.999
+    assert c["class_agree_outside_band"] == 1.0
+    assert c["err_le_1e-6"] >= 0.95
+
+
+@pytest.mark.parametrize("metric", [gg.Schwarzschild(1.0), gg.Kerr(1.0, 0.9), gg.Kerr(1.0, 0.998)])
+def test_rhs_fast_fp64_accuracy(renderer, oracle, metric):
+    rng = np.random.default_rng(4)
+    pos, vel = _points(rng, 4096)
+    a = renderer.geodesic_acceleration(metric, pos, vel, "fp64fast")
+    b = oracle.geodesic_acceleration(metric, pos, vel)
+    scale = np.max(np.abs(b), axis=1, keepdims=True)
+    rel = np.max(np.abs(a - b) / scale, axis=1)
+    print(f"{metric}: fast fp64 rhs max rel {rel.max():.2e}")
+    assert rel.max() < 1e-12
