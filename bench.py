@@ -76,7 +76,7 @@ def flops(stats, metric_kind):
     return f32, f64
 
 
-def cpu_baseline(scene, stride=8):
+def cpu_baseline(scene, stride=2):
     """FP64 oracle on every `stride`-th row and column of the same rays."""
     from oracle import Oracle
     w = scene.width
@@ -92,7 +92,7 @@ def cpu_baseline(scene, stride=8):
     ora.render_rays(scene, idx, state=False)
     dt = time.perf_counter() - t0
     one = Oracle(threads=1)
-    sub = idx[:: max(1, len(idx) // 1024)]
+    sub = idx[:: max(1, len(idx) // 2048)]
     t1 = time.perf_counter()
     one.render_rays(scene, sub, state=False)
     dt1 = time.perf_counter() - t1
@@ -128,13 +128,16 @@ def main():
 
     wall_m, kms_m, st_m = time_precision(torch, rend, scene, "mixed", args.steps, args.warmup, outs_m)
     wall_d, kms_d, st_d = time_precision(torch, rend, scene, "fp64", args.steps, args.warmup, outs_d)
+    outs_f = _dev_outputs(torch, n)
+    wall_f, kms_f, st_f = time_precision(torch, rend, scene, "fp64fast", args.steps, args.warmup, outs_f)
 
     # accuracy of the mixed kernel against the FP64 kernel (same inputs)
     host = lambda o: {k: v.cpu().numpy() for k, v in o.items()}
-    hm, hd = host(outs_m), host(outs_d)
-    for h in (hm, hd):
+    hm, hd, hf = host(outs_m), host(outs_d), host(outs_f)
+    for h in (hm, hd, hf):
         h["attempts"] = h["attempts"].astype(np.uint32)
     cmp = compare(hm, hd, shape=(scene.height, scene.width))
+    cmp_f = compare(hf, hd, shape=(scene.height, scene.width))
 
     value = n * args.steps / wall_m
     ms_step = 1e3 * wall_m / args.steps
@@ -159,8 +162,11 @@ def main():
         "fp64": {"value": n * args.steps / wall_d, "kernel_ms": k_ms_d,
                  "roofline_frac": (d32 + d64) / (k_ms_d * 1e-3) / 1e12 / PEAK_FP64,
                  "mixed_speedup": k_ms_d / k_ms},
+        "fp64fast": {"value": n * args.steps / wall_f, "kernel_ms": float(np.mean(kms_f)),
+                     "vs_fp64": {k: cmp_f[k] for k in ("class_agree", "err_le_1e-6", "attempts_equal")}},
         "mixed_vs_fp64": {k: cmp[k] for k in ("class_agree", "class_agree_outside_band", "err_le_1e-6",
-                                             "err_le_1e-6_outside_band", "psnr_intensity", "state_bit_identical")},
+                                             "err_le_1e-6_outside_band", "attempts_equal", "psnr_intensity",
+                                             "state_bit_identical")},
         "work": {"attempts": st_m["attempts"], "rhs_evals": st_m["rhs_evals"], "rhs_evals_fp32": st_m["rhs_evals_fp32"],
                  "attempts_per_ray": st_m["attempts"] / n},
         "cpu_baseline": None,
