@@ -167,6 +167,29 @@ class Scene:
     def n_rays(self) -> int:
         return self.width * self.samples_per_axis * self.height * self.samples_per_axis
 
+    def to_dict(self) -> dict:
+        """Plain-JSON form of the scene (see from_dict)."""
+        from dataclasses import asdict
+        m = self.metric
+        return {"metric": {"kind": "kerr" if isinstance(m, Kerr) else "schwarzschild", **asdict(m)},
+                "camera": {k: list(v) if isinstance(v, (tuple, list)) else v for k, v in asdict(self.camera).items()},
+                "disk": asdict(self.disk) if self.disk is not None else None,
+                "integrator": asdict(self.integrator), "width": self.width, "height": self.height,
+                "samples_per_axis": self.samples_per_axis, "jitter": self.jitter, "seed": self.seed}
+
+    @staticmethod
+    def from_dict(d: dict) -> "Scene":
+        md = dict(d["metric"])
+        kind = md.pop("kind")
+        metric = Kerr(**md) if kind == "kerr" else Schwarzschild(**md)
+        cam = d["camera"]
+        return Scene(metric=metric,
+                     camera=Camera(position=tuple(cam["position"]), look_at=tuple(cam["look_at"]), up=tuple(cam["up"]),
+                                   fov_y_radians=cam["fov_y_radians"], aspect=cam["aspect"]),
+                     disk=AccretionDisk(**d["disk"]) if d["disk"] is not None else None,
+                     integrator=RK45Integrator(**d["integrator"]), width=d["width"], height=d["height"],
+                     samples_per_axis=d["samples_per_axis"], jitter=d["jitter"], seed=d["seed"])
+
 
 # --------------------------------------------------------------- renderer --
 
