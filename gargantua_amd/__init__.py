@@ -256,8 +256,8 @@ class Renderer:
         except Exception:
             pass
 
-    def set_launch(self, block_threads=0, blocks_per_cu=0, refill_interval=0):
-        L = _abi.gg_launch(block_threads, blocks_per_cu, refill_interval)
+    def set_launch(self, block_threads=0, blocks_per_cu=0, refill_interval=0, graph=0):
+        L = _abi.gg_launch(block_threads, blocks_per_cu, refill_interval, graph)
         _check(self.lib, self.lib.gg_set_launch(self.ctx, C.byref(L)), "gg_set_launch")
 
     def get_launch(self) -> dict:
@@ -286,6 +286,20 @@ class Renderer:
         _check(self.lib, self.lib.gg_render_batch(self.ctx, arr, len(scenes), PRECISIONS[precision], oarr,
                                                   C.byref(st)), "gg_render_batch")
         return outs, stats_dict(st)
+
+    def autotune(self, scenes: Sequence[Scene], precision: str = "mixed", candidates=None) -> tuple:
+        """Time the batch under candidate launch shapes; install and return the fastest."""
+        arr = (_abi.gg_scene * len(scenes))(*[s.to_c() for s in scenes])
+        if candidates:
+            cands = (_abi.gg_launch * len(candidates))(*[_abi.gg_launch(*c) for c in candidates])
+            nc = len(candidates)
+        else:
+            cands, nc = None, 0
+        best = _abi.gg_launch()
+        ms = C.c_double()
+        _check(self.lib, self.lib.gg_autotune(self.ctx, arr, len(scenes), PRECISIONS[precision], cands, nc,
+                                              C.byref(best), C.byref(ms)), "gg_autotune")
+        return {k: getattr(best, k) for k, _ in _abi.gg_launch._fields_}, ms.value
 
     def trace_states(self, scene: Scene, states: np.ndarray, precision: str = "fp64") -> dict:
         """trace_ray / trace_ray_with_disk from given states [n, 9] (t r th ph vt vr vth vph lambda)."""

@@ -73,7 +73,7 @@ class gg_stats(C.Structure):
 
 class gg_launch(C.Structure):
     _fields_ = [("block_threads", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("refill_interval", C.c_uint32)]
+                ("refill_interval", C.c_uint32), ("graph", C.c_uint32)]
 
 
 class gg_grad_scene(C.Structure):
@@ -100,6 +100,8 @@ EXPORTS = [
     ("gg_trace_states", C.c_int, [_P, C.POINTER(gg_scene), C.c_int, C.c_uint64, _P, C.POINTER(gg_outputs),
                                   C.POINTER(gg_stats)]),
     ("gg_geodesic_acceleration", C.c_int, [_P, C.POINTER(gg_metric), C.c_int, C.c_uint64, _P, _P, _P]),
+    ("gg_autotune", C.c_int, [_P, C.POINTER(gg_scene), C.c_uint32, C.c_int, C.POINTER(gg_launch), C.c_uint32,
+                              C.POINTER(gg_launch), C.POINTER(C.c_double)]),
     ("gg_grad", C.c_int, [_P, C.POINTER(gg_grad_scene), _P, C.c_uint32, C.POINTER(C.c_double),
                           C.POINTER(C.c_double), _P, C.POINTER(gg_stats)]),
 ]

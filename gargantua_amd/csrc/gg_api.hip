@@ -74,7 +74,17 @@ struct gg_context {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   uint32_t num_cus = 256;
-  gg_launch launch{256, 4, 16};
+  gg_launch launch{256, 4, 16, 1};
+  // cached batch graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  struct GraphKey {
+    uint32_t n_scenes;
+    int metric, disk, precision;
+    uint64_t n_rays;
+    gg_launch launch;
+    void *scenes, *outs, *counters;
+  } graph_key{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf scenes, outs, counters, states_in;
   DevBuf stage[9];       // device staging for host-memory outputs
@@ -378,6 +388,8 @@ int gg_context_destroy(gg_context *ctx) {
   ctx->rhs_io.release();
   for (auto &b : ctx->stage) b.release();
   for (auto &b : ctx->batch_stage) b.release();
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
@@ -391,6 +403,8 @@ int gg_set_launch(gg_context *ctx, const gg_launch *l) {
   if (v.block_threads == 0) v.block_threads = 256;
   if (v.blocks_per_cu == 0) v.blocks_per_cu = 4;
   if (v.refill_interval == 0) v.refill_interval = 16;
+  if (v.graph == 0) v.graph = 1;
+  if (v.graph > 2) return fail(GG_ERR_INVALID_ARG, "graph must be 0, 1 or 2");
   if (v.block_threads % 64 || v.block_threads > 256)
     return fail(GG_ERR_INVALID_ARG, "block_threads must be a multiple of 64 in [64, 256]");
   if (v.blocks_per_cu > 64) return fail(GG_ERR_INVALID_ARG, "blocks_per_cu must be <= 64");
@@ -504,10 +518,38 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
   T.refill_interval = ctx->launch.refill_interval;
   uint32_t grid = grid_for(ctx, total);
   uint32_t block = ctx->launch.block_threads;
-  HIP_TRY(launch_reset(T.counters, s));
-  HIP_TRY(hipEventRecord(ctx->ev0, s));
-  if (total > 0) HIP_TRY(launch_trace(scenes[0].metric.kind, scenes[0].disk.kind, precision, T, grid, block, s));
-  HIP_TRY(hipEventRecord(ctx->ev1, s));
+  const bool use_graph = (n_scenes > 1) && ctx->launch.graph == 1 && total > 0;
+  if (use_graph) {
+    gg_context::GraphKey key{n_scenes, scenes[0].metric.kind, scenes[0].disk.kind, precision, total, ctx->launch,
+                             ctx->scenes.p, ctx->outs.p, ctx->counters.p};
+    if (!ctx->graph_exec || std::memcmp(&key, &ctx->graph_key, sizeof(key)) != 0) {
+      if (ctx->graph_exec) { (void)hipGraphExecDestroy(ctx->graph_exec); ctx->graph_exec = nullptr; }
+      if (ctx->graph) { (void)hipGraphDestroy(ctx->graph); ctx->graph = nullptr; }
+      // The scene and output tables are device buffers rewritten before every
+      // replay, so the captured kernel arguments stay valid for equal keys.
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      hipError_t e1 = launch_reset(T.counters, s);
+      hipError_t e2 = launch_trace(scenes[0].metric.kind, scenes[0].disk.kind, precision, T, grid, block, s);
+      hipGraph_t g = nullptr;
+      hipError_t e3 = hipStreamEndCapture(s, &g);
+      if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(GG_ERR_DEVICE, "graph capture failed: %s / %s / %s", hipGetErrorString(e1),
+                    hipGetErrorString(e2), hipGetErrorString(e3));
+      }
+      ctx->graph = g;
+      HIP_TRY(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
+      ctx->graph_key = key;
+    }
+    HIP_TRY(hipEventRecord(ctx->ev0, s));
+    HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));
+    HIP_TRY(hipEventRecord(ctx->ev1, s));
+  } else {
+    HIP_TRY(launch_reset(T.counters, s));
+    HIP_TRY(hipEventRecord(ctx->ev0, s));
+    if (total > 0) HIP_TRY(launch_trace(scenes[0].metric.kind, scenes[0].disk.kind, precision, T, grid, block, s));
+    HIP_TRY(hipEventRecord(ctx->ev1, s));
+  }
   for (uint32_t i = 0; i < n_scenes; ++i) {
     if (plans[i][8].dev) {
       const DevOut *dop = (const DevOut *)ctx->outs.p + i;
@@ -549,6 +591,38 @@ int gg_render(gg_context *ctx, const gg_scene *scene, int precision, const gg_ou
 int gg_render_batch(gg_context *ctx, const gg_scene *scenes, uint32_t n, int precision, const gg_outputs *out,
                     gg_stats *stats) {
   return render_impl(ctx, scenes, n, precision, out, nullptr, 0, stats);
+}
+
+int gg_autotune(gg_context *ctx, const gg_scene *scenes, uint32_t n, int precision, const gg_launch *candidates,
+                uint32_t n_candidates, gg_launch *best, double *best_kernel_ms) {
+  if (!ctx || !scenes || n == 0) return fail(GG_ERR_INVALID_ARG, "NULL argument");
+  std::vector<gg_launch> cands;
+  if (candidates && n_candidates) {
+    cands.assign(candidates, candidates + n_candidates);
+  } else {
+    for (uint32_t bt : {64u, 128u, 256u})
+      for (uint32_t bpc : {2u, 4u, 8u})
+        for (uint32_t ri : {8u, 32u}) cands.push_back(gg_launch{bt, bpc * 256u / bt, ri, 2});
+  }
+  const gg_launch saved = ctx->launch;
+  std::vector<gg_outputs> none(n);
+  std::memset(none.data(), 0, sizeof(gg_outputs) * n);
+  double best_ms = 1e300;
+  gg_launch best_l = saved;
+  for (const gg_launch &c : cands) {
+    int rc = gg_set_launch(ctx, &c);
+    if (rc) { ctx->launch = saved; return rc; }
+    gg_stats st;
+    rc = render_impl(ctx, scenes, n, precision, none.data(), nullptr, 0, &st);  // warm-up (and graph build)
+    if (!rc) rc = render_impl(ctx, scenes, n, precision, none.data(), nullptr, 0, &st);
+    if (rc) { ctx->launch = saved; return rc; }
+    if (st.kernel_ms < best_ms) { best_ms = st.kernel_ms; best_l = ctx->launch; }
+  }
+  best_l.graph = saved.graph;
+  ctx->launch = best_l;
+  if (best) *best = best_l;
+  if (best_kernel_ms) *best_kernel_ms = best_ms;
+  return GG_OK;
 }
 
 int gg_trace_states(gg_context *ctx, const gg_scene *scene, int precision, uint64_t n, const double *states_in,

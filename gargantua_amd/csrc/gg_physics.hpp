@@ -321,10 +321,15 @@ __device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 // Kerr geodesic acceleration in FP32: the same Christoffel symbols with the
 // divisions folded into 1/Sigma, 1/Delta, 1/sin^2 and the symmetric pairs of
 // the contraction merged.
-__device__ inline f4 rhs32_kerr(float m, float a, float r, float theta, f4 v) {
+// psi = theta - pi/2, formed in FP64 by the caller: near the equatorial
+// plane FP32 keeps full relative precision in cos(theta) = -sin(psi), which
+// a rounded FP32 theta would not (its 6e-8 absolute error feeds straight
+// into the theta acceleration of rays crossing the disk plane).
+__device__ inline f4 rhs32_kerr(float m, float a, float r, float psi, f4 v) {
 #pragma clang fp contract(fast)
-  float st, ct;
-  sincosf(theta, &st, &ct);
+  float sp, cp;
+  sincosf(psi, &sp, &cp);
+  const float st = cp, ct = -sp;
   float a2 = a * a;
   float r2 = r * r;
   float sin2 = st * st;
@@ -383,10 +388,11 @@ __device__ inline f4 rhs32_kerr(float m, float a, float r, float theta, f4 v) {
   return acc;
 }
 
-__device__ inline f4 rhs32_schw(float rs, float r, float theta, f4 v) {
+__device__ inline f4 rhs32_schw(float rs, float r, float psi, f4 v) {
 #pragma clang fp contract(fast)
-  float st, ct;
-  sincosf(theta, &st, &ct);
+  float sp, cp;
+  sincosf(psi, &sp, &cp);
+  const float st = cp, ct = -sp;
   float ir = frcp(r);
   float rmrs = r - rs;
   float ifd = frcp(r * rmrs);
@@ -402,9 +408,10 @@ __device__ inline f4 rhs32_schw(float rs, float r, float theta, f4 v) {
 }
 
 template <int METRIC>
-__device__ inline f4 rhs32(const DevScene &S, float r, float theta, f4 v) {
-  if (METRIC == 1) return rhs32_kerr((float)S.mass, (float)S.spin, r, theta, v);
-  return rhs32_schw((float)S.rs, r, theta, v);
+__device__ inline f4 rhs32(const DevScene &S, double r, double theta, f4 v) {
+  const float psi = (float)(theta - 1.5707963267948966);
+  if (METRIC == 1) return rhs32_kerr((float)S.mass, (float)S.spin, (float)r, psi, v);
+  return rhs32_schw((float)S.rs, (float)r, psi, v);
 }
 
 // ------------------------------------------------------ redshift & disk ----

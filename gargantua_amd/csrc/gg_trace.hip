@@ -95,7 +95,7 @@ __device__ inline d4 eval_rhs(const DevScene &S, double r, double th, d4 v) {
   if (ARITH == ARITH_EXACT) return rhs64<METRIC>(S, r, th, v);
   if (ARITH == ARITH_F32) {
     f4 vf{(float)v.x, (float)v.y, (float)v.z, (float)v.w};
-    return to64(rhs32<METRIC>(S, (float)r, (float)th, vf));
+    return to64(rhs32<METRIC>(S, r, th, vf));
   }
   return rhs64f<METRIC>(S, r, th, v);
 }

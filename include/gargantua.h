@@ -188,9 +188,12 @@ typedef struct gg_stats {
 
 /* Launch shape of the persistent trace kernel. 0 = library default. */
 typedef struct gg_launch {
-  uint32_t block_threads;     /* 64..1024, multiple of 64                     */
-  uint32_t blocks_per_cu;     /* resident blocks requested per CU             */
-  uint32_t refill_interval;   /* attempts between retired-lane refills        */
+  uint32_t block_threads;     /* 64..256, multiple of 64 (default 256)        */
+  uint32_t blocks_per_cu;     /* blocks launched per CU (default 4)           */
+  uint32_t refill_interval;   /* attempts between retired-lane refills (16)   */
+  uint32_t graph;             /* gg_render_batch: 1 = capture the launch in a
+                                 hipGraph and replay it on later calls with the
+                                 same shape (default), 2 = launch directly     */
 } gg_launch;
 
 typedef struct gg_context gg_context;
@@ -211,9 +214,20 @@ int gg_scene_rays(const gg_scene *scene, uint64_t *n_rays);
 int gg_render(gg_context *ctx, const gg_scene *scene, int precision,
               const gg_outputs *out, gg_stats *stats);
 
-/* n scenes in one persistent launch; out[i] belongs to scenes[i]. */
+/* n scenes in one persistent launch; out[i] belongs to scenes[i].  All scenes
+ * share metric kind and disk kind.  The launch (queue reset + trace kernel) is
+ * captured in a hipGraph cached on the context and replayed by later calls
+ * with the same number of scenes, kinds, precision and launch shape. */
 int gg_render_batch(gg_context *ctx, const gg_scene *scenes, uint32_t n,
                     int precision, const gg_outputs *out, gg_stats *stats);
+
+/* Times the batch (outputs discarded) under each candidate launch shape and
+ * installs the fastest on the context.  candidates == NULL: a built-in grid
+ * of block sizes, blocks per CU and refill intervals.  No reference
+ * counterpart (the reference schedules rows with rayon). */
+int gg_autotune(gg_context *ctx, const gg_scene *scenes, uint32_t n, int precision,
+                const gg_launch *candidates, uint32_t n_candidates, gg_launch *best,
+                double *best_kernel_ms);
 
 /* Trace n caller-supplied states (9 doubles each, AoS: t r th ph vt vr vth vph
  * lambda).  Camera and image fields of the scene are ignored except for the

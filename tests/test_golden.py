@@ -37,8 +37,24 @@ def test_gpu_matches_golden(renderer, name, precision):
     print(name, precision, c)
     assert c["class_agree"] == 1.0
     assert c["err_le_1e-6"] >= 0.99
-    assert np.mean(np.abs(got["rgb"].astype(int) - want["rgb"].astype(int)) <= 1) >= 0.99
+    # RGB: every pixel whose sub-rays did not escape agrees within 1/255; the
+    # procedural sky hashes the raw f64 bits of the escape direction
+    # (sky.rs:26-40), so an escaped ray whose direction differs in the last
+    # bit can gain or lose a star and is held only to a looser bound.
+    spa = scene.samples_per_axis
+    cls_px = want["cls"].reshape(scene.height, spa, scene.width, spa)
+    escaped_px = (cls_px == gg.GG_CLASS_ESCAPED).any(axis=(1, 3)).ravel()
+    close = np.all(np.abs(got["rgb"].astype(int) - want["rgb"].astype(int)) <= 1, axis=1)
+    assert np.all(close[~escaped_px])
+    if escaped_px.any():
+        assert np.mean(close[escaped_px]) >= 0.9
+    # Disk intensities within 1e-6 relative.  The volumetric disk weighs each
+    # step by exp(-z^2 / 2H^2) (disk.rs:170), whose log-derivative is -z/H^2:
+    # the ~1e-7 position noise of the mixed kernel's FP32 steps inside the
+    # disk band becomes ~1e-6 of intensity there, so mixed volumetric scenes
+    # are held to 1e-5.
     disk = want["cls"] == gg.GG_CLASS_DISK
     if disk.any():
+        tol = 1e-5 if (precision == "mixed" and scene.disk.kind == gg.GG_DISK_VOLUMETRIC) else 1e-6
         rel = np.abs(got["intensity"][disk] - want["intensity"][disk]) / np.abs(want["intensity"][disk])
-        assert np.mean(rel <= 1e-6) >= 0.99
+        assert np.mean(rel <= tol) >= 0.99
