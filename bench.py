@@ -39,6 +39,10 @@ PEAK_FP32 = 157.3   # TFLOP/s, MI355X vector FP32 (published; needs packed math)
 METRIC = "geodesics/s Kerr a=0.9 1024²: mixed FP32 vs FP64 at ≤1e-6 err; adjoint grads/s"
 
 
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def _dev_outputs(torch, n):
     d = "cuda"
     return {
@@ -103,6 +107,78 @@ def cpu_baseline(scene, stride=2):
             "value_1core": len(sub) / dt1}
 
 
+def bench_batch64(args):
+    """64 seeded scenes (a~U[0,0.998], i~U[5,89] deg, r_cam~U[30,500], res in {128..1024}) in one
+    hipGraph-captured persistent launch after the autotuner picked the launch shape."""
+    import torch
+    import gargantua_amd as gg
+    scenes = gg.configs.batch_64(seed=2026)
+    n = sum(s.n_rays for s in scenes)
+    rend = gg.Renderer(0, stream=torch.cuda.current_stream().cuda_stream)
+    outs = [_ptrs(_dev_outputs(torch, s.n_rays)) for s in scenes]
+    # tune on every 8th scene (same distribution, an eighth of the rays)
+    t0 = time.perf_counter()
+    best, best_ms = rend.autotune(scenes[::8], "mixed")
+    tune_s = time.perf_counter() - t0
+    _log(f"autotune: {best} ({best_ms:.1f} ms on {len(scenes[::8])} scenes) in {tune_s:.1f} s")
+    res = {}
+    for prec in ("mixed", "fp64"):
+        for _ in range(args.warmup):
+            rend.render_batch(scenes, prec, outputs=[dict(o) for o in outs])
+        torch.cuda.synchronize()
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, st = rend.render_batch(scenes, prec, outputs=[dict(o) for o in outs])
+            kms.append(st["kernel_ms"])
+            _log(f"batch64 {prec}: {st['kernel_ms']:.1f} ms")
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        res[prec] = {"value": n * args.steps / wall, "kernel_ms": float(np.mean(kms)),
+                     "attempts_per_ray": st["attempts"] / n, "fp32_rhs_share": st["rhs_evals_fp32"] / st["rhs_evals"]}
+    rend.close()
+    return {"metric": "geodesics/s, 64 seeded Kerr scenes in one graph-captured launch", "value": res["mixed"]["value"],
+            "unit": "geodesics/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 * n / res["mixed"]["value"], "higher_is_better": True, "dtype": "f32+f64",
+            "config": {"workload": "batch of 64 scenes, generator seed 2026", "rays": n,
+                       "resolutions": [s.width for s in scenes], "autotuned_launch": best,
+                       "autotune_best_ms": best_ms, "autotune_seconds": tune_s},
+            "mixed": res["mixed"], "fp64": res["fp64"], "mixed_speedup": res["fp64"]["kernel_ms"] / res["mixed"]["kernel_ms"]}
+
+
+def bench_kerr4096(args):
+    """Near-extremal Kerr a=0.998 at 4096x4096, camera r=1000M, i=85 deg, FOV 20 deg: mixed and FP64,
+    classification agreement inside and outside a +-2 px band around class boundaries."""
+    import torch
+    import gargantua_amd as gg
+    from tests.parity import compare
+    size = args.size if args.size != 1024 else 4096
+    scene = gg.configs.kerr_4096(size, size)
+    n = scene.n_rays
+    rend = gg.Renderer(0, stream=torch.cuda.current_stream().cuda_stream)
+    res, host = {}, {}
+    for prec in ("mixed", "fp64"):
+        outs = _dev_outputs(torch, n)
+        wall, kms, st = time_precision(torch, rend, scene, prec, 1, 0, outs)
+        _log(f"kerr4096 {prec}: {kms[0]:.1f} ms")
+        res[prec] = {"value": n / wall, "kernel_ms": kms[0], "attempts_per_ray": st["attempts"] / n,
+                     "max_attempts": None, "fp32_rhs_share": st["rhs_evals_fp32"] / st["rhs_evals"]}
+        h = {k: v.cpu().numpy() for k, v in outs.items()}
+        h["attempts"] = h["attempts"].astype(np.uint32)
+        res[prec]["max_attempts"] = int(h["attempts"].max())
+        res[prec]["classes"] = np.bincount(h["cls"], minlength=4).tolist()
+        host[prec] = h
+        del outs
+        torch.cuda.empty_cache()
+    cmp = compare(host["mixed"], host["fp64"], shape=(size, size))
+    rend.close()
+    return {"metric": f"geodesics/s Kerr a=0.998 {size}²", "value": res["mixed"]["value"], "unit": "geodesics/s",
+            "n_gpus": 1, "dtype": "f32+f64", "config": {"workload": f"Kerr a=0.998 {size}x{size}, camera r=1000M "
+                                                        "i=85deg FOV 20deg, escape 1100M, 50000 attempts"},
+            "mixed": res["mixed"], "fp64": res["fp64"], "mixed_speedup": res["fp64"]["kernel_ms"] / res["mixed"]["kernel_ms"],
+            "mixed_vs_fp64": cmp}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,11 +186,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--workload", default="kerr1024", choices=["kerr1024", "batch64", "kerr4096"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
               "(rays are independent; the multi-GPU split is described in the design notes)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch64":
+        print(json.dumps(bench_batch64(args)))
+        return
+    if args.workload == "kerr4096":
+        print(json.dumps(bench_kerr4096(args)))
+        return
 
     import torch
     import gargantua_amd as gg

@@ -74,7 +74,7 @@ struct gg_context {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   uint32_t num_cus = 256;
-  gg_launch launch{256, 4, 16, 1};
+  gg_launch launch{256, 2, 16, 1};
   // cached batch graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -401,7 +401,7 @@ int gg_set_launch(gg_context *ctx, const gg_launch *l) {
   if (!ctx || !l) return fail(GG_ERR_INVALID_ARG, "NULL argument");
   gg_launch v = *l;
   if (v.block_threads == 0) v.block_threads = 256;
-  if (v.blocks_per_cu == 0) v.blocks_per_cu = 4;
+  if (v.blocks_per_cu == 0) v.blocks_per_cu = 2;
   if (v.refill_interval == 0) v.refill_interval = 16;
   if (v.graph == 0) v.graph = 1;
   if (v.graph > 2) return fail(GG_ERR_INVALID_ARG, "graph must be 0, 1 or 2");
@@ -564,6 +564,10 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
     for (auto &p : plans[i])
       if (p.host && p.dev) HIP_TRY(hipMemcpyAsync(p.host, p.dev, p.bytes, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  // exact-once coverage: every ray handed out by the queue retired once
+  if (cnt.retired != total)
+    return fail(GG_ERR_INTERNAL, "ray coverage: %llu of %llu rays retired", (unsigned long long)cnt.retired,
+                (unsigned long long)total);
   if (stats) {
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

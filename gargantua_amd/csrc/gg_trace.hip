@@ -62,6 +62,7 @@ struct Lane {
   double lambda;
   double h;
   bool k1_ok;        // LDS slot 0 holds k1 = a(pos, vel)
+  bool k1_f32;       // ... evaluated by an FP32 attempt (mixed kernel)
   uint32_t attempts, accepted;
   double intensity, transmission;
   double err_prev;   // error ratio of the previous full attempt
@@ -142,24 +143,30 @@ struct StageOut {
   bool invalid;
 };
 
-// The six stages of one attempt from k1 (LDS slot 0): k2..k7, the 5th-order
-// velocity and the error ratio (geodesic.rs:244-308).
+// The stages of one attempt (geodesic.rs:244-308): k1 when the lane has no
+// valid one (s = 1), then k2..k7, the 5th-order velocity and the error.  One
+// copy of the loop per arithmetic (ARITH_EXACT, ARITH_F32, ARITH_FAST).
 template <int METRIC, int ARITH>
-__device__ inline StageOut stages(const DevScene &S, const KStore &K, const double p0[4], const d4 vel, double h) {
+__device__ inline StageOut stages(const DevScene &S, const KStore &K, const double p0[4], const d4 vel, double h,
+                                  bool need_k1) {
 #pragma clang fp contract(off)
   StageOut o;
-  const d4 k1 = K.get(0);
-  d4 s5{C1 * k1.x, C1 * k1.y, C1 * k1.z, C1 * k1.w};   // sum c_i k_i (:294-295)
-  d4 s4{D1 * k1.x, D1 * k1.y, D1 * k1.z, D1 * k1.w};   // sum d_i k_i (:299-305)
+  d4 k1{0, 0, 0, 0}, s5{0, 0, 0, 0}, s4{0, 0, 0, 0};
   o.nv = d4{0, 0, 0, 0};
   o.k7 = d4{0, 0, 0, 0};
   o.nr = 0;
   o.invalid = false;
 #pragma unroll 1
-  for (int s = 2; s <= 7; ++s) {
+  for (int s = 1; s <= 7; ++s) {
     double qr, qth;
     d4 vs;
-    if (s <= 6) {
+    bool eval = true;
+    if (s == 1) {
+      qr = p0[1];                                                   // k1 = a(pos, vel) (:245)
+      qth = p0[2];
+      vs = vel;
+      eval = need_k1;
+    } else if (s <= 6) {
       // stage position pos + (h a_s) vel on the forward-Euler line (:248-284)
       double ha = h * kA[s];
       double q0 = madd<ARITH>(ha, vel.x, p0[0]), q1 = madd<ARITH>(ha, vel.y, p0[1]);
@@ -189,10 +196,19 @@ __device__ inline StageOut stages(const DevScene &S, const KStore &K, const doub
       qth = madd<ARITH>(h, vel.z, p0[2]);
       vs = o.nv;
     }
-    d4 k = eval_rhs<METRIC, ARITH>(S, qr, qth, vs);
-    ++o.nr;
+    d4 k;
+    if (eval) {
+      k = eval_rhs<METRIC, ARITH>(S, qr, qth, vs);
+      ++o.nr;
+    } else {
+      k = K.get(0);
+    }
     if (s <= 5) K.put(s - 1, k);
-    if (s >= 3) {
+    if (s == 1) {
+      k1 = k;
+      s5 = d4{C1 * k1.x, C1 * k1.y, C1 * k1.z, C1 * k1.w};       // sum c_i k_i (:294-295)
+      s4 = d4{D1 * k1.x, D1 * k1.y, D1 * k1.z, D1 * k1.w};       // sum d_i k_i (:299-305)
+    } else if (s >= 3) {
       if (s <= 6) {
         double c = kC[s];
         s5.x = madd<ARITH>(c, k.x, s5.x); s5.y = madd<ARITH>(c, k.y, s5.y);
@@ -232,13 +248,34 @@ __device__ inline StageOut stages(const DevScene &S, const KStore &K, const doub
   return o;
 }
 
+// Whether the next attempt of this ray may run its stages in FP32: the
+// controller is saturated (h == max_step, previous error ratio < 0.3), the
+// ray is outgoing beyond fp32_min_r (photons outside the photon-orbit region
+// never turn back, so FP32 rounding is not amplified by a later close
+// approach) or inbound beyond fp32_far_r, theta is away from the poles and
+// the FP32 noise bound is below 0.02 in error-ratio units.  A function of the
+// ray's own state only, so every ray's arithmetic is independent of which
+// lane or wavefront carries it.
+__device__ inline bool fp32_eligible(const DevScene &S, const Lane &L) {
+  const double r = L.pos[1], th = L.pos[2];
+  bool far_enough = (L.vel.y > 0.0) ? (r >= S.fp32_min_r) : (r >= S.fp32_far_r);
+  if (!(far_enough && (L.h == S.max_step) && (L.err_prev < 0.3) && (th > 1e-3) && (th < PI - 1e-3)))
+    return false;
+  return fp32_noise(S, L.pos, L.vel, L.h) < 0.02f;
+}
+
 // One RK45Integrator::step (geodesic.rs:185-328) on lane state L with the
-// stage values in LDS.  PREC 0: exact FP64; 1: mixed (FP32 stages where the
-// controller is saturated, exact FP64 otherwise, so that every decision and
-// every FP64 step match the FP64 kernel); 2: fast FP64.  Returns RES_*;
-// `acc` reports acceptance; p0/v0 keep the pre-step position and velocity.
+// stage values in LDS.  PREC 0: exact FP64; 2: fast FP64; 1: mixed: a
+// wavefront runs its stages in FP32 when every live lane's ray is
+// FP32-eligible, and in exact FP64 otherwise (FP64 is always allowed), so a
+// wavefront never executes both loops.  Every accept/reject decision and
+// every h equal those of the FP64 kernel either way; which FP32-eligible
+// steps do run in FP32 depends on the lane grouping, so mixed final states
+// reproduce across launches to FP32 rounding (~1e-12 relative), not bitwise.
+// Returns RES_*; `acc` reports acceptance; p0/v0 keep the pre-step state.
 template <int METRIC, int PREC>
-__device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool &acc, double p0[4], d4 &v0) {
+__device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool &acc, double p0[4], d4 &v0,
+                              bool use32) {
 #pragma clang fp contract(off)
   constexpr int BASE = (PREC == 2) ? ARITH_FAST : ARITH_EXACT;
   acc = false;
@@ -249,36 +286,28 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
   const d4 vel = L.vel;
   for (int i = 0; i < 4; ++i) p0[i] = L.pos[i];
   v0 = vel;
-  if (!L.k1_ok) {                                     // k1 (:245); otherwise k7 of the last accepted step
-    K.put(0, eval_rhs<METRIC, BASE>(S, p0[1], p0[2], vel));
-    L.k1_ok = true;
-    L.n_rhs++;
-  }
   const double h = L.h;
-
-  bool use32 = false;
-  if (PREC == 1) {
-    // Outgoing photons beyond the photon-orbit region never turn back, so
-    // FP32 rounding on them is not amplified by a later close approach;
-    // inbound rays only take FP32 steps far out.
-    const double th = p0[2];
-    bool far_enough = (vel.y > 0.0) ? (r >= S.fp32_min_r) : (r >= S.fp32_far_r);
-    use32 = far_enough && (h == S.max_step) && (L.err_prev < 0.3) && (th > 1e-3) && (th < PI - 1e-3) &&
-            (fp32_noise(S, p0, vel, h) < 0.02f);
-  }
   StageOut o;
   bool done = false;
   if (PREC == 1 && use32) {
-    o = stages<METRIC, ARITH_F32>(S, K, p0, vel, h);
+    const bool need_k1 = !L.k1_ok;
+    o = stages<METRIC, ARITH_F32>(S, K, p0, vel, h, need_k1);
     L.n_rhs += o.nr;
     L.n_rhs32 += o.nr;
-    // FP32 result near a decision (or an invalid stage): redo in FP64
+    if (need_k1) { L.k1_ok = true; L.k1_f32 = true; }   // slot 0 now holds an FP32 k1
+    // an FP32 result near a decision (or with an invalid stage) is redone in FP64
     done = !o.invalid && (o.err / o.thresh < 0.4);
+    if (!done) use32 = false;
   }
   if (!done) {
-    o = stages<METRIC, BASE>(S, K, p0, vel, h);
+    // An FP64 attempt needs an FP64 k1: after an FP32 attempt k7 (the FSAL
+    // k1) carries FP32 rounding, so recompute it.
+    const bool need_k1 = !L.k1_ok || L.k1_f32;
+    o = stages<METRIC, BASE>(S, K, p0, vel, h, need_k1);
     L.n_rhs += o.nr;
+    if (need_k1) L.k1_f32 = false;
   }
+  L.k1_ok = true;
   if (o.invalid) { L.h = h * 0.5; return RES_CONTINUE; }
   const double err = o.err, thresh = o.thresh;
   L.n_full++;
@@ -296,6 +325,7 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
     L.pos[3] = ph < 0.0 ? ph + TWO_PI : ph;
     K.put(0, o.k7);                                 // FSAL: k7 is the next k1 ...
     L.k1_ok = !refl;                                // ... unless theta was reflected
+    L.k1_f32 = use32;
     if (err > 0.0) {
       double hn = h * 0.9 * pow(thresh / err, 0.2);
       if (hn < S.min_step) hn = S.min_step;
@@ -354,7 +384,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
   const KStore K{lds_k + threadIdx.x, blockDim.x};
   Lane L;
   L.n_rhs = 0; L.n_rhs32 = 0; L.n_full = 0;
-  uint32_t n_att = 0;
+  uint32_t n_att = 0, n_retired = 0;
   bool active = false;
   uint64_t ray = 0;
   uint32_t si = 0;        // scene of this lane's ray
@@ -365,56 +395,121 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
   // holding ~75 doubles live across the persistent loop.
   const DevScene *__restrict__ Sp = T.scenes;
 
+  // Start ray `ray` in this lane.
+  auto init_ray = [&](uint64_t rid) {
+    ray = rid;
+    si = single ? 0u : find_scene(T.scenes, T.n_scenes, ray);
+    if (!single) Sp = T.scenes + si;
+    uint64_t local = ray - Sp->ray_begin;
+    if (SRC == 0) {
+      uint32_t px, py;
+      ray_pixel(*Sp, local, px, py);
+      out_i = (uint64_t)py * Sp->sub_w + px;
+      camera_ray(*Sp, px, py, L.pos, L.vel);
+      L.lambda = 0.0;
+    } else {
+      out_i = local;
+      const double *st = T.states_in + 9 * ray;
+      for (int c = 0; c < 4; ++c) L.pos[c] = st[c];
+      L.vel = d4{st[4], st[5], st[6], st[7]};
+      L.lambda = st[8];
+    }
+    L.h = Sp->h0;
+    L.k1_ok = false;
+    L.k1_f32 = false;
+    L.attempts = 0; L.accepted = 0;
+    L.intensity = 0.0; L.transmission = 1.0;
+    L.err_prev = 1.0;
+    active = true;
+  };
+
+  __shared__ uint32_t s_cnt[2][4];
+  __shared__ unsigned long long s_base;
   uint32_t iter = 0;
   for (;;) {
-    // ---- refill retired lanes: one atomicAdd per wavefront ----
-    bool need = (iter == 0) || ((iter % T.refill_interval) == 0) || !__any(active);
-    if (need && !exhausted) {
-      unsigned long long idle = __ballot(!active);
-      uint32_t cnt = __popcll(idle);
-      if (cnt) {
-        uint32_t leader = __ffsll((long long)idle) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(&T.counters->queue, (unsigned long long)cnt);
-        base = __shfl(base, leader, 64);
-        if (base >= n) exhausted = true;
-        if (!active) {
-          uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
-          ray = base + rank;
-          if (ray < n) {
-            si = single ? 0u : find_scene(T.scenes, T.n_scenes, ray);
-            if (!single) Sp = T.scenes + si;
-            uint64_t local = ray - Sp->ray_begin;
-            if (SRC == 0) {
-              uint32_t px, py;
-              ray_pixel(*Sp, local, px, py);
-              out_i = (uint64_t)py * Sp->sub_w + px;
-              camera_ray(*Sp, px, py, L.pos, L.vel);
-              L.lambda = 0.0;
-            } else {
-              out_i = local;
-              const double *st = T.states_in + 9 * ray;
-              for (int c = 0; c < 4; ++c) L.pos[c] = st[c];
-              L.vel = d4{st[4], st[5], st[6], st[7]};
-              L.lambda = st[8];
-            }
-            L.h = Sp->h0;
-            L.k1_ok = false;
-            L.attempts = 0; L.accepted = 0;
-            L.intensity = 0.0; L.transmission = 1.0;
-            L.err_prev = 1.0;
-            active = true;
-          }
+    {
+      // ---- block regroup: compaction (and precision sorting when mixed) ----
+      // Every refill_interval attempts the block's wavefronts exchange ray
+      // states through LDS: live rays move to the lowest lanes (for the mixed
+      // kernel FP32-eligible rays first, so that whole wavefronts can take
+      // FP32 steps), retired lanes gather at the tail and take new rays with
+      // one atomicAdd for the block.
+      if (iter % T.refill_interval == 0) {
+        const uint32_t tid = threadIdx.x, B = blockDim.x, wave = tid >> 6;
+        const uint32_t key = !active ? 2u : ((PREC == 1 && fp32_eligible(*Sp, L)) ? 0u : 1u);
+        const unsigned long long m0 = __ballot(key == 0), m1 = __ballot(key == 1);
+        if (lane == 0) { s_cnt[0][wave] = __popcll(m0); s_cnt[1][wave] = __popcll(m1); }
+        __syncthreads();
+        uint32_t n0 = 0, n1 = 0, o0 = 0, o1 = 0;
+        for (uint32_t w = 0; w < (B >> 6); ++w) {
+          if (w < wave) { o0 += s_cnt[0][w]; o1 += s_cnt[1][w]; }
+          n0 += s_cnt[0][w];
+          n1 += s_cnt[1][w];
         }
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const uint32_t dst = key == 0 ? o0 + __popcll(m0 & below) : n0 + o1 + __popcll(m1 & below);
+        const uint32_t n_act = n0 + n1;
+        // The exchange reuses stage slots 1..4 (free between attempts) as
+        // 16 planes of B doubles; two passes of at most 14 fields.
+        double *X = lds_k + 4 * B;
+        d4 k1 = K.get(0);
+        __syncthreads();
+        if (active) {
+          X[0 * B + dst] = L.pos[0]; X[1 * B + dst] = L.pos[1]; X[2 * B + dst] = L.pos[2]; X[3 * B + dst] = L.pos[3];
+          X[4 * B + dst] = L.vel.x; X[5 * B + dst] = L.vel.y; X[6 * B + dst] = L.vel.z; X[7 * B + dst] = L.vel.w;
+          X[8 * B + dst] = L.lambda; X[9 * B + dst] = L.h;
+          X[10 * B + dst] = k1.x; X[11 * B + dst] = k1.y; X[12 * B + dst] = k1.z; X[13 * B + dst] = k1.w;
+        }
+        __syncthreads();
+        const bool now_active = tid < n_act;
+        if (now_active) {
+          L.pos[0] = X[0 * B + tid]; L.pos[1] = X[1 * B + tid]; L.pos[2] = X[2 * B + tid]; L.pos[3] = X[3 * B + tid];
+          L.vel = d4{X[4 * B + tid], X[5 * B + tid], X[6 * B + tid], X[7 * B + tid]};
+          L.lambda = X[8 * B + tid]; L.h = X[9 * B + tid];
+          k1 = d4{X[10 * B + tid], X[11 * B + tid], X[12 * B + tid], X[13 * B + tid]};
+        }
+        __syncthreads();
+        if (active) {
+          X[0 * B + dst] = L.intensity; X[1 * B + dst] = L.transmission; X[2 * B + dst] = L.err_prev;
+          X[3 * B + dst] = __longlong_as_double((long long)ray);
+          X[4 * B + dst] = __longlong_as_double((long long)out_i);
+          X[5 * B + dst] = __longlong_as_double((long long)(((uint64_t)L.attempts << 32) | L.accepted));
+          X[6 * B + dst] = __longlong_as_double(
+              (long long)(((uint64_t)si << 2) | (L.k1_f32 ? 2u : 0u) | (L.k1_ok ? 1u : 0u)));
+        }
+        __syncthreads();
+        if (now_active) {
+          L.intensity = X[0 * B + tid]; L.transmission = X[1 * B + tid]; L.err_prev = X[2 * B + tid];
+          ray = (uint64_t)__double_as_longlong(X[3 * B + tid]);
+          out_i = (uint64_t)__double_as_longlong(X[4 * B + tid]);
+          uint64_t aa = (uint64_t)__double_as_longlong(X[5 * B + tid]);
+          L.attempts = (uint32_t)(aa >> 32); L.accepted = (uint32_t)aa;
+          uint64_t sf = (uint64_t)__double_as_longlong(X[6 * B + tid]);
+          si = (uint32_t)(sf >> 2); L.k1_ok = (sf & 1u) != 0; L.k1_f32 = (sf & 2u) != 0;
+          if (!single) Sp = T.scenes + si;
+          K.put(0, k1);
+        }
+        active = now_active;
+        // refill the tail lanes (NO_FETCH: the block was full or the queue
+        // already ran dry, nothing was taken this round)
+        constexpr unsigned long long NO_FETCH = ~0ull;
+        if (tid == 0) {
+          unsigned long long base = NO_FETCH;
+          if (!exhausted && n_act < B) base = atomicAdd(&T.counters->queue, (unsigned long long)(B - n_act));
+          s_base = base;
+        }
+        __syncthreads();
+        const unsigned long long base = s_base;
+        if (base != NO_FETCH) {
+          if (base >= n) exhausted = true;
+          if (!active && base + (tid - n_act) < n) init_ray(base + (tid - n_act));
+        }
+        // leave together once the queue is empty and no lane of the block is live
+        if (exhausted && n_act == 0 && __syncthreads_or(active) == 0) break;
       }
-    }
-    if (!__any(active)) {
-      if (exhausted) break;
       ++iter;
-      continue;
+      if (!active) continue;
     }
-    ++iter;
-    if (!active) continue;
 
     // ---- one step() call ----
     uint8_t cls = 255;
@@ -425,7 +520,9 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
       double p0[4];
       d4 v0;
       double lam0 = L.lambda;
-      int res = attempt<METRIC, PREC>(*Sp, L, K, acc, p0, v0);
+      bool use32 = false;
+      if (PREC == 1) use32 = __all(fp32_eligible(*Sp, L));   // over the lanes still running
+      int res = attempt<METRIC, PREC>(*Sp, L, K, acc, p0, v0, use32);
       L.attempts++;
       n_att++;
       if (acc) L.accepted++;
@@ -505,12 +602,14 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
         O.state[8 * p + out_i] = L.lambda;
       }
       active = false;
+      n_retired++;
     }
   }
   // ---- tallies: one atomic per wavefront ----
   unsigned long long a = wave_sum(n_att), r = wave_sum(L.n_rhs), r32 = wave_sum(L.n_rhs32),
-                     f = wave_sum(L.n_full);
+                     f = wave_sum(L.n_full), ret = wave_sum(n_retired);
   if (lane == 0) {
+    atomicAdd(&T.counters->retired, ret);
     atomicAdd(&T.counters->attempts, a);
     atomicAdd(&T.counters->rhs, r);
     atomicAdd(&T.counters->rhs32, r32);

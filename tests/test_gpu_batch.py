@@ -1,7 +1,10 @@
 """Batched scenes in one persistent launch (gg_render_batch), its hipGraph
 replay and the launch-shape autotuner.  Every ray's arithmetic is the same
 in a batch as alone, so batch outputs equal single-scene outputs bit for bit
-in every precision."""
+in the FP64 precisions.  The mixed kernel takes FP32 steps only in wavefronts
+whose lanes are all FP32-eligible, so which eligible steps run in FP32 depends
+on the lane grouping: its decisions (classes, attempt counts) are the same in
+any launch and its final states agree to FP32 rounding."""
 import numpy as np
 import pytest
 
@@ -10,6 +13,18 @@ import gargantua_amd as gg
 pytestmark = pytest.mark.gpu
 
 KEYS = ("cls", "r", "theta", "phi", "intensity", "attempts", "accepted")
+EXACT_KEYS = ("cls", "attempts", "accepted")
+
+
+def assert_same(a, b, precision, what=""):
+    if precision != "mixed":
+        for k in KEYS:
+            np.testing.assert_array_equal(a[k], b[k], err_msg=f"{what}{k}")
+        return
+    for k in EXACT_KEYS:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=f"{what}{k}")
+    for k in ("r", "theta", "phi", "intensity"):
+        np.testing.assert_allclose(a[k], b[k], rtol=1e-7, atol=1e-12, err_msg=f"{what}{k}")
 
 
 def _scenes():
@@ -23,8 +38,7 @@ def test_batch_equals_single_renders(renderer, precision):
     assert st["rays"] == sum(s.n_rays for s in scenes)
     for sc, o in zip(scenes, outs):
         ref = renderer.render(sc, precision, state=False)
-        for k in KEYS:
-            np.testing.assert_array_equal(o[k], ref[k], err_msg=k)
+        assert_same(o, ref, precision)
 
 
 def test_graph_replays_match_eager(renderer):
@@ -35,8 +49,7 @@ def test_graph_replays_match_eager(renderer):
     for rep in range(3):
         outs, _ = renderer.render_batch(scenes, "mixed")
         for a, b in zip(outs, eager):
-            for k in KEYS:
-                np.testing.assert_array_equal(a[k], b[k], err_msg=f"replay {rep}: {k}")
+            assert_same(a, b, "mixed", f"replay {rep}: ")
         renderer.render(scenes[rep], "fp64", state=False)   # eager work between replays
     # new scene contents through the same graph (same shapes, different parameters)
     other = gg.configs.batch_64(seed=8, n=6, resolutions=(24, 40, 64))
@@ -46,7 +59,7 @@ def test_graph_replays_match_eager(renderer):
     outs, _ = renderer.render_batch(other, "mixed")
     for sc, o in zip(other, outs):
         ref = renderer.render(sc, "mixed", state=False)
-        np.testing.assert_array_equal(o["r"], ref["r"])
+        assert_same(o, ref, "mixed")
 
 
 def test_autotune_picks_a_valid_shape(renderer):
@@ -57,8 +70,8 @@ def test_autotune_picks_a_valid_shape(renderer):
     assert renderer.get_launch()["block_threads"] == best["block_threads"]
     outs, _ = renderer.render_batch(scenes, "mixed")
     ref = renderer.render(scenes[0], "mixed", state=False)
-    np.testing.assert_array_equal(outs[0]["r"], ref["r"])
-    renderer.set_launch(256, 4, 16, 1)
+    assert_same(outs[0], ref, "mixed")
+    renderer.set_launch(256, 2, 16, 1)
 
 
 def test_batch_rejects_mixed_kinds(renderer):
@@ -66,3 +79,23 @@ def test_batch_rejects_mixed_kinds(renderer):
     scenes[1].disk = None
     with pytest.raises(gg.GargantuaError, match="share metric kind"):
         renderer.render_batch(scenes, "fp64")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp64fast"])
+def test_every_ray_retired_once_with_refills(renderer, precision):
+    """More rays than resident lanes, so lanes are refilled many times: every
+    output slot is written (sentinels gone), the library's own retirement
+    count matches, and a short refill interval changes no bit."""
+    scene = gg.configs.kerr_1024(768, 512)
+    out = gg.alloc_outputs(scene.n_rays, state=False)
+    out["cls"][:] = 255
+    out["attempts"][:] = 0xFFFFFFFF
+    renderer.set_launch(256, 1, 16, 1)
+    a = renderer.render(scene, precision, outputs=out)
+    assert a["stats"]["rays"] == scene.n_rays
+    assert not np.any(a["cls"] == 255)
+    assert a["stats"]["attempts"] == int(a["attempts"].astype(np.uint64).sum())
+    renderer.set_launch(128, 2, 4, 1)
+    b = renderer.render(scene, precision, state=False)
+    renderer.set_launch(256, 2, 16, 1)
+    assert_same(a, b, precision)
