@@ -324,6 +324,51 @@ class Renderer:
                "gg_geodesic_acceleration")
         return acc
 
+    def geodesic_acceleration_vjp(self, metric, pos: np.ndarray, vel: np.ndarray, cot: np.ndarray) -> dict:
+        """w^T J of the Kerr RHS: {"pos_bar": [n,4], "vel_bar": [n,4], "spin_bar": [n]}."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 4)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 4)
+        cot = np.ascontiguousarray(cot, dtype=np.float64).reshape(-1, 4)
+        n = len(pos)
+        pb, vb, sb = np.zeros((n, 4)), np.zeros((n, 4)), np.zeros(n)
+        m = metric.to_c()
+        _check(self.lib, self.lib.gg_geodesic_acceleration_vjp(self.ctx, C.byref(m), n, pos.ctypes.data,
+                                                               vel.ctypes.data, cot.ctypes.data, pb.ctypes.data,
+                                                               vb.ctypes.data, sb.ctypes.data),
+               "gg_geodesic_acceleration_vjp")
+        return {"pos_bar": pb, "vel_bar": vb, "spin_bar": sb}
+
+    def grad(self, gscene, target: np.ndarray, checkpoint_every: int = 32, want_dI: bool = False) -> dict:
+        """Loss, dL/d(spin, inclination, q, r_inner) and the image of a gg_grad_scene (see configs.grad_512_params)."""
+        n = gscene.width * gscene.height
+        target = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
+        if target.size != n:
+            raise ValueError(f"target has {target.size} pixels, scene {n}")
+        loss = C.c_double()
+        g = np.zeros(4)
+        inten = np.zeros(n)
+        dI = np.zeros((n, 4)) if want_dI else None
+        st = _abi.gg_grad_stats()
+        _check(self.lib, self.lib.gg_grad(self.ctx, C.byref(gscene), target.ctypes.data, checkpoint_every,
+                                          C.byref(loss), g.ctypes.data, inten.ctypes.data,
+                                          dI.ctypes.data if want_dI else None, C.byref(st)), "gg_grad")
+        out = {"loss": loss.value, "grad": g, "intensity": inten,
+               "stats": {k: getattr(st, k) for k, _ in _abi.gg_grad_stats._fields_}}
+        if want_dI:
+            out["dI"] = dI
+        return out
+
+    def grad_image(self, gscene) -> np.ndarray:
+        """The FP64 image gg_grad differentiates (gg_render of gg_grad_scene_to_scene)."""
+        sc = _abi.gg_scene()
+        _check(self.lib, self.lib.gg_grad_scene_to_scene(C.byref(gscene), C.byref(sc)), "gg_grad_scene_to_scene")
+        n = gscene.width * gscene.height
+        o = {"intensity": np.zeros(n)}
+        st = _abi.gg_stats()
+        _check(self.lib, self.lib.gg_render(self.ctx, C.byref(sc), PRECISIONS["fp64"], C.byref(outputs_struct(o)),
+                                            C.byref(st)), "gg_render")
+        return o["intensity"]
+
 
 _default_renderer: Optional[Renderer] = None
 

@@ -83,6 +83,13 @@ class gg_grad_scene(C.Structure):
                 ("integrator", gg_integrator), ("width", C.c_uint32), ("height", C.c_uint32)]
 
 
+class gg_grad_stats(C.Structure):
+    _fields_ = [("forward_ms", C.c_double), ("adjoint_ms", C.c_double), ("total_ms", C.c_double),
+                ("rays", C.c_uint64), ("disk_rays", C.c_uint64), ("steps", C.c_uint64), ("attempts", C.c_uint64),
+                ("checkpoints", C.c_uint64), ("checkpoint_bytes", C.c_uint64), ("segment_bytes", C.c_uint64),
+                ("grid_blocks", C.c_uint32), ("block_threads", C.c_uint32)]
+
+
 # Entry points declared in include/gargantua.h: (name, restype, argtypes).
 _P = C.c_void_p
 EXPORTS = [
@@ -102,8 +109,10 @@ EXPORTS = [
     ("gg_geodesic_acceleration", C.c_int, [_P, C.POINTER(gg_metric), C.c_int, C.c_uint64, _P, _P, _P]),
     ("gg_autotune", C.c_int, [_P, C.POINTER(gg_scene), C.c_uint32, C.c_int, C.POINTER(gg_launch), C.c_uint32,
                               C.POINTER(gg_launch), C.POINTER(C.c_double)]),
-    ("gg_grad", C.c_int, [_P, C.POINTER(gg_grad_scene), _P, C.c_uint32, C.POINTER(C.c_double),
-                          C.POINTER(C.c_double), _P, C.POINTER(gg_stats)]),
+    ("gg_grad_scene_to_scene", C.c_int, [C.POINTER(gg_grad_scene), C.POINTER(gg_scene)]),
+    ("gg_grad", C.c_int, [_P, C.POINTER(gg_grad_scene), _P, C.c_uint32, C.POINTER(C.c_double), _P, _P, _P,
+                          C.POINTER(gg_grad_stats)]),
+    ("gg_geodesic_acceleration_vjp", C.c_int, [_P, C.POINTER(gg_metric), C.c_uint64, _P, _P, _P, _P, _P, _P]),
 ]
 
 LIB_NAME = "libgargantua.so"

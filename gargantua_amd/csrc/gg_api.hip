@@ -90,6 +90,9 @@ struct gg_context {
   DevBuf stage[9];       // device staging for host-memory outputs
   DevBuf rhs_io;
   std::vector<DevBuf> batch_stage;
+  // gg_grad: cls, intensity, accepted, target, resid, dI, list, counters,
+  // results, scene, checkpoint pool, segment buffers
+  DevBuf grad[12];
 };
 
 namespace {
@@ -386,6 +389,7 @@ int gg_context_destroy(gg_context *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   ctx->scenes.release(); ctx->outs.release(); ctx->counters.release(); ctx->states_in.release();
   ctx->rhs_io.release();
+  for (auto &b : ctx->grad) b.release();
   for (auto &b : ctx->stage) b.release();
   for (auto &b : ctx->batch_stage) b.release();
   if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
@@ -674,11 +678,199 @@ int gg_geodesic_acceleration(gg_context *ctx, const gg_metric *metric, int preci
   return GG_OK;
 }
 
+int gg_grad_scene_to_scene(const gg_grad_scene *g, gg_scene *s) {
+  if (!g || !s) return fail(GG_ERR_INVALID_ARG, "NULL argument");
+  if (!(g->mass > 0.0) || !std::isfinite(g->mass)) return fail(GG_ERR_INVALID_ARG, "mass must be positive and finite");
+  if (!std::isfinite(g->spin) || !(std::fabs(g->spin) < 0.9999 * g->mass))
+    return fail(GG_ERR_INVALID_ARG, "|spin| must be < 0.9999 mass (Kerr::new clamps beyond, without derivative)");
+  if (!(g->inclination > 0.0) || !(g->inclination < M_PI))
+    return fail(GG_ERR_INVALID_ARG, "inclination must be in (0, pi)");
+  double r_h = g->mass + std::sqrt(g->mass * g->mass - g->spin * g->spin);
+  if (!(g->r_camera > r_h) || !std::isfinite(g->r_camera))
+    return fail(GG_ERR_INVALID_ARG, "r_camera must be outside the horizon");
+  if (!std::isfinite(g->emissivity_q) || !std::isfinite(g->r_inner) || !(g->r_outer > 0.0) || !std::isfinite(g->r_outer))
+    return fail(GG_ERR_INVALID_ARG, "disk q, r_inner, r_outer must be finite (r_outer > 0)");
+  if (!(g->edge_width >= 0.0) || !std::isfinite(g->edge_width)) return fail(GG_ERR_INVALID_ARG, "edge_width must be >= 0");
+  if (g->width == 0 || g->height == 0 || g->width > 65536 || g->height > 65536)
+    return fail(GG_ERR_INVALID_ARG, "width/height must be in [1, 65536]");
+  std::memset(s, 0, sizeof(*s));
+  s->metric.kind = GG_METRIC_KERR;
+  s->metric.mass = g->mass;
+  s->metric.spin = g->spin;
+  const double st = std::sin(g->inclination), ct = std::cos(g->inclination);
+  s->camera.position[1] = g->r_camera;
+  s->camera.position[2] = g->inclination;
+  s->camera.look_at[0] = -(st * 1.0);
+  s->camera.look_at[1] = -(st * 0.0);
+  s->camera.look_at[2] = -ct;
+  s->camera.up[2] = 1.0;
+  s->camera.fov_y_radians = g->fov_y_radians;
+  s->camera.aspect = (double)g->width / (double)g->height;
+  s->disk.kind = GG_DISK_THIN;
+  s->disk.r_inner = g->r_inner;
+  s->disk.r_outer = g->r_outer;
+  s->disk.emissivity_q = g->emissivity_q;
+  s->disk.edge_width = g->edge_width;
+  s->disk.scale_height = 1.0;
+  s->disk.density_0 = 1.0;
+  s->disk.temperature_0 = 1.0;
+  s->disk.absorption = 0.1;
+  s->integrator = g->integrator;
+  s->image.width = g->width;
+  s->image.height = g->height;
+  s->image.samples_per_axis = 1;
+  return validate_scene(s, true);
+}
+
 int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, uint32_t checkpoint_every,
-            double *loss, double grad[4], double *intensity, gg_stats *stats) {
-  (void)ctx; (void)scene; (void)target; (void)checkpoint_every; (void)loss; (void)grad; (void)intensity;
-  (void)stats;
-  return fail(GG_ERR_UNSUPPORTED, "gg_grad is not built yet");
+            double *loss, double grad[4], double *intensity, double *d_intensity, gg_grad_stats *stats) {
+  auto t_start = std::chrono::steady_clock::now();
+  if (!ctx || !scene || !target) return fail(GG_ERR_INVALID_ARG, "NULL context, scene or target");
+  if (checkpoint_every < 1 || checkpoint_every > 65536)
+    return fail(GG_ERR_INVALID_ARG, "checkpoint_every must be in [1, 65536]");
+  gg_scene sc;
+  int rc = gg_grad_scene_to_scene(scene, &sc);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const uint64_t n = (uint64_t)scene->width * scene->height;
+  const uint32_t K = checkpoint_every;
+  DevBuf *B = ctx->grad;
+  if ((rc = B[0].ensure(n)) || (rc = B[1].ensure(8 * n)) || (rc = B[2].ensure(4 * n)) ||
+      (rc = B[3].ensure(8 * n)) || (rc = B[4].ensure(8 * n)) || (rc = B[5].ensure(32 * n)) ||
+      (rc = B[6].ensure(4 * n)) || (rc = B[7].ensure(sizeof(GradCounters))) || (rc = B[8].ensure(64)) ||
+      (rc = B[9].ensure(sizeof(DevScene))))
+    return rc;
+  uint8_t *d_cls = (uint8_t *)B[0].p;
+  double *d_int = (double *)B[1].p, *d_target = (double *)B[3].p, *d_resid = (double *)B[4].p;
+  double *d_dI = (double *)B[5].p, *d_out5 = (double *)B[8].p;
+  uint32_t *d_acc = (uint32_t *)B[2].p, *d_list = (uint32_t *)B[6].p;
+  GradCounters *d_cnt = (GradCounters *)B[7].p;
+  hipStream_t s = ctx->stream;
+
+  // forward render (FP64): classes, intensities, accepted counts
+  gg_outputs o;
+  std::memset(&o, 0, sizeof(o));
+  o.cls = d_cls;
+  o.intensity = d_int;
+  o.accepted = d_acc;
+  gg_stats rs;
+  if ((rc = render_impl(ctx, &sc, 1, GG_PREC_FP64, &o, nullptr, 0, &rs))) return rc;
+
+  DevScene D;
+  build_dev_scene(sc, D);
+  HIP_TRY(hipMemcpyAsync(B[9].p, &D, sizeof(D), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d_target, target, 8 * n, hipMemcpyDefault, s));
+  HIP_TRY(launch_grad_reset(d_cnt, s));
+  GradSelect sel{n, d_cls, d_int, d_target, d_acc, K, d_resid, d_dI, d_list, d_cnt};
+  HIP_TRY(launch_grad_select(sel, s));
+  GradCounters cnt;
+  HIP_TRY(hipMemcpyAsync(&cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const uint64_t n_list = cnt.n_list, chk_total = cnt.chk_total;
+
+  // launch shape: 64-thread blocks (1 wavefront), 4 per CU, no more
+  // wavefronts than disk rays
+  const uint32_t block = 64;
+  uint64_t grid = (uint64_t)ctx->num_cus * 4;
+  const uint64_t waves = (n_list + 63) / 64;
+  if (waves < grid) grid = waves;
+  if (grid < 1) grid = 1;
+  const uint64_t n_threads = grid * block;
+  const uint64_t chk_bytes = 72 * (chk_total ? chk_total : 1);
+  const uint64_t seg_bytes = 72 * (uint64_t)K * n_threads;
+  if ((rc = B[10].ensure(chk_bytes)) || (rc = B[11].ensure(seg_bytes))) return rc;
+
+  float adj_ms = 0.0f;
+  if (n_list > 0) {
+    GradLaunch G;
+    G.scene = (const DevScene *)B[9].p;
+    G.list = d_list;
+    G.n_list = n_list;
+    G.accepted = d_acc;
+    G.chk = (double *)B[10].p;
+    G.chk_cap = chk_total;
+    G.seg = (double *)B[11].p;
+    G.K = K;
+    G.width = scene->width;
+    G.height = scene->height;
+    G.incl = scene->inclination;
+    G.r_cam = scene->r_camera;
+    G.dI = d_dI;
+    G.cnt = d_cnt;
+    HIP_TRY(hipEventRecord(ctx->ev0, s));
+    HIP_TRY(launch_adjoint(G, (uint32_t)grid, block, s));
+    HIP_TRY(hipEventRecord(ctx->ev1, s));
+  }
+  HIP_TRY(launch_grad_reduce(n, d_resid, d_dI, d_out5, s));
+  double out5[5];
+  HIP_TRY(hipMemcpyAsync(out5, d_out5, sizeof(out5), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, s));
+  if (intensity) HIP_TRY(hipMemcpyAsync(intensity, d_int, 8 * n, hipMemcpyDefault, s));
+  if (d_intensity) HIP_TRY(hipMemcpyAsync(d_intensity, d_dI, 32 * n, hipMemcpyDefault, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (n_list > 0) HIP_TRY(hipEventElapsedTime(&adj_ms, ctx->ev0, ctx->ev1));
+  if (cnt.bad != 0 || cnt.done != n_list || cnt.chk_cursor != chk_total)
+    return fail(GG_ERR_INTERNAL, "adjoint: %llu of %llu rays done, %llu left the rendered path, %llu/%llu checkpoints",
+                (unsigned long long)cnt.done, (unsigned long long)n_list, (unsigned long long)cnt.bad,
+                (unsigned long long)cnt.chk_cursor, (unsigned long long)chk_total);
+  if (loss) *loss = out5[0];
+  if (grad)
+    for (int k = 0; k < 4; ++k) grad[k] = out5[1 + k];
+  if (stats) {
+    stats->forward_ms = rs.kernel_ms;
+    stats->adjoint_ms = adj_ms;
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    stats->rays = n;
+    stats->disk_rays = n_list;
+    stats->steps = cnt.steps;
+    stats->attempts = cnt.attempts;
+    stats->checkpoints = chk_total;
+    stats->checkpoint_bytes = 72 * chk_total;
+    stats->segment_bytes = seg_bytes;
+    stats->grid_blocks = (uint32_t)grid;
+    stats->block_threads = block;
+  }
+  return GG_OK;
+}
+
+int gg_geodesic_acceleration_vjp(gg_context *ctx, const gg_metric *metric, uint64_t n, const double *pos,
+                                 const double *vel, const double *cot, double *pos_bar, double *vel_bar,
+                                 double *spin_bar) {
+  if (!ctx || !metric) return fail(GG_ERR_INVALID_ARG, "NULL argument");
+  if (n == 0) return GG_OK;
+  if (!pos || !vel || !cot || !pos_bar || !vel_bar || !spin_bar) return fail(GG_ERR_INVALID_ARG, "NULL array");
+  int rc = validate_metric(*metric);
+  if (rc) return rc;
+  if (metric->kind != GG_METRIC_KERR) return fail(GG_ERR_UNSUPPORTED, "the VJP is implemented for Kerr only");
+  HIP_TRY(hipSetDevice(ctx->device));
+  gg_scene sc;
+  std::memset(&sc, 0, sizeof(sc));
+  sc.metric = *metric;
+  sc.camera.position[1] = 10.0 * metric->mass;
+  sc.camera.position[2] = M_PI / 2.0;
+  sc.camera.look_at[0] = -1.0;
+  sc.camera.up[2] = 1.0;
+  sc.camera.fov_y_radians = 1.0;
+  sc.camera.aspect = 1.0;
+  DevScene D;
+  build_dev_scene(sc, D);
+  const size_t bytes = 32 * n;
+  if ((rc = ctx->rhs_io.ensure(6 * bytes + 8 * n + 256 * ((sizeof(DevScene) + 255) / 256)))) return rc;
+  char *base = (char *)ctx->rhs_io.p;
+  DevScene *dS = (DevScene *)base;
+  double *dp = (double *)(base + 256 * ((sizeof(DevScene) + 255) / 256));
+  double *dv = dp + 4 * n, *dw = dv + 4 * n, *dpb = dw + 4 * n, *dvb = dpb + 4 * n, *dsb = dvb + 4 * n;
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(dS, &D, sizeof(D), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dp, pos, bytes, hipMemcpyDefault, s));
+  HIP_TRY(hipMemcpyAsync(dv, vel, bytes, hipMemcpyDefault, s));
+  HIP_TRY(hipMemcpyAsync(dw, cot, bytes, hipMemcpyDefault, s));
+  HIP_TRY(launch_rhs_vjp(dS, n, dp, dv, dw, dpb, dvb, dsb, s));
+  HIP_TRY(hipMemcpyAsync(pos_bar, dpb, bytes, hipMemcpyDefault, s));
+  HIP_TRY(hipMemcpyAsync(vel_bar, dvb, bytes, hipMemcpyDefault, s));
+  HIP_TRY(hipMemcpyAsync(spin_bar, dsb, 8 * n, hipMemcpyDefault, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return GG_OK;
 }
 
 }  // extern "C"

@@ -38,6 +38,54 @@ struct TraceLaunch {
   uint32_t refill_interval;
 };
 
+// Tallies of gg_grad (zeroed by launch_grad_reset).
+struct GradCounters {
+  unsigned long long queue;        // next disk ray to hand out
+  unsigned long long chk_cursor;   // checkpoints claimed in the pool
+  unsigned long long attempts;     // attempts of re-traces and recomputations
+  unsigned long long steps;        // accepted steps back-propagated
+  unsigned long long done;         // rays whose gradient was written
+  unsigned long long bad;          // rays whose re-trace left the rendered path
+  unsigned long long rhs;          // RHS evaluations inside attempts
+  unsigned long long n_list;       // disk rays found by select
+  unsigned long long chk_total;    // checkpoints they need
+  unsigned long long pad[7];
+};
+
+struct GradSelect {
+  uint64_t n;                      // pixels
+  const uint8_t *cls;
+  const double *intensity, *target;
+  const uint32_t *accepted;
+  uint32_t K;
+  double *resid, *dI;
+  uint32_t *list;
+  GradCounters *cnt;
+};
+
+struct GradLaunch {
+  const DevScene *scene;
+  const uint32_t *list;
+  uint64_t n_list;
+  const uint32_t *accepted;        // per pixel, from the render
+  double *chk;                     // checkpoint pool, 9 doubles each
+  unsigned long long chk_cap;
+  double *seg;                     // per-thread segment buffers, K x 9 doubles
+  uint32_t K;
+  uint32_t width, height;
+  double incl, r_cam;
+  double *dI;                      // [pixels][4]
+  GradCounters *cnt;
+};
+
+size_t adjoint_lds_bytes(uint32_t block);
+hipError_t launch_adjoint(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s);
+hipError_t launch_grad_select(const GradSelect &p, hipStream_t s);
+hipError_t launch_grad_reduce(uint64_t n, const double *resid, const double *dI, double *out5, hipStream_t s);
+hipError_t launch_grad_reset(GradCounters *c, hipStream_t s);
+hipError_t launch_rhs_vjp(const DevScene *S, uint64_t n, const double *pos, const double *vel, const double *cot,
+                          double *pos_bar, double *vel_bar, double *spin_bar, hipStream_t s);
+
 // Launch the persistent trace kernel for (metric, disk, precision).
 hipError_t launch_trace(int metric_kind, int disk_kind, int precision, const TraceLaunch &t, uint32_t grid,
                         uint32_t block, hipStream_t stream);

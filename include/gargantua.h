@@ -26,6 +26,9 @@
  *   gg_grad              no reference counterpart: gradient of an image loss
  *                        through the thin-disk render (reverse mode, hand
  *                        written adjoint with checkpoints).
+ *   gg_grad_scene_to_scene, gg_geodesic_acceleration_vjp
+ *                        no reference counterpart: the scene gg_grad renders
+ *                        and the RHS vector-Jacobian product it uses.
  *
  * Errors: every entry point returns GG_OK (0) or a negative gg_status; the
  * message of the last failure on the calling thread is gg_last_error().
@@ -257,12 +260,54 @@ typedef struct gg_grad_scene {
   uint32_t height;
 } gg_grad_scene;
 
-/* Loss L = mean_p (I_p - target_p)^2 over width*height pixel-centre rays.
- * grad[0..3] = dL/d(spin, inclination, q, r_inner).  intensity (optional,
- * [N]) receives the forward image.  checkpoint_every = K accepted steps. */
+/* The gg_scene gg_grad renders: Kerr(mass, spin), camera at
+ * (t, r, theta, phi) = (0, r_camera, inclination, 0) looking at -r_hat with
+ * up = +z, aspect width/height, the thin disk (r_inner, r_outer, q,
+ * edge_width), one ray per pixel centre.  Render it with gg_render at
+ * GG_PREC_FP64 for the image gg_grad differentiates. */
+int gg_grad_scene_to_scene(const gg_grad_scene *grad_scene, gg_scene *scene);
+
+/* Statistics of one gg_grad call. */
+typedef struct gg_grad_stats {
+  double forward_ms;        /* trace kernel of the loss render                */
+  double adjoint_ms;        /* adjoint kernel                                 */
+  double total_ms;          /* whole call                                     */
+  uint64_t rays;            /* width*height                                   */
+  uint64_t disk_rays;       /* rays differentiated (class GG_CLASS_DISK)      */
+  uint64_t steps;           /* accepted steps back-propagated                 */
+  uint64_t attempts;        /* attempts re-run by the adjoint (re-trace +
+                               segment recomputation)                         */
+  uint64_t checkpoints;     /* checkpoints stored                             */
+  uint64_t checkpoint_bytes;
+  uint64_t segment_bytes;   /* per-thread segment buffers                     */
+  uint32_t grid_blocks;
+  uint32_t block_threads;
+} gg_grad_stats;
+
+/* Loss L = mean_p (I_p - target_p)^2 over the width*height pixel-centre rays
+ * of gg_grad_scene_to_scene(scene), rendered in FP64.  grad[0..3] =
+ * dL/d(spin, inclination, emissivity_q, r_inner) of that discrete algorithm
+ * with each ray's step sizes, accept/reject pattern and disk-crossing step
+ * held fixed (reverse mode; checkpoints every checkpoint_every accepted
+ * steps, segments recomputed between them; the result does not depend on
+ * checkpoint_every).  Only rays of class GG_CLASS_DISK carry a gradient.
+ * target: [N] host or device.  Optional outputs: intensity [N] (the image),
+ * d_intensity [4*N] (dI_p/d(spin, inclination, q, r_inner), AoS).
+ * Requires |spin| < 0.9999 mass (the clamp of Kerr::new has no derivative),
+ * 0 < inclination < pi, 1 <= checkpoint_every <= 65536. */
 int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target,
             uint32_t checkpoint_every, double *loss, double grad[4],
-            double *intensity, gg_stats *stats);
+            double *intensity, double *d_intensity, gg_grad_stats *stats);
+
+/* Vector-Jacobian product of gg_geodesic_acceleration for Kerr (FP64):
+ * for each point, cot[4*i..] = w, pos_bar[4*i..] = w^T da/dpos (t and phi
+ * components 0), vel_bar[4*i..] = w^T da/dvel, spin_bar[i] = w^T da/dspin.
+ * The building block of gg_grad's adjoint. */
+int gg_geodesic_acceleration_vjp(gg_context *ctx, const gg_metric *metric,
+                                 uint64_t n, const double *pos,
+                                 const double *vel, const double *cot,
+                                 double *pos_bar, double *vel_bar,
+                                 double *spin_bar);
 
 #ifdef __cplusplus
 }

