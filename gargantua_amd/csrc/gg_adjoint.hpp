@@ -5,14 +5,13 @@
 // and are held fixed; the adjoint differentiates the map from (spin,
 // inclination, emissivity index q, inner radius r_in) to each ray's disk
 // intensity through the camera ray, every accepted Dormand-Prince step and
-// the crossing.  Derivatives of the Christoffel symbols with respect to r,
-// theta and the spin come from forward-mode dual numbers (one tangent per
-// pass, so a pass costs about two RHS evaluations and little register
-// space); the velocity part of each RHS Jacobian is exact algebra because
-// a^mu = -Gamma^mu_ab v^a v^b is quadratic in v.
+// the crossing.  The per-stage RHS vector-Jacobian product is reverse mode
+// (gg_vjp.hpp); the crossing and the camera ray, evaluated once per ray,
+// are differentiated here in forward mode with one-tangent dual numbers.
 #pragma once
 
 #include "gg_physics.hpp"
+#include "gg_vjp.hpp"
 
 namespace gg {
 
@@ -62,148 +61,6 @@ __device__ inline Tn dlog(Tn a) { return {log(a.v), a.d / a.v}; }
 __device__ inline Tn dlogistic(Tn x) { return 1.0 / (1.0 + dexp(-x)); }
 // value x with tangent 1 when k == seed, else 0
 __device__ inline Tn seeded(double x, int k, int seed) { return {x, k == seed ? 1.0 : 0.0}; }
-
-// The twenty distinct Kerr Christoffel symbols (kerr.rs:100-424, the same
-// algebra as rhs64f_kerr) with one tangent, handed one at a time to
-// emit(index, Gamma) so no array of them is ever live.  Index order:
-// 001 002 013 023 100 111 122 133 112 103 200 211 222 233 212 203 301 302 313 323
-template <class F>
-__device__ inline void kerr_gamma_each(double m, Tn a, Tn r, Tn theta, F &&emit) {
-  Tn st, ct;
-  dsincos(theta, st, ct);
-  Tn a2 = a * a, r2 = r * r, sin2 = st * st, sc = st * ct;
-  Tn sigma = r2 + a2 * ct * ct;
-  double twom = 2.0 * m;
-  Tn delta = r2 - twom * r + a2;
-  Tn is = 1.0 / sigma, id = 1.0 / delta;
-  Tn is2 = is * is, isd = is * id;
-  Tn dsig_th = -2.0 * (a2 * sc);
-  Tn r2pa2 = r2 + a2;
-  Tn sig_m2r2 = sigma - 2.0 * r2;
-  Tn dtt_dr = twom * (sig_m2r2 * is2);
-  Tn dtt_dth = -(twom * (r * dsig_th * is2));
-  Tn dtp_dr = -(twom * (a * sin2 * sig_m2r2 * is2));
-  Tn dtp_dth = -(twom * (a * r * (2.0 * (sc * sigma) - sin2 * dsig_th) * is2));
-  Tn big_p = twom * (a2 * r);
-  Tn dpp_dr = 2.0 * (r * sin2) + sin2 * sin2 * (twom * (a2 * sig_m2r2 * is2));
-  Tn dpp_dth = 2.0 * (r2pa2 * sc) + big_p * (4.0 * (sin2 * sc * sigma) - sin2 * sin2 * dsig_th) * is2;
-  {
-    Tn ginv_tt = -((r2pa2 * r2pa2 - a2 * delta * sin2) * isd);
-    Tn ginv_tph = -(twom * (a * r * isd));
-    emit(0, 0.5 * (ginv_tt * dtt_dr + ginv_tph * dtp_dr));
-    emit(1, 0.5 * (ginv_tt * dtt_dth + ginv_tph * dtp_dth));
-    emit(2, 0.5 * (ginv_tt * dtp_dr + ginv_tph * dpp_dr));
-    emit(3, 0.5 * (ginv_tt * dtp_dth + ginv_tph * dpp_dth));
-    Tn ginv_phph = (delta - a2 * sin2) * isd / sin2;
-    emit(16, 0.5 * (ginv_tph * dtt_dr + ginv_phph * dtp_dr));
-    emit(17, 0.5 * (ginv_tph * dtt_dth + ginv_phph * dtp_dth));
-    emit(18, 0.5 * (ginv_tph * dtp_dr + ginv_phph * dpp_dr));
-    emit(19, 0.5 * (ginv_tph * dtp_dth + ginv_phph * dpp_dth));
-  }
-  Tn drr_dth = dsig_th * id;
-  {
-    Tn hrr = 0.5 * (delta * is);
-    Tn drr_dr = (2.0 * (r * delta) - sigma * (2.0 * r - twom)) * id * id;
-    emit(4, -(hrr * dtt_dr));
-    emit(5, hrr * drr_dr);
-    emit(6, -(hrr * (2.0 * r)));
-    emit(7, -(hrr * dpp_dr));
-    emit(8, hrr * drr_dth);
-    emit(9, -(hrr * dtp_dr));
-  }
-  Tn hth = 0.5 * is;
-  emit(10, -(hth * dtt_dth));
-  emit(11, -(hth * drr_dth));
-  emit(12, hth * dsig_th);
-  emit(13, -(hth * dpp_dth));
-  emit(14, hth * (2.0 * r));
-  emit(15, -(hth * dtp_dth));
-}
-
-// Coefficient of Gamma_i in w . a, a^mu = -Gamma^mu_ab v^a v^b with the
-// symmetric pairs merged (metric.rs:37-52).
-__device__ inline double gamma_coef(int i, d4 v, d4 w) {
-  switch (i) {
-    case 0: return -2.0 * w.x * v.x * v.y;
-    case 1: return -2.0 * w.x * v.x * v.z;
-    case 2: return -2.0 * w.x * v.y * v.w;
-    case 3: return -2.0 * w.x * v.z * v.w;
-    case 4: return -w.y * v.x * v.x;
-    case 5: return -w.y * v.y * v.y;
-    case 6: return -w.y * v.z * v.z;
-    case 7: return -w.y * v.w * v.w;
-    case 8: return -2.0 * w.y * v.y * v.z;
-    case 9: return -2.0 * w.y * v.x * v.w;
-    case 10: return -w.z * v.x * v.x;
-    case 11: return -w.z * v.y * v.y;
-    case 12: return -w.z * v.z * v.z;
-    case 13: return -w.z * v.w * v.w;
-    case 14: return -2.0 * w.z * v.y * v.z;
-    case 15: return -2.0 * w.z * v.x * v.w;
-    case 16: return -2.0 * w.w * v.x * v.y;
-    case 17: return -2.0 * w.w * v.x * v.z;
-    case 18: return -2.0 * w.w * v.y * v.w;
-    default: return -2.0 * w.w * v.z * v.w;
-  }
-}
-
-// Adds Gamma_i times the v-gradient of gamma_coef(i, v, w) to gv.
-__device__ inline void gamma_coef_dv(int i, double g, d4 v, d4 w, d4 &gv) {
-  switch (i) {
-    case 0: { double t = -2.0 * w.x * g; gv.x += t * v.y; gv.y += t * v.x; break; }
-    case 1: { double t = -2.0 * w.x * g; gv.x += t * v.z; gv.z += t * v.x; break; }
-    case 2: { double t = -2.0 * w.x * g; gv.y += t * v.w; gv.w += t * v.y; break; }
-    case 3: { double t = -2.0 * w.x * g; gv.z += t * v.w; gv.w += t * v.z; break; }
-    case 4: gv.x += -2.0 * w.y * g * v.x; break;
-    case 5: gv.y += -2.0 * w.y * g * v.y; break;
-    case 6: gv.z += -2.0 * w.y * g * v.z; break;
-    case 7: gv.w += -2.0 * w.y * g * v.w; break;
-    case 8: { double t = -2.0 * w.y * g; gv.y += t * v.z; gv.z += t * v.y; break; }
-    case 9: { double t = -2.0 * w.y * g; gv.x += t * v.w; gv.w += t * v.x; break; }
-    case 10: gv.x += -2.0 * w.z * g * v.x; break;
-    case 11: gv.y += -2.0 * w.z * g * v.y; break;
-    case 12: gv.z += -2.0 * w.z * g * v.z; break;
-    case 13: gv.w += -2.0 * w.z * g * v.w; break;
-    case 14: { double t = -2.0 * w.z * g; gv.y += t * v.z; gv.z += t * v.y; break; }
-    case 15: { double t = -2.0 * w.z * g; gv.x += t * v.w; gv.w += t * v.x; break; }
-    case 16: { double t = -2.0 * w.w * g; gv.x += t * v.y; gv.y += t * v.x; break; }
-    case 17: { double t = -2.0 * w.w * g; gv.x += t * v.z; gv.z += t * v.x; break; }
-    case 18: { double t = -2.0 * w.w * g; gv.y += t * v.w; gv.w += t * v.y; break; }
-    default: { double t = -2.0 * w.w * g; gv.z += t * v.w; gv.w += t * v.z; break; }
-  }
-}
-
-struct RhsBar {
-  double r, th, a;
-  d4 v;
-};
-
-// Vector-Jacobian product of the Kerr RHS at (r, theta, v) for spin a:
-// w^T d a / d(r, theta, v, a).  Three tangent passes over the symbols
-// (r, theta, a); the v part is exact algebra on the first pass's values.
-__device__ inline RhsBar kerr_rhs_vjp(double m, double a, double r, double th, d4 v, d4 w) {
-  RhsBar o;
-  o.v = d4{0.0, 0.0, 0.0, 0.0};
-  double acc[3];
-  // one pass per tangent, kept rolled so the passes do not interleave and
-  // multiply the live registers
-#pragma unroll 1
-  for (int p = 0; p < 3; ++p) {
-    double t = 0.0;
-    d4 gv{0.0, 0.0, 0.0, 0.0};
-    kerr_gamma_each(m, Tn{a, p == 2 ? 1.0 : 0.0}, Tn{r, p == 0 ? 1.0 : 0.0}, Tn{th, p == 1 ? 1.0 : 0.0},
-                    [&](int i, Tn g) {
-                      t += g.d * gamma_coef(i, v, w);
-                      gamma_coef_dv(i, g.v, v, w, gv);
-                    });
-    acc[p] = t;
-    if (p == 0) o.v = gv;
-  }
-  o.r = acc[0];
-  o.th = acc[1];
-  o.a = acc[2];
-  return o;
-}
 
 // Kerr metric components at (r, theta) with one tangent (kerr.rs:61-94).
 struct GmnD {

@@ -10,18 +10,16 @@
 //   rhs32_*  FP32, contracted, three reciprocals instead of fifteen
 //            divisions, symmetric pairs merged.  Used by the mixed kernel on
 //            attempts where the step-size controller is saturated (see
-//            gg_trace.hpp), where its rounding cannot change a decision.
+//            gg_rk45.hpp), where its rounding cannot change a decision.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "gargantua.h"
+#include "gg_types.hpp"
 
 namespace gg {
-
-struct d4 { double x, y, z, w; };
-struct f4 { float x, y, z, w; };
 
 // Per-scene constants, computed once on the host (with the host libm, so
 // that tan/sin/cos of the camera are the values the reference computes).
