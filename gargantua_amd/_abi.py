@@ -86,7 +86,7 @@ class gg_grad_scene(C.Structure):
 class gg_grad_stats(C.Structure):
     _fields_ = [("forward_ms", C.c_double), ("adjoint_ms", C.c_double), ("total_ms", C.c_double),
                 ("rays", C.c_uint64), ("disk_rays", C.c_uint64), ("steps", C.c_uint64), ("attempts", C.c_uint64),
-                ("checkpoints", C.c_uint64), ("checkpoint_bytes", C.c_uint64), ("segment_bytes", C.c_uint64),
+                ("checkpoints", C.c_uint64), ("checkpoint_bytes", C.c_uint64), ("segment_bytes", C.c_uint64), ("last_segment_bytes", C.c_uint64),
                 ("grid_blocks", C.c_uint32), ("block_threads", C.c_uint32)]
 
 

@@ -91,8 +91,9 @@ struct gg_context {
   DevBuf rhs_io;
   std::vector<DevBuf> batch_stage;
   // gg_grad: cls, intensity, accepted, target, resid, dI, list, counters,
-  // results, scene, checkpoint pool, segment buffers
-  DevBuf grad[12];
+  // results, scene, checkpoint pool, segment buffers, keys, sorted list,
+  // sorted keys, ray records, last-segment records, sort scratch
+  DevBuf grad[18];
 };
 
 namespace {
@@ -738,12 +739,14 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
   if ((rc = B[0].ensure(n)) || (rc = B[1].ensure(8 * n)) || (rc = B[2].ensure(4 * n)) ||
       (rc = B[3].ensure(8 * n)) || (rc = B[4].ensure(8 * n)) || (rc = B[5].ensure(32 * n)) ||
       (rc = B[6].ensure(4 * n)) || (rc = B[7].ensure(sizeof(GradCounters))) || (rc = B[8].ensure(64)) ||
-      (rc = B[9].ensure(sizeof(DevScene))))
+      (rc = B[9].ensure(sizeof(DevScene))) || (rc = B[12].ensure(4 * n)) || (rc = B[13].ensure(4 * n)) ||
+      (rc = B[14].ensure(4 * n)))
     return rc;
   uint8_t *d_cls = (uint8_t *)B[0].p;
   double *d_int = (double *)B[1].p, *d_target = (double *)B[3].p, *d_resid = (double *)B[4].p;
   double *d_dI = (double *)B[5].p, *d_out5 = (double *)B[8].p;
-  uint32_t *d_acc = (uint32_t *)B[2].p, *d_list = (uint32_t *)B[6].p;
+  uint32_t *d_acc = (uint32_t *)B[2].p, *d_list_in = (uint32_t *)B[6].p, *d_keys_in = (uint32_t *)B[12].p;
+  uint32_t *d_list = (uint32_t *)B[13].p, *d_keys = (uint32_t *)B[14].p;
   GradCounters *d_cnt = (GradCounters *)B[7].p;
   hipStream_t s = ctx->stream;
 
@@ -761,34 +764,47 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
   HIP_TRY(hipMemcpyAsync(B[9].p, &D, sizeof(D), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(d_target, target, 8 * n, hipMemcpyDefault, s));
   HIP_TRY(launch_grad_reset(d_cnt, s));
-  GradSelect sel{n, d_cls, d_int, d_target, d_acc, K, d_resid, d_dI, d_list, d_cnt};
+  GradSelect sel{n, d_cls, d_int, d_target, d_acc, K, d_resid, d_dI, d_list_in, d_keys_in, d_cnt};
   HIP_TRY(launch_grad_select(sel, s));
   GradCounters cnt;
   HIP_TRY(hipMemcpyAsync(&cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  const uint64_t n_list = cnt.n_list, chk_total = cnt.chk_total;
+  const uint64_t n_list = cnt.n_list, chk_total = cnt.chk_total, tail_total = cnt.tail_total;
 
-  // launch shape: 64-thread blocks (1 wavefront), 4 per CU, no more
-  // wavefronts than disk rays
-  const uint32_t block = 64;
-  uint64_t grid = (uint64_t)ctx->num_cus * 4;
+  // launch shapes: forward 256-thread blocks, 2 per CU; reverse 64-thread
+  // blocks (its LDS and registers allow one wavefront per SIMD), 4 per CU;
+  // never more wavefronts than disk rays
   const uint64_t waves = (n_list + 63) / 64;
-  if (waves < grid) grid = waves;
-  if (grid < 1) grid = 1;
-  const uint64_t n_threads = grid * block;
+  uint64_t grid_f = (uint64_t)ctx->num_cus * 2, grid_r = (uint64_t)ctx->num_cus * 4;
+  if ((waves + 3) / 4 < grid_f) grid_f = (waves + 3) / 4;
+  if (waves < grid_r) grid_r = waves;
+  if (grid_f < 1) grid_f = 1;
+  if (grid_r < 1) grid_r = 1;
+  const uint64_t n_threads = grid_r * 64;
   const uint64_t chk_bytes = 72 * (chk_total ? chk_total : 1);
+  const uint64_t tail_bytes = 72 * (tail_total ? tail_total : 1);
   const uint64_t seg_bytes = 72 * (uint64_t)K * n_threads;
-  if ((rc = B[10].ensure(chk_bytes)) || (rc = B[11].ensure(seg_bytes))) return rc;
+  if ((rc = B[10].ensure(chk_bytes)) || (rc = B[11].ensure(seg_bytes)) || (rc = B[15].ensure(8 * GRAD_REC * (n_list + 1))) ||
+      (rc = B[16].ensure(tail_bytes)))
+    return rc;
 
   float adj_ms = 0.0f;
   if (n_list > 0) {
+    int key_bits = 1;
+    while (key_bits < 32 && (sc.integrator.max_steps >> key_bits) != 0) ++key_bits;
+    size_t temp_bytes = 0;
+    HIP_TRY(sort_rays_desc(d_keys_in, d_keys, d_list_in, d_list, n_list, key_bits, nullptr, temp_bytes, s));
+    if ((rc = B[17].ensure(temp_bytes ? temp_bytes : 1))) return rc;
     GradLaunch G;
     G.scene = (const DevScene *)B[9].p;
     G.list = d_list;
+    G.keys = d_keys;
     G.n_list = n_list;
-    G.accepted = d_acc;
+    G.rec = (double *)B[15].p;
     G.chk = (double *)B[10].p;
     G.chk_cap = chk_total;
+    G.tail = (double *)B[16].p;
+    G.tail_cap = tail_total;
     G.seg = (double *)B[11].p;
     G.K = K;
     G.width = scene->width;
@@ -798,7 +814,10 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
     G.dI = d_dI;
     G.cnt = d_cnt;
     HIP_TRY(hipEventRecord(ctx->ev0, s));
-    HIP_TRY(launch_adjoint(G, (uint32_t)grid, block, s));
+    HIP_TRY(sort_rays_desc(d_keys_in, d_keys, d_list_in, d_list, n_list, key_bits, B[17].p, temp_bytes, s));
+    HIP_TRY(launch_adjoint_fwd(G, (uint32_t)grid_f, 256, s));
+    HIP_TRY(launch_adjoint_rev(G, (uint32_t)grid_r, 64, s));
+    HIP_TRY(launch_adjoint_cam(G, s));
     HIP_TRY(hipEventRecord(ctx->ev1, s));
   }
   HIP_TRY(launch_grad_reduce(n, d_resid, d_dI, d_out5, s));
@@ -809,10 +828,13 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
   if (d_intensity) HIP_TRY(hipMemcpyAsync(d_intensity, d_dI, 32 * n, hipMemcpyDefault, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (n_list > 0) HIP_TRY(hipEventElapsedTime(&adj_ms, ctx->ev0, ctx->ev1));
-  if (cnt.bad != 0 || cnt.done != n_list || cnt.chk_cursor != chk_total)
-    return fail(GG_ERR_INTERNAL, "adjoint: %llu of %llu rays done, %llu left the rendered path, %llu/%llu checkpoints",
+  if (cnt.bad != 0 || cnt.done != n_list || cnt.chk_cursor != chk_total || cnt.tail_cursor != tail_total)
+    return fail(GG_ERR_INTERNAL,
+                "adjoint: %llu of %llu rays done, %llu left the rendered path, %llu/%llu checkpoints, %llu/%llu "
+                "step records",
                 (unsigned long long)cnt.done, (unsigned long long)n_list, (unsigned long long)cnt.bad,
-                (unsigned long long)cnt.chk_cursor, (unsigned long long)chk_total);
+                (unsigned long long)cnt.chk_cursor, (unsigned long long)chk_total,
+                (unsigned long long)cnt.tail_cursor, (unsigned long long)tail_total);
   if (loss) *loss = out5[0];
   if (grad)
     for (int k = 0; k < 4; ++k) grad[k] = out5[1 + k];
@@ -827,8 +849,9 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
     stats->checkpoints = chk_total;
     stats->checkpoint_bytes = 72 * chk_total;
     stats->segment_bytes = seg_bytes;
-    stats->grid_blocks = (uint32_t)grid;
-    stats->block_threads = block;
+    stats->last_segment_bytes = 72 * tail_total;
+    stats->grid_blocks = (uint32_t)grid_r;
+    stats->block_threads = 64;
   }
   return GG_OK;
 }

@@ -5,23 +5,26 @@
 // sequence, accept/reject pattern and disk-crossing step held at the values
 // of the forward render (discretise-then-differentiate).
 //
-// Per disk-hitting ray, one lane:
-//   FWD    re-traces the ray from the camera with the trace kernel's own
-//          attempt() (bit-identical steps), storing a checkpoint (pos, vel,
-//          h) every K accepted steps in a pool sized exactly from the
-//          render's accepted counts, and every accepted step's start state
-//          in a K-deep per-lane segment buffer; at the render's accepted
-//          count the step is the disk crossing, whose intensity is
-//          differentiated in forward mode (12 inputs, once per ray).
-//   REV    back-propagates one accepted step per iteration through its
-//          Dormand-Prince stages (the six k_i are recomputed, their
-//          cotangents live in LDS) from the end of the current segment to
-//          its start.
-//   RECOMP re-runs the segment before it from its checkpoint into the
-//          segment buffer (rejections included, so h repeats exactly).
-// Lanes in different phases share a wavefront: a lane takes one attempt or
-// one step adjoint per iteration and a new ray from the queue when done.
+// The disk-hitting rays, sorted by accepted-step count (longest first, so
+// the long photon-ring rays start early and neighbouring lanes stay in the
+// same phase), go through three kernels, one lane per ray:
+//   adjoint_fwd_kernel  re-traces the ray from the camera with the trace
+//       kernel's own attempt() (bit-identical steps), storing a checkpoint
+//       (pos, vel, h) every K accepted steps in a pool sized exactly from
+//       the render's accepted counts, and the start state and h of each step
+//       of the ray's last segment; at the render's accepted count the step is
+//       the disk crossing, whose intensity is differentiated in forward mode
+//       (12 inputs): the seeds of the reverse sweep.
+//   adjoint_rev_kernel  back-propagates one accepted step per iteration
+//       through its Dormand-Prince stages (the six k_i are recomputed, their
+//       cotangents live in LDS), segment by segment from the last; before
+//       each earlier segment it re-runs that segment from its checkpoint into
+//       a per-lane segment buffer (rejections included, so h repeats exactly,
+//       and the end state must equal the next checkpoint bit for bit).
+//   adjoint_cam_kernel  adds the camera ray's dependence on (a, i) and
+//       writes dI/d(a, i, q, r_in) per pixel.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include "gg_adjoint.hpp"
 #include "gg_internal.hpp"
@@ -134,7 +137,150 @@ __device__ inline void get_state(const double *src, double p[4], d4 &v, double &
   h = src[8];
 }
 
-__global__ __launch_bounds__(64) void adjoint_kernel(GradLaunch G) {
+// Per-ray record (REC doubles at list position q): the reverse sweep's seeds
+// and bookkeeping, then its results.
+enum {
+  R_PBR = 0, R_PBTH, R_VB0, R_VB1, R_VB2, R_VB3, R_GA, R_GI, R_GQ, R_GRIN,   // cotangents
+  R_XR0, R_XTH0, R_XV0T, R_XV0P,                                            // crossing's direct terms
+  R_CHK, R_TAIL,                                                            // pool offsets (bits)
+  REC
+};
+static_assert(REC == GRAD_REC, "record layout");
+
+__device__ inline uint64_t as_u64(double x) { return (uint64_t)__double_as_longlong(x); }
+__device__ inline double as_f64(uint64_t x) { return __longlong_as_double((long long)x); }
+
+// Wavefront-aggregated queue: idle lanes of the wavefront take consecutive
+// list positions with one atomicAdd.  Returns the position or ~0 (none).
+__device__ inline uint64_t take(unsigned long long *queue, uint64_t n, bool idle, bool &exhausted) {
+  const uint32_t lane = threadIdx.x & 63;
+  const unsigned long long m = __ballot(idle);
+  uint64_t q = ~0ull;
+  if (!exhausted && m) {
+    const uint32_t cnt = __popcll(m);
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if ((int)lane == leader) base = atomicAdd(queue, (unsigned long long)cnt);
+    base = __shfl(base, leader, 64);
+    if (base + cnt >= n) exhausted = true;
+    const uint64_t mine = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (idle && mine < n) q = mine;
+  }
+  return q;
+}
+
+// A ray whose re-trace left the rendered path: NaN gradient (the host
+// reports the call as failed) and a NaN seed so later kernels skip it.
+__device__ inline void mark_bad(const GradLaunch &G, uint64_t q, uint32_t pix, unsigned long long &n_bad) {
+  double *o = G.dI + 4 * (uint64_t)pix;
+  const double nan = __longlong_as_double(0x7ff8000000000000ll);
+  o[0] = nan; o[1] = nan; o[2] = nan; o[3] = nan;
+  G.rec[REC * q + R_PBR] = nan;
+  ++n_bad;
+}
+
+__global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
+#pragma clang fp contract(off)
+  extern __shared__ double lds_k[];
+  const KStore K{lds_k + threadIdx.x, blockDim.x};
+  const uint32_t lane = threadIdx.x & 63;
+  const DevScene &S = *G.scene;
+  const uint32_t KK = G.K;
+  Lane L;
+  L.n_rhs = 0; L.n_rhs32 = 0; L.n_full = 0;
+  bool active = false, exhausted = false;
+  uint64_t q = 0, chk_base = 0, tail_base = 0;
+  uint32_t pix = 0, acc_target = 0, last0 = 0, att = 0;
+  unsigned long long n_att = 0, n_bad = 0;
+  for (;;) {
+    const uint64_t t = take(&G.cnt->queue, G.n_list, !active, exhausted);
+    if (t != ~0ull) {
+      q = t;
+      pix = G.list[q];
+      acc_target = G.keys[q];
+      const uint32_t nchk = (acc_target + KK - 1) / KK;
+      last0 = (nchk - 1) * KK;
+      chk_base = atomicAdd(&G.cnt->chk_cursor, (unsigned long long)nchk);
+      tail_base = atomicAdd(&G.cnt->tail_cursor, (unsigned long long)(acc_target - last0));
+      camera_ray(S, pix % G.width, pix / G.width, L.pos, L.vel);
+      L.lambda = 0.0;
+      L.h = S.h0;
+      L.k1_ok = false; L.k1_f32 = false;
+      L.accepted = 0;
+      L.err_prev = 1.0;
+      att = 0;
+      active = true;
+      if (acc_target == 0 || chk_base + nchk > G.chk_cap || tail_base + (acc_target - last0) > G.tail_cap) {
+        mark_bad(G, q, pix, n_bad);
+        active = false;
+      } else {
+        put_state(G.chk + 9 * chk_base, L.pos, L.vel, L.h);
+      }
+    }
+    if (exhausted && __all(!active)) break;
+    if (!active) continue;
+
+    const double h_used = L.h;
+    bool acc;
+    double p0[4];
+    d4 v0;
+    const int res = attempt<1, 0>(S, L, K, acc, p0, v0, false);
+    ++n_att;
+    if (res != RES_CONTINUE || ++att > S.max_steps) {   // the render reached the crossing; a re-trace must too
+      mark_bad(G, q, pix, n_bad);
+      active = false;
+      continue;
+    }
+    if (!acc) continue;
+    const uint32_t g = L.accepted++;
+    if (g >= last0) put_state(G.tail + 9 * (tail_base + (g - last0)), p0, v0, h_used);
+    if (L.accepted < acc_target) {
+      if (L.accepted % KK == 0) put_state(G.chk + 9 * (chk_base + L.accepted / KK), L.pos, L.vel, L.h);
+      continue;
+    }
+    // the disk crossing of the last accepted step (same test as the trace kernel)
+    active = false;
+    const bool side0 = p0[2] < HALF_PI, side1 = L.pos[2] < HALF_PI;
+    double rc = 0.0;
+    if (side0 != side1) {
+      double s = (HALF_PI - p0[2]) / (L.pos[2] - p0[2]);
+      rc = p0[1] + s * (L.pos[1] - p0[1]);
+    }
+    const double w = S.edge_width;
+    const bool hit = (side0 != side1) &&
+                     ((w > 0.0) ? (rc <= S.r_out + 8.0 * w && rc > 0.0) : (rc >= S.r_in && rc <= S.r_out));
+    if (!hit) {
+      mark_bad(G, q, pix, n_bad);
+      continue;
+    }
+    const CrossVals x{S.spin, S.q, S.r_in, S.sqrt_neg_gtt_obs, p0[1], p0[2], L.pos[1], L.pos[2],
+                      v0.x, v0.w, L.vel.x, L.vel.w};
+    double *rec = G.rec + REC * q;
+    double d[CX_N];
+#pragma unroll 1
+    for (int k = 0; k < CX_N; ++k) d[k] = crossing_intensity_d(S.mass, S.r_out, w, x, k).d;
+    const Tn so_a = kerr_sqrt_obs_d(S.mass, Tn{S.spin, 1.0}, G.r_cam, dconst(G.incl));
+    const Tn so_i = kerr_sqrt_obs_d(S.mass, dconst(S.spin), G.r_cam, Tn{G.incl, 1.0});
+    rec[R_PBR] = d[CX_R1];
+    rec[R_PBTH] = d[CX_TH1];
+    rec[R_VB0] = d[CX_V1T]; rec[R_VB1] = 0.0; rec[R_VB2] = 0.0; rec[R_VB3] = d[CX_V1P];
+    rec[R_GA] = d[CX_A] + d[CX_SOBS] * so_a.d;
+    rec[R_GI] = d[CX_SOBS] * so_i.d;
+    rec[R_GQ] = d[CX_Q];
+    rec[R_GRIN] = d[CX_RIN];
+    rec[R_XR0] = d[CX_R0]; rec[R_XTH0] = d[CX_TH0]; rec[R_XV0T] = d[CX_V0T]; rec[R_XV0P] = d[CX_V0P];
+    rec[R_CHK] = as_f64(chk_base);
+    rec[R_TAIL] = as_f64(tail_base);
+  }
+  unsigned long long a = wave_sum(n_att), bd = wave_sum(n_bad), r = wave_sum(L.n_rhs);
+  if (lane == 0) {
+    atomicAdd(&G.cnt->attempts, a);
+    atomicAdd(&G.cnt->bad, bd);
+    atomicAdd(&G.cnt->rhs, r);
+  }
+}
+
+__global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
 #pragma clang fp contract(off)
   extern __shared__ double lds_k[];
   const KStore K{lds_k + threadIdx.x, blockDim.x};
@@ -146,135 +292,83 @@ __global__ __launch_bounds__(64) void adjoint_kernel(GradLaunch G) {
 
   Lane L;
   L.n_rhs = 0; L.n_rhs32 = 0; L.n_full = 0;
-  int phase = PH_IDLE;
-  uint32_t pix = 0, acc_target = 0, nchk = 0, c = 0, j = 0, att_phase = 0;
-  uint64_t chk_base = 0;
-  double pb_r = 0, pb_th = 0, ga = 0, gi = 0, gq = 0, grin = 0;
+  enum { IDLE = 0, REV = 1, RECOMP = 2 };
+  int phase = IDLE;
+  bool exhausted = false, first = false;
+  uint64_t q = 0, chk_base = 0;
+  uint32_t pix = 0, c = 0, j = 0, att = 0;
+  const double *src = nullptr;
+  double pb_r = 0, pb_th = 0, ga = 0;
   d4 vb{0, 0, 0, 0};
-  double x_r0 = 0, x_th0 = 0, x_v0t = 0, x_v0p = 0;   // crossing's direct cotangents on the last step's start
-  bool first_rev = false;
-  bool exhausted = false;
   unsigned long long n_att = 0, n_adj = 0, n_done = 0, n_bad = 0;
 
-  auto fail_ray = [&]() {
-    double *o = G.dI + 4 * (uint64_t)pix;
-    const double nan = __longlong_as_double(0x7ff8000000000000ll);
-    o[0] = nan; o[1] = nan; o[2] = nan; o[3] = nan;
-    ++n_bad;
-    phase = PH_IDLE;
-  };
-
   for (;;) {
-    // ---- hand idle lanes new rays: one atomicAdd per wavefront ----
-    const unsigned long long idle = __ballot(phase == PH_IDLE);
-    if (!exhausted && idle) {
-      const uint32_t cnt = __popcll(idle);
-      const int leader = __ffsll((long long)idle) - 1;
-      unsigned long long base = 0;
-      if ((int)lane == leader) base = atomicAdd(&G.cnt->queue, (unsigned long long)cnt);
-      base = __shfl(base, leader, 64);
-      if (base + cnt >= G.n_list) exhausted = true;
-      const uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
-      if (phase == PH_IDLE && base + rank < G.n_list) {
-        pix = G.list[base + rank];
-        acc_target = G.accepted[pix];
-        nchk = (acc_target + KK - 1) / KK;
-        chk_base = atomicAdd(&G.cnt->chk_cursor, (unsigned long long)nchk);
-        const uint32_t px = pix % G.width, py = pix / G.width;
-        camera_ray(S, px, py, L.pos, L.vel);
-        L.lambda = 0.0;
-        L.h = S.h0;
-        L.k1_ok = false; L.k1_f32 = false;
-        L.attempts = 0; L.accepted = 0;
-        L.err_prev = 1.0;
-        pb_r = pb_th = ga = gi = gq = grin = 0.0;
-        vb = d4{0, 0, 0, 0};
-        att_phase = 0;
-        phase = PH_FWD;
-        if (acc_target == 0 || chk_base + nchk > G.chk_cap) fail_ray();
-        else put_state(G.chk + 9 * chk_base, L.pos, L.vel, L.h);
+    const uint64_t t = take(&G.cnt->queue2, G.n_list, phase == IDLE, exhausted);
+    if (t != ~0ull) {
+      q = t;
+      pix = G.list[q];
+      const uint32_t acc_target = G.keys[q];
+      const double *rec = G.rec + REC * q;
+      if (!(rec[R_PBR] == rec[R_PBR])) {   // flagged by the forward pass
+        phase = IDLE;
+      } else {
+        const uint32_t nchk = (acc_target + KK - 1) / KK;
+        c = nchk - 1;
+        j = acc_target - c * KK - 1;
+        chk_base = as_u64(rec[R_CHK]);
+        src = G.tail + 9 * as_u64(rec[R_TAIL]);
+        pb_r = rec[R_PBR]; pb_th = rec[R_PBTH];
+        vb = d4{rec[R_VB0], rec[R_VB1], rec[R_VB2], rec[R_VB3]};
+        ga = rec[R_GA];
+        first = true;
+        phase = REV;
       }
     }
-    if (exhausted && __all(phase == PH_IDLE)) break;
+    if (exhausted && __all(phase == IDLE)) break;
 
-    // ---- FWD / RECOMP: one attempt ----
-    if (phase == PH_FWD || phase == PH_RECOMP) {
+    // ---- RECOMP: one attempt of the segment before ----
+    if (phase == RECOMP) {
       const double h_used = L.h;
       bool acc;
       double p0[4];
       d4 v0;
       const int res = attempt<1, 0>(S, L, K, acc, p0, v0, false);
       ++n_att;
-      ++att_phase;
-      if (res != RES_CONTINUE || att_phase > S.max_steps) {
-        fail_ray();   // the render reached the crossing; a re-trace must too
+      if (res != RES_CONTINUE || ++att > S.max_steps) {
+        mark_bad(G, q, pix, n_bad);
+        phase = IDLE;
       } else if (acc) {
         const uint32_t g = L.accepted++;
-        put_state(seg + 9 * (uint64_t)(g % KK), p0, v0, h_used);
-        if (phase == PH_FWD) {
-          if (L.accepted == acc_target) {
-            // the disk crossing (same test as the trace kernel)
-            const bool side0 = p0[2] < HALF_PI, side1 = L.pos[2] < HALF_PI;
-            double rc = 0.0;
-            if (side0 != side1) {
-              double s = (HALF_PI - p0[2]) / (L.pos[2] - p0[2]);
-              rc = p0[1] + s * (L.pos[1] - p0[1]);
-            }
-            const double w = S.edge_width;
-            const bool hit = (side0 != side1) &&
-                             ((w > 0.0) ? (rc <= S.r_out + 8.0 * w && rc > 0.0) : (rc >= S.r_in && rc <= S.r_out));
-            if (!hit) {
-              fail_ray();
-            } else {
-              CrossVals x{S.spin, S.q, S.r_in, S.sqrt_neg_gtt_obs, p0[1], p0[2], L.pos[1], L.pos[2],
-                          v0.x, v0.w, L.vel.x, L.vel.w};
-              double d[CX_N];
-#pragma unroll 1
-              for (int k = 0; k < CX_N; ++k) d[k] = crossing_intensity_d(S.mass, S.r_out, w, x, k).d;
-              const Tn so_a = kerr_sqrt_obs_d(S.mass, Tn{S.spin, 1.0}, G.r_cam, dconst(G.incl));
-              const Tn so_i = kerr_sqrt_obs_d(S.mass, dconst(S.spin), G.r_cam, Tn{G.incl, 1.0});
-              ga = d[CX_A] + d[CX_SOBS] * so_a.d;
-              gi = d[CX_SOBS] * so_i.d;
-              gq = d[CX_Q];
-              grin = d[CX_RIN];
-              pb_r = d[CX_R1];
-              pb_th = d[CX_TH1];
-              vb = d4{d[CX_V1T], 0.0, 0.0, d[CX_V1P]};
-              x_r0 = d[CX_R0]; x_th0 = d[CX_TH0]; x_v0t = d[CX_V0T]; x_v0p = d[CX_V0P];
-              c = nchk - 1;
-              j = acc_target - 1 - c * KK;
-              first_rev = true;
-              phase = PH_REV;
-            }
-          } else if (L.accepted % KK == 0) {
-            put_state(G.chk + 9 * (chk_base + L.accepted / KK), L.pos, L.vel, L.h);
-          }
-        } else if (L.accepted == (c + 1) * KK) {
+        put_state(seg + 9 * (uint64_t)(g - c * KK), p0, v0, h_used);
+        if (L.accepted == (c + 1) * KK) {
           // the recomputed segment must land bit for bit on the next checkpoint
-          double q[4], hq;
+          double qp[4], hq;
           d4 vq;
-          get_state(G.chk + 9 * (chk_base + c + 1), q, vq, hq);
-          if (q[0] != L.pos[0] || q[1] != L.pos[1] || q[2] != L.pos[2] || q[3] != L.pos[3] || vq.x != L.vel.x ||
-              vq.y != L.vel.y || vq.z != L.vel.z || vq.w != L.vel.w || hq != L.h) {
-            fail_ray();
+          get_state(G.chk + 9 * (chk_base + c + 1), qp, vq, hq);
+          if (qp[0] != L.pos[0] || qp[1] != L.pos[1] || qp[2] != L.pos[2] || qp[3] != L.pos[3] ||
+              vq.x != L.vel.x || vq.y != L.vel.y || vq.z != L.vel.z || vq.w != L.vel.w || hq != L.h) {
+            mark_bad(G, q, pix, n_bad);
+            phase = IDLE;
           } else {
             j = KK - 1;
-            phase = PH_REV;
+            src = seg;
+            phase = REV;
           }
         }
       }
     }
 
     // ---- REV: one accepted step backwards ----
-    if (phase == PH_REV) {
+    if (phase == REV) {
       double p[4], h;
       d4 v;
-      get_state(seg + 9 * (uint64_t)j, p, v, h);
+      get_state(src + 9 * (uint64_t)j, p, v, h);
       step_adjoint(S, K, p, v, h, pb_r, pb_th, vb, ga);
       ++n_adj;
-      if (first_rev) {
-        pb_r += x_r0; pb_th += x_th0; vb.x += x_v0t; vb.w += x_v0p;
-        first_rev = false;
+      if (first) {
+        const double *rec = G.rec + REC * q;
+        pb_r += rec[R_XR0]; pb_th += rec[R_XTH0]; vb.x += rec[R_XV0T]; vb.w += rec[R_XV0P];
+        first = false;
       }
       if (j > 0) {
         --j;
@@ -283,22 +377,15 @@ __global__ __launch_bounds__(64) void adjoint_kernel(GradLaunch G) {
         get_state(G.chk + 9 * (chk_base + c), L.pos, L.vel, L.h);
         L.k1_ok = false; L.k1_f32 = false;
         L.accepted = c * KK;
-        att_phase = 0;
-        phase = PH_RECOMP;
+        att = 0;
+        phase = RECOMP;
       } else {
-        // camera: pos0 = (0, r_cam, i, 0), vel0 from Camera::pixel_direction
-        // and GeodesicRay::from_camera
-        const uint32_t px = pix % G.width, py = pix / G.width;
-        const CamD ca = incl_camera_ray_d(S.mass, Tn{S.spin, 1.0}, dconst(G.incl), G.r_cam, S.half_h, S.half_w,
-                                          px, py, G.width, G.height);
-        const CamD ci = incl_camera_ray_d(S.mass, dconst(S.spin), Tn{G.incl, 1.0}, G.r_cam, S.half_h, S.half_w,
-                                          px, py, G.width, G.height);
-        ga += vb.x * ca.k[0].d + vb.y * ca.k[1].d + vb.z * ca.k[2].d + vb.w * ca.k[3].d;
-        gi += pb_th + vb.x * ci.k[0].d + vb.y * ci.k[1].d + vb.z * ci.k[2].d + vb.w * ci.k[3].d;
-        double *o = G.dI + 4 * (uint64_t)pix;
-        o[0] = ga; o[1] = gi; o[2] = gq; o[3] = grin;
+        double *rec = G.rec + REC * q;
+        rec[R_PBR] = pb_r; rec[R_PBTH] = pb_th;
+        rec[R_VB0] = vb.x; rec[R_VB1] = vb.y; rec[R_VB2] = vb.z; rec[R_VB3] = vb.w;
+        rec[R_GA] = ga;
         ++n_done;
-        phase = PH_IDLE;
+        phase = IDLE;
       }
     }
   }
@@ -313,31 +400,64 @@ __global__ __launch_bounds__(64) void adjoint_kernel(GradLaunch G) {
   }
 }
 
+// The camera ray's dependence on (a, i): pos0 = (0, r_cam, i, 0), vel0 from
+// Camera::pixel_direction and GeodesicRay::from_camera; dI per pixel.
+__global__ __launch_bounds__(256) void adjoint_cam_kernel(GradLaunch G) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= G.n_list) return;
+  const uint32_t pix = G.list[q];
+  const double *rec = G.rec + REC * q;
+  const double *o0 = G.dI + 4 * (uint64_t)pix;
+  if (!(o0[0] == o0[0])) return;   // flagged ray keeps its NaNs
+  const DevScene &S = *G.scene;
+  const uint32_t px = pix % G.width, py = pix / G.width;
+  const CamD ca = incl_camera_ray_d(S.mass, Tn{S.spin, 1.0}, dconst(G.incl), G.r_cam, S.half_h, S.half_w, px, py,
+                                    G.width, G.height);
+  const CamD ci = incl_camera_ray_d(S.mass, dconst(S.spin), Tn{G.incl, 1.0}, G.r_cam, S.half_h, S.half_w, px, py,
+                                    G.width, G.height);
+  const d4 vb{rec[R_VB0], rec[R_VB1], rec[R_VB2], rec[R_VB3]};
+  double *o = G.dI + 4 * (uint64_t)pix;
+  o[0] = rec[R_GA] + vb.x * ca.k[0].d + vb.y * ca.k[1].d + vb.z * ca.k[2].d + vb.w * ca.k[3].d;
+  o[1] = rec[R_GI] + rec[R_PBTH] + vb.x * ci.k[0].d + vb.y * ci.k[1].d + vb.z * ci.k[2].d + vb.w * ci.k[3].d;
+  o[2] = rec[R_GQ];
+  o[3] = rec[R_GRIN];
+}
+
 // Residuals, zeroed gradients and the list of disk rays with their
-// checkpoint count.  One atomic per wavefront for the list and the total.
+// accepted counts (sort keys), checkpoint and last-segment totals.  One
+// atomic per wavefront for the list and the totals.
 __global__ void select_kernel(GradSelect P) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63;
   bool disk = false;
-  uint32_t nchk = 0;
+  uint32_t nchk = 0, last = 0, acc = 0;
   if (i < P.n) {
     P.resid[i] = P.intensity[i] - P.target[i];
     double *o = P.dI + 4 * i;
     o[0] = 0.0; o[1] = 0.0; o[2] = 0.0; o[3] = 0.0;
     disk = P.cls[i] == GG_CLASS_DISK;
-    if (disk) nchk = (P.accepted[i] + P.K - 1) / P.K;
+    if (disk) {
+      acc = P.accepted[i];
+      nchk = (acc + P.K - 1) / P.K;
+      last = nchk ? acc - (nchk - 1) * P.K : 0;
+    }
   }
   const unsigned long long m = __ballot(disk);
-  unsigned long long tot = wave_sum(nchk);
+  const unsigned long long tot = wave_sum(nchk), tl = wave_sum(last);
   if (m) {
     const int leader = __ffsll((long long)m) - 1;
     unsigned long long base = 0;
     if ((int)lane == leader) {
       base = atomicAdd(&P.cnt->n_list, (unsigned long long)__popcll(m));
       atomicAdd(&P.cnt->chk_total, tot);
+      atomicAdd(&P.cnt->tail_total, tl);
     }
     base = __shfl(base, leader, 64);
-    if (disk) P.list[base + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)i;
+    if (disk) {
+      const uint64_t at = base + __popcll(m & ((1ull << lane) - 1ull));
+      P.list[at] = (uint32_t)i;
+      P.keys[at] = acc;
+    }
   }
 }
 
@@ -387,12 +507,32 @@ __global__ __launch_bounds__(256) void vjp_kernel(const DevScene *S, uint64_t n,
 
 }  // namespace
 
-// LDS of the adjoint kernel: k_1..k_6 and their cotangents, 4 doubles each.
+// LDS of the reverse kernel: k_1..k_6 and their cotangents, 4 doubles each;
+// of the forward kernel: the trace kernel's k_1..k_5.
 size_t adjoint_lds_bytes(uint32_t block) { return (size_t)block * 12 * 4 * sizeof(double); }
+size_t adjoint_fwd_lds_bytes(uint32_t block) { return (size_t)block * 5 * 4 * sizeof(double); }
 
-hipError_t launch_adjoint(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s) {
-  hipLaunchKernelGGL(adjoint_kernel, dim3(grid), dim3(block), adjoint_lds_bytes(block), s, g);
+hipError_t launch_adjoint_fwd(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s) {
+  hipLaunchKernelGGL(adjoint_fwd_kernel, dim3(grid), dim3(block), adjoint_fwd_lds_bytes(block), s, g);
   return hipGetLastError();
+}
+
+hipError_t launch_adjoint_rev(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s) {
+  hipLaunchKernelGGL(adjoint_rev_kernel, dim3(grid), dim3(block), adjoint_lds_bytes(block), s, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_cam(const GradLaunch &g, hipStream_t s) {
+  if (g.n_list == 0) return hipSuccess;
+  hipLaunchKernelGGL(adjoint_cam_kernel, dim3((uint32_t)((g.n_list + 255) / 256)), dim3(256), 0, s, g);
+  return hipGetLastError();
+}
+
+// Disk rays by accepted count, longest first (radix sort of (count, pixel)).
+hipError_t sort_rays_desc(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
+                          uint64_t n, int key_bits, void *temp, size_t &temp_bytes, hipStream_t s) {
+  return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out,
+                                                      (int)n, 0, key_bits, s);
 }
 
 hipError_t launch_grad_select(const GradSelect &p, hipStream_t s) {

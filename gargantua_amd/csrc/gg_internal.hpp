@@ -40,16 +40,19 @@ struct TraceLaunch {
 
 // Tallies of gg_grad (zeroed by launch_grad_reset).
 struct GradCounters {
-  unsigned long long queue;        // next disk ray to hand out
+  unsigned long long queue;        // next disk ray for the forward kernel
+  unsigned long long queue2;       // next disk ray for the reverse kernel
   unsigned long long chk_cursor;   // checkpoints claimed in the pool
+  unsigned long long tail_cursor;  // last-segment step records claimed
   unsigned long long attempts;     // attempts of re-traces and recomputations
   unsigned long long steps;        // accepted steps back-propagated
-  unsigned long long done;         // rays whose gradient was written
+  unsigned long long done;         // rays whose reverse sweep finished
   unsigned long long bad;          // rays whose re-trace left the rendered path
   unsigned long long rhs;          // RHS evaluations inside attempts
   unsigned long long n_list;       // disk rays found by select
   unsigned long long chk_total;    // checkpoints they need
-  unsigned long long pad[7];
+  unsigned long long tail_total;   // step records of their last segments
+  unsigned long long pad[4];
 };
 
 struct GradSelect {
@@ -59,18 +62,21 @@ struct GradSelect {
   const uint32_t *accepted;
   uint32_t K;
   double *resid, *dI;
-  uint32_t *list;
+  uint32_t *list, *keys;           // disk pixels and their accepted counts (unsorted)
   GradCounters *cnt;
 };
 
 struct GradLaunch {
   const DevScene *scene;
-  const uint32_t *list;
+  const uint32_t *list;            // disk pixels, longest first
+  const uint32_t *keys;            // their accepted counts
   uint64_t n_list;
-  const uint32_t *accepted;        // per pixel, from the render
+  double *rec;                     // per-ray records
   double *chk;                     // checkpoint pool, 9 doubles each
   unsigned long long chk_cap;
-  double *seg;                     // per-thread segment buffers, K x 9 doubles
+  double *tail;                    // last-segment step records, 9 doubles each
+  unsigned long long tail_cap;
+  double *seg;                     // per-thread segment buffers of the reverse kernel, K x 9 doubles
   uint32_t K;
   uint32_t width, height;
   double incl, r_cam;
@@ -78,8 +84,14 @@ struct GradLaunch {
   GradCounters *cnt;
 };
 
+constexpr uint32_t GRAD_REC = 16;  // doubles per ray record
 size_t adjoint_lds_bytes(uint32_t block);
-hipError_t launch_adjoint(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s);
+size_t adjoint_fwd_lds_bytes(uint32_t block);
+hipError_t launch_adjoint_fwd(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s);
+hipError_t launch_adjoint_rev(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s);
+hipError_t launch_adjoint_cam(const GradLaunch &g, hipStream_t s);
+hipError_t sort_rays_desc(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
+                          uint64_t n, int key_bits, void *temp, size_t &temp_bytes, hipStream_t s);
 hipError_t launch_grad_select(const GradSelect &p, hipStream_t s);
 hipError_t launch_grad_reduce(uint64_t n, const double *resid, const double *dI, double *out5, hipStream_t s);
 hipError_t launch_grad_reset(GradCounters *c, hipStream_t s);

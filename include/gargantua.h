@@ -279,7 +279,9 @@ typedef struct gg_grad_stats {
                                segment recomputation)                         */
   uint64_t checkpoints;     /* checkpoints stored                             */
   uint64_t checkpoint_bytes;
-  uint64_t segment_bytes;   /* per-thread segment buffers                     */
+  uint64_t segment_bytes;   /* per-thread segment buffers of the reverse pass */
+  uint64_t last_segment_bytes; /* step records of each ray's last segment,
+                               stored by the re-trace                         */
   uint32_t grid_blocks;
   uint32_t block_threads;
 } gg_grad_stats;
