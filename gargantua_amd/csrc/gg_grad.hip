@@ -72,13 +72,14 @@ __device__ inline void step_adjoint(const DevScene &S, const KStore &K, const do
   if (th1 < 0.0) { th1 = -th1; sg = -sg; }
   if (th1 > PI) sg = -sg;
   pb_th *= sg;
-  // the six stage values k_1..k_6 (slots 0..5)
-  K.put(0, rhs64<1>(S, p[1], p[2], v));
+  // the six stage values k_1..k_6 (slots 0..5); they only feed derivatives,
+  // so the reciprocal-form RHS (a few ulp from the exact one) serves
+  K.put(0, rhs64f<1>(S, p[1], p[2], v));
 #pragma unroll 1
   for (int s = 2; s <= 6; ++s) {
     double ha = h * kA[s];
     d4 vs = stage_vel(K, s, v, h);
-    K.put(s - 1, rhs64<1>(S, ha * v.y + p[1], ha * v.z + p[2], vs));
+    K.put(s - 1, rhs64f<1>(S, ha * v.y + p[1], ha * v.z + p[2], vs));
   }
   // new_vel = v + h sum_i c_i k_i: kbar_i = h c_i vb; pos' = p + h v
   const d4 vbp = vb;
