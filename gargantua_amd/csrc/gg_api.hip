@@ -782,8 +782,8 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
   if (grid_r < 1) grid_r = 1;
   const uint64_t n_threads = grid_r * 64;
   const uint64_t chk_bytes = 72 * (chk_total ? chk_total : 1);
-  const uint64_t tail_bytes = 72 * (tail_total ? tail_total : 1);
-  const uint64_t seg_bytes = 72 * (uint64_t)K * n_threads;
+  const uint64_t tail_bytes = 8 * GRAD_STEP_REC * (tail_total ? tail_total : 1);
+  const uint64_t seg_bytes = 8 * GRAD_STEP_REC * (uint64_t)K * n_threads;
   if ((rc = B[10].ensure(chk_bytes)) || (rc = B[11].ensure(seg_bytes)) || (rc = B[15].ensure(8 * GRAD_REC * (n_list + 1))) ||
       (rc = B[16].ensure(tail_bytes)))
     return rc;
@@ -849,7 +849,7 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
     stats->checkpoints = chk_total;
     stats->checkpoint_bytes = 72 * chk_total;
     stats->segment_bytes = seg_bytes;
-    stats->last_segment_bytes = 72 * tail_total;
+    stats->last_segment_bytes = 8 * GRAD_STEP_REC * tail_total;
     stats->grid_blocks = (uint32_t)grid_r;
     stats->block_threads = 64;
   }

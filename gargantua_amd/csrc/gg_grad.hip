@@ -36,72 +36,49 @@ namespace {
 
 enum { PH_IDLE = 0, PH_FWD = 1, PH_REV = 2, PH_RECOMP = 3 };
 
-constexpr int KBAR = 6;   // LDS slots 6..11 hold the stage cotangents
-
-// v + h sum_j b_sj k_j of stage s (s = 2..6) from the k_j in LDS slots
-// 0..s-2, in the forward kernel's order.
-__device__ inline d4 stage_vel(const KStore &K, int s, d4 v, double h) {
-#pragma clang fp contract(off)
-  d4 k1 = K.get(0);
-  if (s == 2) {
-    double hb = h * B21;
-    return d4{hb * k1.x + v.x, hb * k1.y + v.y, hb * k1.z + v.z, hb * k1.w + v.w};
-  }
-  double b = kB[s][1];
-  d4 sum{b * k1.x, b * k1.y, b * k1.z, b * k1.w};
-#pragma unroll 1
-  for (int j = 2; j < s; ++j) {
-    double bj = kB[s][j];
-    d4 kj = K.get(j - 1);
-    sum.x = bj * kj.x + sum.x; sum.y = bj * kj.y + sum.y; sum.z = bj * kj.z + sum.z; sum.w = bj * kj.w + sum.w;
-  }
-  return d4{h * sum.x + v.x, h * sum.y + v.y, h * sum.z + v.z, h * sum.w + v.w};
-}
+static_assert(STEP_REC == GRAD_STEP_REC, "step record layout");
 
 // Adjoint of one accepted Kerr step (p, v) -> (p + h v normalised,
-// v + h sum_i c_i k_i).  On entry (pb_r, pb_th, vb) is the cotangent of the
+// v + h sum_i c_i k_i), k_i = a(p + h a_i v, v_i), from its step record
+// (p, v, h, v_2..v_6).  On entry (pb_r, pb_th, vb) is the cotangent of the
 // step's end state (t and phi carry none: nothing depends on them); on exit
-// that of its start.  ga accumulates the spin cotangent.
-__device__ inline void step_adjoint(const DevScene &S, const KStore &K, const double p[4], d4 v, double h,
-                                    double &pb_r, double &pb_th, d4 &vb, double &ga) {
+// that of its start.  ga accumulates the spin cotangent.  LDS slots 0..5
+// hold the stage cotangents kbar_1..kbar_6.
+__device__ inline void step_adjoint(const KStore &K, const double *rec, double m, double a, double &pb_r,
+                                    double &pb_th, d4 &vb, double &ga) {
 #pragma clang fp contract(off)
+  const double r0 = rec[1], th0 = rec[2];
+  const d4 v{rec[4], rec[5], rec[6], rec[7]};
+  const double h = rec[8];
   // normalize_spherical_coords (geodesic.rs:337-352): theta -> -theta or
   // 2 pi - theta flips the sign of its cotangent
-  double th1 = h * v.z + p[2];
+  double th1 = h * v.z + th0;
   double sg = 1.0;
   if (th1 < 0.0) { th1 = -th1; sg = -sg; }
   if (th1 > PI) sg = -sg;
   pb_th *= sg;
-  // the six stage values k_1..k_6 (slots 0..5); they only feed derivatives,
-  // so the reciprocal-form RHS (a few ulp from the exact one) serves
-  K.put(0, rhs64f<1>(S, p[1], p[2], v));
-#pragma unroll 1
-  for (int s = 2; s <= 6; ++s) {
-    double ha = h * kA[s];
-    d4 vs = stage_vel(K, s, v, h);
-    K.put(s - 1, rhs64f<1>(S, ha * v.y + p[1], ha * v.z + p[2], vs));
-  }
   // new_vel = v + h sum_i c_i k_i: kbar_i = h c_i vb; pos' = p + h v
   const d4 vbp = vb;
 #pragma unroll 1
   for (int i = 1; i <= 6; ++i) {
     double hc = h * kC[i];
-    K.put(KBAR + i - 1, d4{hc * vbp.x, hc * vbp.y, hc * vbp.z, hc * vbp.w});
+    K.put(i - 1, d4{hc * vbp.x, hc * vbp.y, hc * vbp.z, hc * vbp.w});
   }
   double pr = pb_r, pth = pb_th;
   d4 vbar{vbp.x, vbp.y + h * pb_r, vbp.z + h * pb_th, vbp.w};
-  const double m = S.mass, a = S.spin;
 #pragma unroll 1
   for (int i = 6; i >= 1; --i) {
-    const d4 w = K.get(KBAR + i - 1);
-    double qr = p[1], qth = p[2];
+    const d4 w = K.get(i - 1);
+    double qr = r0, qth = th0;
     d4 vs = v;
     double ha = 0.0;
     if (i >= 2) {
+      // stage position p + h a_i v, stage velocity v + h sum_j b_ij k_j
       ha = h * kA[i];
-      qr = ha * v.y + p[1];
-      qth = ha * v.z + p[2];
-      vs = stage_vel(K, i, v, h);
+      qr = ha * v.y + r0;
+      qth = ha * v.z + th0;
+      const double *o = rec + 9 + 4 * (i - 2);
+      vs = d4{o[0], o[1], o[2], o[3]};
     }
     const RhsBar jb = kerr_rhs_vjp(m, a, qr, qth, vs, w);
     ga += jb.a;
@@ -109,15 +86,14 @@ __device__ inline void step_adjoint(const DevScene &S, const KStore &K, const do
     pth += jb.th;
     vbar.x += jb.v.x; vbar.y += jb.v.y; vbar.z += jb.v.z; vbar.w += jb.v.w;
     if (i >= 2) {
-      // stage position p + h a_i v, stage velocity v + h sum_j b_ij k_j
       vbar.y += ha * jb.r;
       vbar.z += ha * jb.th;
 #pragma unroll 1
       for (int j = 1; j < i; ++j) {
         double hb = h * kB[i][j];
-        d4 kb = K.get(KBAR + j - 1);
+        d4 kb = K.get(j - 1);
         kb.x += hb * jb.v.x; kb.y += hb * jb.v.y; kb.z += hb * jb.v.z; kb.w += hb * jb.v.w;
-        K.put(KBAR + j - 1, kb);
+        K.put(j - 1, kb);
       }
     }
   }
@@ -221,11 +197,11 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
     if (exhausted && __all(!active)) break;
     if (!active) continue;
 
-    const double h_used = L.h;
     bool acc;
     double p0[4];
     d4 v0;
-    const int res = attempt<1, 0>(S, L, K, acc, p0, v0, false);
+    double *recp = L.accepted >= last0 ? G.tail + STEP_REC * (tail_base + (L.accepted - last0)) : nullptr;
+    const int res = attempt<1, 0, true>(S, L, K, acc, p0, v0, false, recp);
     ++n_att;
     if (res != RES_CONTINUE || ++att > S.max_steps) {   // the render reached the crossing; a re-trace must too
       mark_bad(G, q, pix, n_bad);
@@ -233,8 +209,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
       continue;
     }
     if (!acc) continue;
-    const uint32_t g = L.accepted++;
-    if (g >= last0) put_state(G.tail + 9 * (tail_base + (g - last0)), p0, v0, h_used);
+    L.accepted++;
     if (L.accepted < acc_target) {
       if (L.accepted % KK == 0) put_state(G.chk + 9 * (chk_base + L.accepted / KK), L.pos, L.vel, L.h);
       continue;
@@ -289,7 +264,7 @@ __global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const DevScene &S = *G.scene;
   const uint32_t KK = G.K;
-  double *const seg = G.seg + gtid * (uint64_t)KK * 9;
+  double *const seg = G.seg + gtid * (uint64_t)KK * STEP_REC;
 
   Lane L;
   L.n_rhs = 0; L.n_rhs32 = 0; L.n_full = 0;
@@ -317,7 +292,7 @@ __global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
         c = nchk - 1;
         j = acc_target - c * KK - 1;
         chk_base = as_u64(rec[R_CHK]);
-        src = G.tail + 9 * as_u64(rec[R_TAIL]);
+        src = G.tail + STEP_REC * as_u64(rec[R_TAIL]);
         pb_r = rec[R_PBR]; pb_th = rec[R_PBTH];
         vb = d4{rec[R_VB0], rec[R_VB1], rec[R_VB2], rec[R_VB3]};
         ga = rec[R_GA];
@@ -329,18 +304,17 @@ __global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
 
     // ---- RECOMP: one attempt of the segment before ----
     if (phase == RECOMP) {
-      const double h_used = L.h;
       bool acc;
       double p0[4];
       d4 v0;
-      const int res = attempt<1, 0>(S, L, K, acc, p0, v0, false);
+      double *recp = seg + STEP_REC * (uint64_t)(L.accepted - c * KK);
+      const int res = attempt<1, 0, true>(S, L, K, acc, p0, v0, false, recp);
       ++n_att;
       if (res != RES_CONTINUE || ++att > S.max_steps) {
         mark_bad(G, q, pix, n_bad);
         phase = IDLE;
       } else if (acc) {
-        const uint32_t g = L.accepted++;
-        put_state(seg + 9 * (uint64_t)(g - c * KK), p0, v0, h_used);
+        L.accepted++;
         if (L.accepted == (c + 1) * KK) {
           // the recomputed segment must land bit for bit on the next checkpoint
           double qp[4], hq;
@@ -361,10 +335,7 @@ __global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
 
     // ---- REV: one accepted step backwards ----
     if (phase == REV) {
-      double p[4], h;
-      d4 v;
-      get_state(src + 9 * (uint64_t)j, p, v, h);
-      step_adjoint(S, K, p, v, h, pb_r, pb_th, vb, ga);
+      step_adjoint(K, src + STEP_REC * (uint64_t)j, S.mass, S.spin, pb_r, pb_th, vb, ga);
       ++n_adj;
       if (first) {
         const double *rec = G.rec + REC * q;
@@ -508,9 +479,10 @@ __global__ __launch_bounds__(256) void vjp_kernel(const DevScene *S, uint64_t n,
 
 }  // namespace
 
-// LDS of the reverse kernel: k_1..k_6 and their cotangents, 4 doubles each;
-// of the forward kernel: the trace kernel's k_1..k_5.
-size_t adjoint_lds_bytes(uint32_t block) { return (size_t)block * 12 * 4 * sizeof(double); }
+// LDS of the reverse kernel: six slots of 4 doubles (the recomputation's
+// k_1..k_5, the reverse step's cotangents kbar_1..kbar_6); of the forward
+// kernel: the trace kernel's k_1..k_5.
+size_t adjoint_lds_bytes(uint32_t block) { return (size_t)block * 6 * 4 * sizeof(double); }
 size_t adjoint_fwd_lds_bytes(uint32_t block) { return (size_t)block * 5 * 4 * sizeof(double); }
 
 hipError_t launch_adjoint_fwd(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s) {

@@ -74,9 +74,9 @@ struct GradLaunch {
   double *rec;                     // per-ray records
   double *chk;                     // checkpoint pool, 9 doubles each
   unsigned long long chk_cap;
-  double *tail;                    // last-segment step records, 9 doubles each
+  double *tail;                    // last-segment step records, GRAD_STEP_REC doubles each
   unsigned long long tail_cap;
-  double *seg;                     // per-thread segment buffers of the reverse kernel, K x 9 doubles
+  double *seg;                     // per-thread segment buffers of the reverse kernel, K step records
   uint32_t K;
   uint32_t width, height;
   double incl, r_cam;
@@ -84,7 +84,8 @@ struct GradLaunch {
   GradCounters *cnt;
 };
 
-constexpr uint32_t GRAD_REC = 16;  // doubles per ray record
+constexpr uint32_t GRAD_REC = 16;       // doubles per ray record
+constexpr uint32_t GRAD_STEP_REC = 29;  // doubles per step record: pos, vel, h, v_2..v_6
 size_t adjoint_lds_bytes(uint32_t block);
 size_t adjoint_fwd_lds_bytes(uint32_t block);
 hipError_t launch_adjoint_fwd(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s);

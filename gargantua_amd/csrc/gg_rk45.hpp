@@ -141,12 +141,17 @@ struct StageOut {
   bool invalid;
 };
 
+// Step record of an accepted step for the adjoint: start position (4),
+// velocity (4), h, then the stage velocities v_2..v_6 (4 each).
+constexpr int STEP_REC = 29;
+
 // The stages of one attempt (geodesic.rs:244-308): k1 when the lane has no
 // valid one (s = 1), then k2..k7, the 5th-order velocity and the error.  One
 // copy of the loop per arithmetic (ARITH_EXACT, ARITH_F32, ARITH_FAST).
-template <int METRIC, int ARITH>
+// REC: write the stage velocities into rec (when non-null) as they form.
+template <int METRIC, int ARITH, bool REC = false>
 __device__ inline StageOut stages(const DevScene &S, const KStore &K, const double p0[4], const d4 vel, double h,
-                                  bool need_k1) {
+                                  bool need_k1, double *rec = nullptr) {
 #pragma clang fp contract(off)
   StageOut o;
   d4 k1{0, 0, 0, 0}, s5{0, 0, 0, 0}, s4{0, 0, 0, 0};
@@ -189,6 +194,10 @@ __device__ inline StageOut stages(const DevScene &S, const KStore &K, const doub
       }
       qr = q1;
       qth = q2;
+      if (REC && rec) {
+        double *o = rec + 9 + 4 * (s - 2);
+        o[0] = vs.x; o[1] = vs.y; o[2] = vs.z; o[3] = vs.w;
+      }
     } else {
       qr = madd<ARITH>(h, vel.y, p0[1]);                            // k7 at (pos6, new_vel) (:298)
       qth = madd<ARITH>(h, vel.z, p0[2]);
@@ -271,9 +280,11 @@ __device__ inline bool fp32_eligible(const DevScene &S, const Lane &L) {
 // steps do run in FP32 depends on the lane grouping, so mixed final states
 // reproduce across launches to FP32 rounding (~1e-12 relative), not bitwise.
 // Returns RES_*; `acc` reports acceptance; p0/v0 keep the pre-step state.
-template <int METRIC, int PREC>
+// REC (adjoint kernels, PREC 0): rec, when non-null, receives the step
+// record of the attempt if it is accepted.
+template <int METRIC, int PREC, bool REC = false>
 __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool &acc, double p0[4], d4 &v0,
-                              bool use32) {
+                              bool use32, double *rec = nullptr) {
 #pragma clang fp contract(off)
   constexpr int BASE = (PREC == 2) ? ARITH_FAST : ARITH_EXACT;
   acc = false;
@@ -301,7 +312,7 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
     // An FP64 attempt needs an FP64 k1: after an FP32 attempt k7 (the FSAL
     // k1) carries FP32 rounding, so recompute it.
     const bool need_k1 = !L.k1_ok || L.k1_f32;
-    o = stages<METRIC, BASE>(S, K, p0, vel, h, need_k1);
+    o = stages<METRIC, BASE, REC>(S, K, p0, vel, h, need_k1, rec);
     L.n_rhs += o.nr;
     if (need_k1) L.k1_f32 = false;
   }
@@ -313,6 +324,11 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
   if (err < thresh || h <= S.min_step) {           // accept (:310-321)
     L.pos[0] = madd<BASE>(h, vel.x, p0[0]); L.pos[1] = madd<BASE>(h, vel.y, p0[1]);
     L.pos[2] = madd<BASE>(h, vel.z, p0[2]); L.pos[3] = madd<BASE>(h, vel.w, p0[3]);
+    if (REC && rec) {
+      rec[0] = p0[0]; rec[1] = p0[1]; rec[2] = p0[2]; rec[3] = p0[3];
+      rec[4] = vel.x; rec[5] = vel.y; rec[6] = vel.z; rec[7] = vel.w;
+      rec[8] = h;
+    }
     L.vel = o.nv;
     L.lambda = L.lambda + h;
     // normalize_spherical_coords (:337-352)
