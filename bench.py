@@ -80,8 +80,8 @@ def flops(stats, metric_kind):
     return f32, f64
 
 
-def cpu_baseline(scene, stride=2):
-    """FP64 oracle on every `stride`-th row and column of the same rays."""
+def cpu_baseline(scene, stride=1):
+    """FP64 oracle on every `stride`-th row and column of the same rays (stride 1: all of them)."""
     from oracle import Oracle
     w = scene.width
     idx = np.array([py * w + px for py in range(0, scene.height, stride) for px in range(0, w, stride)],
@@ -101,10 +101,26 @@ def cpu_baseline(scene, stride=2):
     one.render_rays(scene, sub, state=False)
     dt1 = time.perf_counter() - t1
     return {"value": len(idx) / dt, "unit": "geodesics/s", "cores": cores, "kind": "port",
-            "sample": f"every {stride}th row and column of the 1024x1024 jittered Kerr a=0.9 scene "
+            "sample": f"every {stride}. row and column of the {scene.width}x{scene.height} jittered Kerr a=0.9 scene "
                       f"({len(idx)} rays, {dt:.1f} s on {cores} threads); 1 thread: {len(sub) / dt1:.1f} geodesics/s "
                       f"on {len(sub)} rays",
             "value_1core": len(sub) / dt1}
+
+
+def com_drift(rend, scene, precision):
+    """Drift of E, L_z, Carter Q from the camera state to the final state (untimed render with states)."""
+    s0 = rend.camera_rays(scene)
+    o = rend.render(scene, precision, state=True)
+    s1 = np.ascontiguousarray(o["state"].T)
+    c0 = rend.constants_of_motion(scene.metric, s0)
+    c1 = rend.constants_of_motion(scene.metric, s1)
+    d = np.abs(c1 - c0)
+    rel = {"E": d[:, 0] / np.abs(c0[:, 0]), "L_z": d[:, 1] / (np.abs(c0[:, 1]) + 1e-3 * np.abs(c0[:, 0])),
+           "Q": d[:, 2] / (np.abs(c0[:, 2]) + c0[:, 1] ** 2)}
+    out = {}
+    for k, v in rel.items():
+        out[k] = {"median": float(np.median(v)), "p99": float(np.quantile(v, 0.99))}
+    return out
 
 
 def bench_batch64(args):
@@ -349,6 +365,7 @@ def main():
                  "attempts_per_ray": st_m["attempts"] / n},
         "cpu_baseline": None,
     }
+    line["constants_of_motion_drift"] = {p: com_drift(rend, scene, p) for p in ("fp64", "mixed")}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(scene)
     rend.close()

@@ -324,6 +324,24 @@ class Renderer:
                "gg_geodesic_acceleration")
         return acc
 
+    def camera_rays(self, scene: Scene) -> np.ndarray:
+        """GeodesicRay::from_camera for every sub-pixel ray: [n, 9] (t r th ph vt vr vth vph lambda)."""
+        out = np.zeros((scene.n_rays, 9))
+        sc = scene.to_c()
+        _check(self.lib, self.lib.gg_camera_rays(self.ctx, C.byref(sc), out.ctypes.data), "gg_camera_rays")
+        return out
+
+    def constants_of_motion(self, metric, states: np.ndarray) -> np.ndarray:
+        """[n, 4]: E, L_z, Carter Q, mu^2 of states [n, 9] (or [n, 8])."""
+        states = np.ascontiguousarray(states, dtype=np.float64)
+        if states.shape[1] == 8:
+            states = np.ascontiguousarray(np.concatenate([states, np.zeros((len(states), 1))], 1))
+        out = np.zeros((len(states), 4))
+        m = metric.to_c()
+        _check(self.lib, self.lib.gg_constants_of_motion(self.ctx, C.byref(m), len(states), states.ctypes.data,
+                                                         out.ctypes.data), "gg_constants_of_motion")
+        return out
+
     def geodesic_acceleration_vjp(self, metric, pos: np.ndarray, vel: np.ndarray, cot: np.ndarray) -> dict:
         """w^T J of the Kerr RHS: {"pos_bar": [n,4], "vel_bar": [n,4], "spin_bar": [n]}."""
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 4)

@@ -109,6 +109,8 @@ EXPORTS = [
     ("gg_geodesic_acceleration", C.c_int, [_P, C.POINTER(gg_metric), C.c_int, C.c_uint64, _P, _P, _P]),
     ("gg_autotune", C.c_int, [_P, C.POINTER(gg_scene), C.c_uint32, C.c_int, C.POINTER(gg_launch), C.c_uint32,
                               C.POINTER(gg_launch), C.POINTER(C.c_double)]),
+    ("gg_camera_rays", C.c_int, [_P, C.POINTER(gg_scene), _P]),
+    ("gg_constants_of_motion", C.c_int, [_P, C.POINTER(gg_metric), C.c_uint64, _P, _P]),
     ("gg_grad_scene_to_scene", C.c_int, [C.POINTER(gg_grad_scene), C.POINTER(gg_scene)]),
     ("gg_grad", C.c_int, [_P, C.POINTER(gg_grad_scene), _P, C.c_uint32, C.POINTER(C.c_double), _P, _P, _P,
                           C.POINTER(gg_grad_stats)]),

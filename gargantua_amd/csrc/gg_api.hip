@@ -88,7 +88,7 @@ struct gg_context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf scenes, outs, counters, states_in;
   DevBuf stage[9];       // device staging for host-memory outputs
-  DevBuf rhs_io;
+  DevBuf rhs_io, misc_io;
   std::vector<DevBuf> batch_stage;
   // gg_grad: cls, intensity, accepted, target, resid, dI, list, counters,
   // results, scene, checkpoint pool, segment buffers, keys, sorted list,
@@ -390,6 +390,7 @@ int gg_context_destroy(gg_context *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   ctx->scenes.release(); ctx->outs.release(); ctx->counters.release(); ctx->states_in.release();
   ctx->rhs_io.release();
+  ctx->misc_io.release();
   for (auto &b : ctx->grad) b.release();
   for (auto &b : ctx->stage) b.release();
   for (auto &b : ctx->batch_stage) b.release();
@@ -675,6 +676,60 @@ int gg_geodesic_acceleration(gg_context *ctx, const gg_metric *metric, int preci
   HIP_TRY(hipMemcpyAsync(dv, vel, bytes, hipMemcpyDefault, s));
   HIP_TRY(launch_rhs(metric->kind, precision, dS, n, dp, dv, da, s));
   HIP_TRY(hipMemcpyAsync(acc, da, bytes, hipMemcpyDefault, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return GG_OK;
+}
+
+int gg_camera_rays(gg_context *ctx, const gg_scene *scene, double *states) {
+  if (!ctx || !states) return fail(GG_ERR_INVALID_ARG, "NULL argument");
+  int rc = validate_scene(scene, true);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  DevScene D;
+  build_dev_scene(*scene, D);
+  const uint64_t n = (uint64_t)D.sub_w * D.sub_h;
+  const size_t head = 256 * ((sizeof(DevScene) + 255) / 256);
+  const bool dev = is_device_ptr(states);
+  if ((rc = ctx->misc_io.ensure(head + (dev ? 0 : 72 * n)))) return rc;
+  char *base = (char *)ctx->misc_io.p;
+  double *out = dev ? states : (double *)(base + head);
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(base, &D, sizeof(D), hipMemcpyHostToDevice, s));
+  HIP_TRY(launch_camera_rays((const DevScene *)base, n, out, s));
+  if (!dev) HIP_TRY(hipMemcpyAsync(states, out, 72 * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return GG_OK;
+}
+
+int gg_constants_of_motion(gg_context *ctx, const gg_metric *metric, uint64_t n, const double *states, double *out) {
+  if (!ctx || !metric) return fail(GG_ERR_INVALID_ARG, "NULL argument");
+  if (n == 0) return GG_OK;
+  if (!states || !out) return fail(GG_ERR_INVALID_ARG, "NULL states/out");
+  int rc = validate_metric(*metric);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  gg_scene sc;
+  std::memset(&sc, 0, sizeof(sc));
+  sc.metric = *metric;
+  sc.camera.position[1] = 10.0 * metric->mass;
+  sc.camera.position[2] = M_PI / 2.0;
+  sc.camera.look_at[0] = -1.0;
+  sc.camera.up[2] = 1.0;
+  sc.camera.fov_y_radians = 1.0;
+  sc.camera.aspect = 1.0;
+  DevScene D;
+  build_dev_scene(sc, D);
+  const size_t head = 256 * ((sizeof(DevScene) + 255) / 256);
+  const bool din = is_device_ptr(states), dout = is_device_ptr(out);
+  if ((rc = ctx->misc_io.ensure(head + (din ? 0 : 72 * n) + (dout ? 0 : 32 * n)))) return rc;
+  char *base = (char *)ctx->misc_io.p;
+  const double *sin_ = din ? states : (const double *)(base + head);
+  double *o = dout ? out : (double *)(base + head + (din ? 0 : 72 * n));
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(base, &D, sizeof(D), hipMemcpyHostToDevice, s));
+  if (!din) HIP_TRY(hipMemcpyAsync((void *)sin_, states, 72 * n, hipMemcpyHostToDevice, s));
+  HIP_TRY(launch_constants(metric->kind, (const DevScene *)base, n, sin_, o, s));
+  if (!dout) HIP_TRY(hipMemcpyAsync(out, o, 32 * n, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return GG_OK;
 }

@@ -105,6 +105,9 @@ hipError_t launch_trace(int metric_kind, int disk_kind, int precision, const Tra
 hipError_t launch_reset(Counters *c, hipStream_t stream);
 hipError_t launch_rhs(int metric_kind, int precision, const DevScene *scene, uint64_t n, const double *pos,
                       const double *vel, double *acc, hipStream_t stream);
+hipError_t launch_constants(int metric_kind, const DevScene *scene, uint64_t n, const double *states, double *out,
+                            hipStream_t stream);
+hipError_t launch_camera_rays(const DevScene *scene, uint64_t n, double *states, hipStream_t stream);
 // RGB image (renderer.rs:44-80 shading + sRGB) from per-ray outputs.
 hipError_t launch_shade(const DevScene *scene, const DevOut *out, uint32_t width, uint32_t height, uint32_t spa,
                         uint8_t *rgb, hipStream_t stream);

@@ -484,6 +484,27 @@ __device__ inline double disk_emission(const DevScene &S, double rho, double r, 
   return rho * rho * temp_proxy;
 }
 
+// Constants of motion of a geodesic at (pos, vel): E = -p_t, L_z = p_phi,
+// Carter's Q = p_theta^2 + cos^2(theta) (a^2 (mu^2 - E^2) + L_z^2 / sin^2(theta))
+// and the mass shell mu^2 = -g(v, v) (0 for a null ray; the camera rays of
+// ray.rs:81-83 drop g_tphi, so Kerr camera rays carry mu^2 = -2 g_tphi k^t k^phi).
+// Exact for the geodesic flow; the integrator's forward-Euler positions
+// make them drift at O(h), which is what they diagnose.
+template <int METRIC>
+__device__ inline void constants_of_motion(const DevScene &S, const double pos[4], d4 v, double out[4]) {
+  const Gmn g = metric64<METRIC>(S, pos[1], pos[2]);
+  const double pt = g.tt * v.x + g.tph * v.w;
+  const double pph = g.tph * v.x + g.phph * v.w;
+  const double pth = g.thth * v.z;
+  const double mu2 = -(pt * v.x + g.rr * v.y * v.y + pth * v.z + pph * v.w);
+  const double st = sin(pos[2]), ct = cos(pos[2]);
+  const double a = METRIC == 1 ? S.spin : 0.0;
+  out[0] = -pt;
+  out[1] = pph;
+  out[2] = pth * pth + ct * ct * (a * a * (mu2 - pt * pt) + pph * pph / (st * st));
+  out[3] = mu2;
+}
+
 // SplitMix64 (Vigna): n-th output (n >= 1) for `seed`.
 __device__ inline uint64_t splitmix64(uint64_t seed, uint64_t n) {
   uint64_t z = seed + n * 0x9E3779B97F4A7C15ull;

@@ -304,6 +304,30 @@ __global__ void rhs_kernel(const DevScene *S, uint64_t n, const double *pos, con
   acc[4 * i] = a.x; acc[4 * i + 1] = a.y; acc[4 * i + 2] = a.z; acc[4 * i + 3] = a.w;
 }
 
+template <int METRIC>
+__global__ void com_kernel(const DevScene *S, uint64_t n, const double *states, double *out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *st = states + 9 * i;
+  const double pos[4] = {st[0], st[1], st[2], st[3]};
+  double o[4];
+  constants_of_motion<METRIC>(*S, pos, d4{st[4], st[5], st[6], st[7]}, o);
+  for (int k = 0; k < 4; ++k) out[4 * i + k] = o[k];
+}
+
+// GeodesicRay::from_camera for every sub-pixel ray, row-major.
+__global__ void camera_rays_kernel(const DevScene *S, double *states) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n = (uint64_t)S->sub_w * S->sub_h;
+  if (i >= n) return;
+  double pos[4];
+  d4 v;
+  camera_ray(*S, (uint32_t)(i % S->sub_w), (uint32_t)(i / S->sub_w), pos, v);
+  double *o = states + 9 * i;
+  o[0] = pos[0]; o[1] = pos[1]; o[2] = pos[2]; o[3] = pos[3];
+  o[4] = v.x; o[5] = v.y; o[6] = v.z; o[7] = v.w; o[8] = 0.0;
+}
+
 // ProceduralSky::sample (sky.rs:26-40).
 __device__ inline void sky(double d0, double d1, double d2, double rgb[3]) {
   auto hash = [](double x) -> uint64_t {
@@ -423,6 +447,23 @@ hipError_t launch_rhs(int metric_kind, int precision, const DevScene *scene, uin
     if (precision == 1) GG_RHS_LAUNCH(0, 1); else if (precision == 2) GG_RHS_LAUNCH(0, 2); else GG_RHS_LAUNCH(0, 0);
   }
 #undef GG_RHS_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_constants(int metric_kind, const DevScene *scene, uint64_t n, const double *states, double *out,
+                            hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const uint32_t grid = (uint32_t)((n + 255) / 256);
+  if (metric_kind == 1)
+    hipLaunchKernelGGL((com_kernel<1>), dim3(grid), dim3(256), 0, stream, scene, n, states, out);
+  else
+    hipLaunchKernelGGL((com_kernel<0>), dim3(grid), dim3(256), 0, stream, scene, n, states, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_camera_rays(const DevScene *scene, uint64_t n, double *states, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(camera_rays_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, scene, states);
   return hipGetLastError();
 }
 

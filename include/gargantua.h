@@ -29,6 +29,9 @@
  *   gg_grad_scene_to_scene, gg_geodesic_acceleration_vjp
  *                        no reference counterpart: the scene gg_grad renders
  *                        and the RHS vector-Jacobian product it uses.
+ *   gg_camera_rays       GeodesicRay::from_camera (ray.rs:44-91) per pixel.
+ *   gg_constants_of_motion
+ *                        no reference counterpart: E, L_z, Carter Q, mu^2.
  *
  * Errors: every entry point returns GG_OK (0) or a negative gg_status; the
  * message of the last failure on the calling thread is gg_last_error().
@@ -243,6 +246,18 @@ int gg_trace_states(gg_context *ctx, const gg_scene *scene, int precision,
 int gg_geodesic_acceleration(gg_context *ctx, const gg_metric *metric,
                              int precision, uint64_t n, const double *pos,
                              const double *vel, double *acc);
+
+/* GeodesicRay::from_camera (crates/raytracer/src/curved/ray.rs:44-91) for
+ * every sub-pixel ray of the scene: states[9*i..] = (t r th ph vt vr vth vph
+ * lambda=0), i = sub_py*(width*spa) + sub_px (with jitter when set). */
+int gg_camera_rays(gg_context *ctx, const gg_scene *scene, double *states);
+
+/* Constants of motion of n states (9 doubles each, AoS, as gg_trace_states
+ * takes them): out[4*i..] = (E = -p_t, L_z = p_phi, Carter Q, mu^2 =
+ * -g(v,v)).  No reference counterpart: diagnostics of the integrator's
+ * drift (exact along geodesics; forward-Euler positions drift at O(h)). */
+int gg_constants_of_motion(gg_context *ctx, const gg_metric *metric, uint64_t n,
+                           const double *states, double *out);
 
 /* ---- differentiable thin-disk render (no reference counterpart) ---- */
 typedef struct gg_grad_scene {

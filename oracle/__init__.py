@@ -21,6 +21,7 @@ _SIGS = [
     ("oracle_metric_tensor", C.c_int, [C.POINTER(_abi.gg_metric), _P, _P]),
     ("oracle_christoffel", C.c_int, [C.POINTER(_abi.gg_metric), _P, _P]),
     ("oracle_geodesic_acceleration", C.c_int, [C.POINTER(_abi.gg_metric), C.c_uint64, _P, _P, _P]),
+    ("oracle_constants_of_motion", C.c_int, [C.POINTER(_abi.gg_metric), C.c_uint64, _P, _P]),
     ("oracle_rhs_jacobian", C.c_int, [C.POINTER(_abi.gg_metric), C.c_uint64, _P, _P, _P]),
     ("oracle_event_horizon", C.c_int, [C.POINTER(_abi.gg_metric), C.POINTER(_D)]),
     ("oracle_orbital_four_velocity", C.c_int, [C.POINTER(_abi.gg_metric), _P, _P]),
@@ -87,6 +88,13 @@ class Oracle:
         self.lib.oracle_geodesic_acceleration(C.byref(metric.to_c()), len(pos), pos.ctypes.data, vel.ctypes.data,
                                               acc.ctypes.data)
         return acc
+
+    def constants_of_motion(self, metric, states):
+        """[n, 4]: E, L_z, Carter Q, mu^2 of states [n, 9]."""
+        states = _f64(states).reshape(-1, 9)
+        out = np.zeros((len(states), 4))
+        self.lib.oracle_constants_of_motion(C.byref(metric.to_c()), len(states), states.ctypes.data, out.ctypes.data)
+        return out
 
     def rhs_jacobian(self, metric, pos, vel):
         """[n, 4, 7]: d a^mu / d(r, theta, v^t, v^r, v^theta, v^phi, spin) (dual numbers)."""

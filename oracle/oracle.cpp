@@ -408,6 +408,29 @@ int oracle_geodesic_acceleration(const gg_metric *m, uint64_t n, const double *p
   return 0;
 }
 
+// E = -p_t, L_z = p_phi, Carter Q and mu^2 = -g(v, v) of n states (9
+// doubles each); the textbook Kerr constants in Boyer-Lindquist coordinates.
+int oracle_constants_of_motion(const gg_metric *m, uint64_t n, const double *states, double *out) {
+  MetricP mp = make_metric(*m);
+  const double a = mp.kind == GG_METRIC_KERR ? mp.spin : 0.0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const double *s = states + 9 * i;
+    V4<double> p{{s[0], s[1], s[2], s[3]}};
+    Gmn<double> g = metric_tensor(mp, mp.spin, p);
+    double pt = g.tt * s[4] + g.tph * s[7];
+    double pph = g.tph * s[4] + g.phph * s[7];
+    double pth = g.thth * s[6];
+    double pr = g.rr * s[5];
+    double mu2 = -(pt * s[4] + pr * s[5] + pth * s[6] + pph * s[7]);
+    double st = std::sin(s[2]), ct = std::cos(s[2]);
+    out[4 * i] = -pt;
+    out[4 * i + 1] = pph;
+    out[4 * i + 2] = pth * pth + ct * ct * (a * a * (mu2 - pt * pt) + pph * pph / (st * st));
+    out[4 * i + 3] = mu2;
+  }
+  return 0;
+}
+
 // d a^mu / d(r, theta, v^t, v^r, v^theta, v^phi, spin) of
 // geodesic_acceleration by forward-mode dual numbers; jac[28 i + 7 mu + k].
 int oracle_rhs_jacobian(const gg_metric *m, uint64_t n, const double *pos, const double *vel, double *jac) {

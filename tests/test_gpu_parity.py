@@ -146,3 +146,27 @@ def test_rhs_fast_fp64_accuracy(renderer, oracle, metric):
     rel = np.max(np.abs(a - b) / scale, axis=1)
     print(f"{metric}: fast fp64 rhs max rel {rel.max():.2e}")
     assert rel.max() < 1e-12
+
+
+def test_camera_rays_match_oracle(renderer, oracle):
+    scene = gg.configs.kerr_1024(64, 48)          # jittered, non-square
+    g = renderer.camera_rays(scene)
+    o = np.stack([oracle.from_camera(scene, k % scene.width, k // scene.width) for k in range(scene.n_rays)])
+    scale = np.abs(o).max(axis=0) + 1e-300
+    err = np.abs(g[:, :8] - o[:, :8]) / scale[:8]
+    print(f"camera rays: bit-identical {np.mean(np.all(g[:, :8] == o[:, :8], axis=1)):.4f}, max rel {err.max():.2e}")
+    assert err.max() < 1e-14
+    assert np.all(g[:, 8] == 0.0)
+
+
+@pytest.mark.parametrize("metric", [gg.Schwarzschild(1.0), gg.Kerr(1.0, 0.9)])
+def test_constants_of_motion_match_oracle(renderer, oracle, metric):
+    rng = np.random.default_rng(9)
+    pos, vel = _points(rng, 2048, rmin=3.0)
+    st = np.concatenate([pos, vel, np.zeros((len(pos), 1))], 1)
+    g = renderer.constants_of_motion(metric, st)
+    o = oracle.constants_of_motion(metric, st)
+    scale = np.abs(o).max(axis=1, keepdims=True)
+    err = np.abs(g - o) / scale
+    print(f"{metric}: constants max rel {err.max():.2e}")
+    assert err.max() < 1e-12
