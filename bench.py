@@ -196,13 +196,15 @@ def bench_kerr4096(args):
 
 
 # Algorithmic FLOPs of one accepted step back-propagated by the adjoint
-# kernel: its six RHS recomputed (6 x 237), six RHS vector-Jacobian products
-# (three tangent passes over the Christoffel algebra, 455 ops each in
-# value+tangent arithmetic, plus 190 for the contraction and its v-gradient:
-# 1935 per product) and ~330 for the stage combinations and cotangent
-# updates.  Re-traces and segment recomputations are counted like the trace
-# kernel (237 per RHS, 318 per full attempt).
-F_ADJ_STEP = 6 * 237 + 6 * 1935 + 330
+# reverse kernel: six reverse-mode RHS vector-Jacobian products (628 each,
+# counted from the source of kerr_rhs_vjp: 130 for the forward symbol
+# algebra, 342 for the backward sweep, 52 for the contraction coefficients,
+# 104 for the velocity gradient) and ~250 for the stage cotangents and their
+# combinations.  The stage velocities come from the step record, so no RHS
+# is evaluated.  Re-traces and segment recomputations are counted like the
+# trace kernel (237 per RHS, 318 per full attempt).
+F_VJP = 628
+F_ADJ_STEP = 6 * F_VJP + 250
 
 
 def bench_grad512(args):
@@ -282,7 +284,8 @@ def bench_grad512(args):
                                    "(0.7, 60deg, 2.5, r_ISCO(0.7)); value at K=32",
                        "K": 32},
             "roofline": {"bound": "valu", "achieved": best["adjoint_tflops"], "peak": PEAK_FP64, "unit": "TFLOP/s",
-                         "frac": best["adjoint_frac_fp64_peak"], "traffic": None, "kernel": "adjoint_kernel"},
+                         "frac": best["adjoint_frac_fp64_peak"], "traffic": None,
+                         "kernel": "adjoint_fwd_kernel + adjoint_rev_kernel + adjoint_cam_kernel"},
             "per_K": per_k, "grad_identical_across_K": same, "gradient_descent": gd}
 
 
@@ -293,7 +296,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, default=1024)
-    ap.add_argument("--workload", default="kerr1024", choices=["kerr1024", "batch64", "kerr4096", "grad512"])
+    ap.add_argument("--workload", default="kerr1024", choices=["kerr1024", "batch64", "kerr4096", "grad512", "copy"],
+                    help="copy: one 512 MiB device-to-device copy, the reference kernel that calibrates the "
+                         "FETCH_SIZE/WRITE_SIZE counters for HBM traffic")
     ap.add_argument("--gd-steps", type=int, default=200, help="grad512: gradient-descent iterations (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -308,6 +313,15 @@ def main():
         return
     if args.workload == "grad512":
         print(json.dumps(bench_grad512(args)))
+        return
+    if args.workload == "copy":
+        import torch
+        x = torch.ones(64 * 2**20, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"workload": "copy", "bytes_read": x.numel() * 8, "bytes_written": x.numel() * 8}))
         return
 
     import torch
