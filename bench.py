@@ -334,10 +334,27 @@ def main():
     outs_m = _dev_outputs(torch, n)
     outs_d = _dev_outputs(torch, n)
 
-    wall_m, kms_m, st_m = time_precision(torch, rend, scene, "mixed", args.steps, args.warmup, outs_m)
-    wall_d, kms_d, st_d = time_precision(torch, rend, scene, "fp64", args.steps, args.warmup, outs_d)
+    # Launch shape per precision from the autotuner: block size, blocks per CU,
+    # and whether rays are queued longest first by the previous render's
+    # attempt counts (every ray is traced in full either way).
+    cands = [(256, 2, 16, 2, 0), (256, 2, 16, 2, 1), (64, 8, 16, 2, 0), (64, 8, 16, 2, 1)]
+    tuned = {}
+    for prec in ("mixed", "fp64", "fp64fast"):
+        tuned[prec] = rend.autotune([scene], prec, candidates=cands)[0]
+        _log(f"kerr1024 {prec}: tuned launch {tuned[prec]}")
+    tile_ms = {}
+
+    def run(prec, outs):
+        L = tuned[prec]
+        rend.set_launch(L["block_threads"], L["blocks_per_cu"], L["refill_interval"], 1, 0)
+        tile_ms[prec] = rend.render(scene, prec, outputs=dict(_ptrs(outs)))["stats"]["kernel_ms"]   # first frame
+        rend.set_launch(L["block_threads"], L["blocks_per_cu"], L["refill_interval"], 1, L["order"])
+        return time_precision(torch, rend, scene, prec, args.steps, args.warmup, outs)
+
+    wall_m, kms_m, st_m = run("mixed", outs_m)
+    wall_d, kms_d, st_d = run("fp64", outs_d)
     outs_f = _dev_outputs(torch, n)
-    wall_f, kms_f, st_f = time_precision(torch, rend, scene, "fp64fast", args.steps, args.warmup, outs_f)
+    wall_f, kms_f, st_f = run("fp64fast", outs_f)
 
     # accuracy of the mixed kernel against the FP64 kernel (same inputs)
     host = lambda o: {k: v.cpu().numpy() for k, v in o.items()}
@@ -363,7 +380,10 @@ def main():
         "config": {"workload": f"Kerr a=0.9 {args.size}x{args.size} thin disk r_ISCO..30M q=3, camera r=100M "
                                "i=60deg FOV 25deg, jitter seed 2026, rtol 1e-8 atol 1e-10, escape 1000M, "
                                "capture 1.01 r_h, 20000 attempts", "precision": "mixed",
-                   "rays": n, "launch": rend.get_launch()},
+                   "rays": n, "launch": tuned["mixed"],
+                   "schedule": "autotuned launch; order 1 = rays queued longest first by the previous render's "
+                               "attempt counts (learned in the warm-up), every ray traced in full",
+                   "first_frame_tile_order_ms": tile_ms},
         "roofline": {"bound": "valu", "achieved": tflops, "peak": peak, "unit": "TFLOP/s", "frac": tflops / peak,
                      "traffic": None, "kernel": "trace_kernel<Kerr, thin, mixed>",
                      "kernel_ms": k_ms, "flop_per_launch": f32 + f64, "fp32_flop_share": f32 / (f32 + f64)},

@@ -256,8 +256,8 @@ class Renderer:
         except Exception:
             pass
 
-    def set_launch(self, block_threads=0, blocks_per_cu=0, refill_interval=0, graph=0):
-        L = _abi.gg_launch(block_threads, blocks_per_cu, refill_interval, graph)
+    def set_launch(self, block_threads=0, blocks_per_cu=0, refill_interval=0, graph=0, order=0):
+        L = _abi.gg_launch(block_threads, blocks_per_cu, refill_interval, graph, order)
         _check(self.lib, self.lib.gg_set_launch(self.ctx, C.byref(L)), "gg_set_launch")
 
     def get_launch(self) -> dict:

@@ -73,7 +73,7 @@ class gg_stats(C.Structure):
 
 class gg_launch(C.Structure):
     _fields_ = [("block_threads", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("refill_interval", C.c_uint32), ("graph", C.c_uint32)]
+                ("refill_interval", C.c_uint32), ("graph", C.c_uint32), ("order", C.c_uint32)]
 
 
 class gg_grad_scene(C.Structure):

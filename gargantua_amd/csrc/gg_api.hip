@@ -8,6 +8,7 @@
 // on the context's stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -74,7 +75,7 @@ struct gg_context {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   uint32_t num_cus = 256;
-  gg_launch launch{256, 2, 16, 1};
+  gg_launch launch{256, 2, 16, 1, 0};
   // cached batch graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -84,11 +85,16 @@ struct gg_context {
     uint64_t n_rays;
     gg_launch launch;
     void *scenes, *outs, *counters;
+    const void *order;
   } graph_key{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf scenes, outs, counters, states_in;
   DevBuf stage[9];       // device staging for host-memory outputs
   DevBuf rhs_io, misc_io;
+  // learned ray order (gg_launch.order = 1): queue position -> ray of the
+  // last camera render, the signature of its grids, the sort's work buffers
+  DevBuf order, order_keys, order_ids, order_keys_out, order_att, order_tmp;
+  uint64_t order_sig = 0;
   std::vector<DevBuf> batch_stage;
   // gg_grad: cls, intensity, accepted, target, resid, dI, list, counters,
   // results, scene, checkpoint pool, segment buffers, keys, sorted list,
@@ -391,6 +397,8 @@ int gg_context_destroy(gg_context *ctx) {
   ctx->scenes.release(); ctx->outs.release(); ctx->counters.release(); ctx->states_in.release();
   ctx->rhs_io.release();
   ctx->misc_io.release();
+  ctx->order.release(); ctx->order_keys.release(); ctx->order_ids.release(); ctx->order_keys_out.release();
+  ctx->order_att.release(); ctx->order_tmp.release();
   for (auto &b : ctx->grad) b.release();
   for (auto &b : ctx->stage) b.release();
   for (auto &b : ctx->batch_stage) b.release();
@@ -411,6 +419,7 @@ int gg_set_launch(gg_context *ctx, const gg_launch *l) {
   if (v.refill_interval == 0) v.refill_interval = 16;
   if (v.graph == 0) v.graph = 1;
   if (v.graph > 2) return fail(GG_ERR_INVALID_ARG, "graph must be 0, 1 or 2");
+  if (v.order > 1) return fail(GG_ERR_INVALID_ARG, "order must be 0 or 1");
   if (v.block_threads % 64 || v.block_threads > 256)
     return fail(GG_ERR_INVALID_ARG, "block_threads must be a multiple of 64 in [64, 256]");
   if (v.blocks_per_cu > 64) return fail(GG_ERR_INVALID_ARG, "blocks_per_cu must be <= 64");
@@ -487,6 +496,7 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
       size_t rgb_bytes = 3ull * scenes[i].image.width * scenes[i].image.height;
       if ((rc = plan_out(st[8], o.rgb, rgb_bytes, plans[i][8]))) return rc;
     }
+
     DevOut &d = dout[i];
     d.cls = (uint8_t *)plans[i][0].dev;
     d.r = (double *)plans[i][1].dev;
@@ -497,6 +507,19 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
     d.accepted = (uint32_t *)plans[i][6].dev;
     d.state = (double *)plans[i][7].dev;
     d.n_rays = n;
+  }
+  // learned ray order: attempts are the sort keys of the next render's queue
+  const bool learn_order = !from_states && ctx->launch.order == 1 && total > 0 && total <= 0xFFFFFFFFull;
+  uint64_t sig = 1469598103934665603ull;   // FNV-1a over the scenes' ray grids
+  if (learn_order) {
+    auto mix = [&](uint64_t v) { sig = (sig ^ v) * 1099511628211ull; };
+    mix(n_scenes);
+    for (uint32_t i = 0; i < n_scenes; ++i) { mix(ds[i].sub_w); mix(ds[i].sub_h); mix(ds[i].tiled); }
+    bool need = false;
+    for (uint32_t i = 0; i < n_scenes; ++i) need |= !plans[i][5].dev;
+    if (need && (rc = ctx->order_att.ensure(4 * total))) return rc;
+    for (uint32_t i = 0; i < n_scenes; ++i)
+      if (!plans[i][5].dev) dout[i].attempts = (uint32_t *)ctx->order_att.p + ds[i].ray_begin;
   }
   if ((rc = ctx->scenes.ensure(sizeof(DevScene) * n_scenes))) return rc;
   if ((rc = ctx->outs.ensure(sizeof(DevOut) * n_scenes))) return rc;
@@ -522,12 +545,13 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
   T.states_in = states_dev;
   T.counters = (Counters *)ctx->counters.p;
   T.refill_interval = ctx->launch.refill_interval;
+  T.order = (learn_order && ctx->order_sig == sig) ? (const uint32_t *)ctx->order.p : nullptr;
   uint32_t grid = grid_for(ctx, total);
   uint32_t block = ctx->launch.block_threads;
   const bool use_graph = (n_scenes > 1) && ctx->launch.graph == 1 && total > 0;
   if (use_graph) {
     gg_context::GraphKey key{n_scenes, scenes[0].metric.kind, scenes[0].disk.kind, precision, total, ctx->launch,
-                             ctx->scenes.p, ctx->outs.p, ctx->counters.p};
+                             ctx->scenes.p, ctx->outs.p, ctx->counters.p, (const void *)T.order};
     if (!ctx->graph_exec || std::memcmp(&key, &ctx->graph_key, sizeof(key)) != 0) {
       if (ctx->graph_exec) { (void)hipGraphExecDestroy(ctx->graph_exec); ctx->graph_exec = nullptr; }
       if (ctx->graph) { (void)hipGraphDestroy(ctx->graph); ctx->graph = nullptr; }
@@ -563,6 +587,25 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
       HIP_TRY(launch_shade(dsp, dop, scenes[i].image.width, scenes[i].image.height,
                            scenes[i].image.samples_per_axis, (uint8_t *)plans[i][8].dev, s));
     }
+  }
+  if (learn_order) {
+    // the next render's queue: rays by this render's attempt counts, longest first
+    if ((rc = ctx->order.ensure(4 * total)) || (rc = ctx->order_keys.ensure(4 * total)) ||
+        (rc = ctx->order_ids.ensure(4 * total)) || (rc = ctx->order_keys_out.ensure(4 * total)))
+      return rc;
+    uint32_t max_steps = 0;
+    for (uint32_t i = 0; i < n_scenes; ++i) max_steps = std::max(max_steps, scenes[i].integrator.max_steps);
+    int key_bits = 1;
+    while (key_bits < 32 && (max_steps >> key_bits) != 0) ++key_bits;
+    size_t temp_bytes = 0;
+    HIP_TRY(sort_rays_desc(nullptr, nullptr, nullptr, nullptr, total, key_bits, nullptr, temp_bytes, s));
+    if ((rc = ctx->order_tmp.ensure(temp_bytes ? temp_bytes : 1))) return rc;
+    HIP_TRY(launch_order_keys((const DevScene *)ctx->scenes.p, (const DevOut *)ctx->outs.p, n_scenes, total,
+                              (uint32_t *)ctx->order_keys.p, (uint32_t *)ctx->order_ids.p, s));
+    HIP_TRY(sort_rays_desc((const uint32_t *)ctx->order_keys.p, (uint32_t *)ctx->order_keys_out.p,
+                           (const uint32_t *)ctx->order_ids.p, (uint32_t *)ctx->order.p, total, key_bits,
+                           ctx->order_tmp.p, temp_bytes, s));
+    ctx->order_sig = sig;
   }
   Counters cnt;
   HIP_TRY(hipMemcpyAsync(&cnt, T.counters, sizeof(Counters), hipMemcpyDeviceToHost, s));
@@ -612,21 +655,35 @@ int gg_autotune(gg_context *ctx, const gg_scene *scenes, uint32_t n, int precisi
   } else {
     for (uint32_t bt : {64u, 128u, 256u})
       for (uint32_t bpc : {2u, 4u, 8u})
-        for (uint32_t ri : {8u, 32u}) cands.push_back(gg_launch{bt, bpc * 256u / bt, ri, 2});
+        for (uint32_t ri : {8u, 32u}) cands.push_back(gg_launch{bt, bpc * 256u / bt, ri, 2, 0});
   }
   const gg_launch saved = ctx->launch;
   std::vector<gg_outputs> none(n);
   std::memset(none.data(), 0, sizeof(gg_outputs) * n);
   double best_ms = 1e300;
   gg_launch best_l = saved;
-  for (const gg_launch &c : cands) {
+  // Each candidate renders twice and the second launch is timed: graphs are
+  // built and, with order = 1, the ray order is learned by the first.
+  auto trial = [&](const gg_launch &c) -> int {
     int rc = gg_set_launch(ctx, &c);
-    if (rc) { ctx->launch = saved; return rc; }
+    if (rc) return rc;
     gg_stats st;
-    rc = render_impl(ctx, scenes, n, precision, none.data(), nullptr, 0, &st);  // warm-up (and graph build)
+    rc = render_impl(ctx, scenes, n, precision, none.data(), nullptr, 0, &st);
     if (!rc) rc = render_impl(ctx, scenes, n, precision, none.data(), nullptr, 0, &st);
-    if (rc) { ctx->launch = saved; return rc; }
+    if (rc) return rc;
     if (st.kernel_ms < best_ms) { best_ms = st.kernel_ms; best_l = ctx->launch; }
+    return GG_OK;
+  };
+  for (const gg_launch &c : cands) {
+    int rc = trial(c);
+    if (rc) { ctx->launch = saved; return rc; }
+  }
+  if (!(candidates && n_candidates)) {
+    // second stage of the built-in search: the best shape with the learned order
+    gg_launch c = best_l;
+    c.order = 1;
+    int rc = trial(c);
+    if (rc) { ctx->launch = saved; return rc; }
   }
   best_l.graph = saved.graph;
   ctx->launch = best_l;

@@ -36,6 +36,7 @@ struct TraceLaunch {
   const double *states_in;    // gg_trace_states: 9 doubles per ray (AoS), else null
   Counters *counters;
   uint32_t refill_interval;
+  const uint32_t *order;      // camera renders: queue position -> global ray, or null (ray order)
 };
 
 // Tallies of gg_grad (zeroed by launch_grad_reset).
@@ -108,6 +109,10 @@ hipError_t launch_rhs(int metric_kind, int precision, const DevScene *scene, uin
 hipError_t launch_constants(int metric_kind, const DevScene *scene, uint64_t n, const double *states, double *out,
                             hipStream_t stream);
 hipError_t launch_camera_rays(const DevScene *scene, uint64_t n, double *states, hipStream_t stream);
+// keys[i] = attempts of global ray i (from its scene's attempts output),
+// ids[i] = i: input of the cost-order sort.
+hipError_t launch_order_keys(const DevScene *scenes, const DevOut *outs, uint32_t n_scenes, uint64_t n,
+                             uint32_t *keys, uint32_t *ids, hipStream_t stream);
 // RGB image (renderer.rs:44-80 shading + sRGB) from per-ray outputs.
 hipError_t launch_shade(const DevScene *scene, const DevOut *out, uint32_t width, uint32_t height, uint32_t spa,
                         uint8_t *rgb, hipStream_t stream);

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
 
   // Start ray `ray` in this lane.
   auto init_ray = [&](uint64_t rid) {
-    ray = rid;
+    ray = (SRC == 0 && T.order) ? (uint64_t)T.order[rid] : rid;
     si = single ? 0u : find_scene(T.scenes, T.n_scenes, ray);
     if (!single) Sp = T.scenes + si;
     uint64_t local = ray - Sp->ray_begin;
@@ -328,6 +328,19 @@ __global__ void camera_rays_kernel(const DevScene *S, double *states) {
   o[4] = v.x; o[5] = v.y; o[6] = v.z; o[7] = v.w; o[8] = 0.0;
 }
 
+// Sort keys of the learned ray order: the attempt count of global ray i,
+// read from its scene's attempts output at its pixel.
+__global__ void order_keys_kernel(const DevScene *scenes, const DevOut *outs, uint32_t n_scenes, uint64_t n,
+                                  uint32_t *keys, uint32_t *ids) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DevScene &S = scenes[find_scene(scenes, n_scenes, i)];
+  uint32_t px, py;
+  ray_pixel(S, i - S.ray_begin, px, py);
+  keys[i] = outs[S.out_index].attempts[(uint64_t)py * S.sub_w + px];
+  ids[i] = (uint32_t)i;
+}
+
 // ProceduralSky::sample (sky.rs:26-40).
 __device__ inline void sky(double d0, double d1, double d2, double rgb[3]) {
   auto hash = [](double x) -> uint64_t {
@@ -464,6 +477,14 @@ hipError_t launch_constants(int metric_kind, const DevScene *scene, uint64_t n, 
 hipError_t launch_camera_rays(const DevScene *scene, uint64_t n, double *states, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(camera_rays_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, scene, states);
+  return hipGetLastError();
+}
+
+hipError_t launch_order_keys(const DevScene *scenes, const DevOut *outs, uint32_t n_scenes, uint64_t n,
+                             uint32_t *keys, uint32_t *ids, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(order_keys_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, scenes, outs,
+                     n_scenes, n, keys, ids);
   return hipGetLastError();
 }
 

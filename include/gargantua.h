@@ -149,8 +149,8 @@ typedef struct gg_scene {
 typedef enum gg_precision {
   GG_PREC_FP64 = 0,       /* every operation in FP64, reference expression order */
   GG_PREC_MIXED = 1,      /* RHS stages in FP32 where the step-size controller is
-                             saturated and FP32 rounding cannot change a decision;
-                             fast FP64 elsewhere                                  */
+                             saturated and FP32 rounding cannot change a decision
+                             (whole wavefronts); exact FP64 elsewhere             */
   GG_PREC_FP64_FAST = 2   /* FP64 with reciprocal-based RHS algebra and FMA      */
 } gg_precision;
 
@@ -200,6 +200,13 @@ typedef struct gg_launch {
   uint32_t graph;             /* gg_render_batch: 1 = capture the launch in a
                                  hipGraph and replay it on later calls with the
                                  same shape (default), 2 = launch directly     */
+  uint32_t order;             /* gg_render of one scene: 0 = rays in 8x8-tile
+                                 order (default); 1 = longest first, by the
+                                 per-ray attempt counts of the previous render
+                                 with the same sub-pixel grid (tile order for
+                                 the first).  Every ray is traced in full
+                                 either way; FP64 outputs are identical, mixed
+                                 outputs agree to FP32 rounding.              */
 } gg_launch;
 
 typedef struct gg_context gg_context;
@@ -228,9 +235,12 @@ int gg_render_batch(gg_context *ctx, const gg_scene *scenes, uint32_t n,
                     int precision, const gg_outputs *out, gg_stats *stats);
 
 /* Times the batch (outputs discarded) under each candidate launch shape and
- * installs the fastest on the context.  candidates == NULL: a built-in grid
- * of block sizes, blocks per CU and refill intervals.  No reference
- * counterpart (the reference schedules rows with rayon). */
+ * installs the fastest on the context (each candidate renders twice, the
+ * second launch is timed).  candidates == NULL: a built-in grid of block
+ * sizes, blocks per CU and refill intervals in tile order, then the best of
+ * them with the learned ray order (order = 1).  The graph setting of the
+ * context is kept.  No reference counterpart (the reference schedules rows
+ * with rayon). */
 int gg_autotune(gg_context *ctx, const gg_scene *scenes, uint32_t n, int precision,
                 const gg_launch *candidates, uint32_t n_candidates, gg_launch *best,
                 double *best_kernel_ms);

@@ -104,3 +104,34 @@ def test_every_ray_retired_once_with_refills(renderer, precision):
     b = renderer.render(scene, precision, state=False)
     renderer.set_launch(256, 2, 16, 1)
     assert_same(a, b, precision)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_learned_ray_order_changes_no_result(renderer, precision):
+    """gg_launch.order = 1 queues rays longest first by the previous render's
+    attempt counts: only the schedule changes (FP64 bitwise, mixed to FP32
+    rounding), and every ray is still traced."""
+    scene = gg.configs.kerr_1024(256, 256)
+    renderer.set_launch(256, 2, 16, 1, 0)
+    ref = renderer.render(scene, precision, state=False)
+    renderer.set_launch(256, 2, 16, 1, 1)
+    first = renderer.render(scene, precision, state=False)     # learns the order
+    second = renderer.render(scene, precision, state=False)    # uses it
+    renderer.set_launch(256, 2, 16, 1, 0)
+    for o in (first, second):
+        assert o["stats"]["rays"] == scene.n_rays
+        assert o["stats"]["attempts"] == int(o["attempts"].astype(np.uint64).sum())
+        assert_same(o, ref, precision)
+
+
+def test_learned_ray_order_in_batches(renderer):
+    scenes = _scenes()
+    renderer.set_launch(256, 2, 16, 1, 0)
+    ref, _ = renderer.render_batch(scenes, "fp64")
+    renderer.set_launch(256, 2, 16, 1, 1)
+    for rep in range(3):                      # learn, then replay the graph with the order
+        outs, st = renderer.render_batch(scenes, "fp64")
+        assert st["rays"] == sum(s.n_rays for s in scenes)
+        for a, b in zip(outs, ref):
+            assert_same(a, b, "fp64")
+    renderer.set_launch(256, 2, 16, 1, 0)
