@@ -291,6 +291,7 @@ void build_dev_scene(const gg_scene &s, DevScene &D) {
   D.edge_width = s.disk.edge_width;
   D.fp32_min_r = 10.0 * D.mass;
   D.fp32_far_r = 300.0 * D.mass;
+  D.fp32_b2_min = 100.0 * D.mass * D.mass;   // b >= 10 M: periapsis beyond ~8 M for any spin
 }
 
 bool is_device_ptr(const void *p) {

@@ -185,6 +185,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
       L.k1_ok = false; L.k1_f32 = false;
       L.accepted = 0;
       L.err_prev = 1.0;
+      L.b2 = 0.0;
       att = 0;
       active = true;
       if (acc_target == 0 || chk_base + nchk > G.chk_cap || tail_base + (acc_target - last0) > G.tail_cap) {

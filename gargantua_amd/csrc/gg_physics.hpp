@@ -49,8 +49,9 @@ struct DevScene {
   // disk
   double r_in, r_out, scale_height, density_0, absorption, q, edge_width;
   // mixed precision policy: FP32 stages on outgoing rays beyond fp32_min_r,
-  // on inbound rays beyond fp32_far_r
-  double fp32_min_r, fp32_far_r;
+  // on inbound rays beyond fp32_far_r, and on inbound rays beyond fp32_min_r
+  // whose squared impact parameter (Q + L_z^2)/E^2 is at least fp32_b2_min
+  double fp32_min_r, fp32_far_r, fp32_b2_min;
   // batch bookkeeping
   uint64_t ray_begin;     // first global ray index of this scene
   uint64_t out_index;     // which output set (batch)

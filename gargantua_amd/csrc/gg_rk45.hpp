@@ -64,6 +64,7 @@ struct Lane {
   uint32_t attempts, accepted;
   double intensity, transmission;
   double err_prev;   // error ratio of the previous full attempt
+  double b2;         // (Q + L_z^2) / E^2 of the ray's start state (mixed kernel)
   uint32_t n_rhs, n_rhs32, n_full;
 };
 
@@ -257,15 +258,18 @@ __device__ inline StageOut stages(const DevScene &S, const KStore &K, const doub
 
 // Whether the next attempt of this ray may run its stages in FP32: the
 // controller is saturated (h == max_step, previous error ratio < 0.3), the
-// ray is outgoing beyond fp32_min_r (photons outside the photon-orbit region
-// never turn back, so FP32 rounding is not amplified by a later close
-// approach) or inbound beyond fp32_far_r, theta is away from the poles and
-// the FP32 noise bound is below 0.02 in error-ratio units.  A function of the
-// ray's own state only, so every ray's arithmetic is independent of which
-// lane or wavefront carries it.
+// ray is far enough from the photon-orbit region that FP32 rounding is not
+// amplified by a later close approach -- outgoing beyond fp32_min_r
+// (outgoing photons there never turn back), inbound beyond fp32_far_r, or
+// inbound beyond fp32_min_r with an impact parameter b = sqrt((Q+L_z^2)/E^2)
+// of at least 10 M, whose periapsis lies beyond ~8 M for any spin -- theta
+// is away from the poles and the FP32 noise bound is below 0.02 in
+// error-ratio units.  A function of the ray's own state only, so every ray's
+// arithmetic is independent of which lane or wavefront carries it.
 __device__ inline bool fp32_eligible(const DevScene &S, const Lane &L) {
   const double r = L.pos[1], th = L.pos[2];
-  bool far_enough = (L.vel.y > 0.0) ? (r >= S.fp32_min_r) : (r >= S.fp32_far_r);
+  bool far_enough = (L.vel.y > 0.0) ? (r >= S.fp32_min_r)
+                                     : (r >= S.fp32_far_r || (r >= S.fp32_min_r && L.b2 >= S.fp32_b2_min));
   if (!(far_enough && (L.h == S.max_step) && (L.err_prev < 0.3) && (th > 1e-3) && (th < PI - 1e-3)))
     return false;
   return fp32_noise(S, L.pos, L.vel, L.h) < 0.02f;

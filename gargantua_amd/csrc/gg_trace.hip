@@ -88,6 +88,12 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
     L.h = Sp->h0;
     L.k1_ok = false;
     L.k1_f32 = false;
+    L.b2 = 0.0;
+    if (PREC == 1) {
+      double c[4];
+      constants_of_motion<METRIC>(*Sp, L.pos, L.vel, c);
+      L.b2 = c[0] > 0.0 ? (c[2] + c[1] * c[1]) / (c[0] * c[0]) : 0.0;
+    }
     L.attempts = 0; L.accepted = 0;
     L.intensity = 0.0; L.transmission = 1.0;
     L.err_prev = 1.0;
@@ -147,6 +153,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
           X[5 * B + dst] = __longlong_as_double((long long)(((uint64_t)L.attempts << 32) | L.accepted));
           X[6 * B + dst] = __longlong_as_double(
               (long long)(((uint64_t)si << 2) | (L.k1_f32 ? 2u : 0u) | (L.k1_ok ? 1u : 0u)));
+          X[7 * B + dst] = L.b2;
         }
         __syncthreads();
         if (now_active) {
@@ -157,6 +164,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
           L.attempts = (uint32_t)(aa >> 32); L.accepted = (uint32_t)aa;
           uint64_t sf = (uint64_t)__double_as_longlong(X[6 * B + tid]);
           si = (uint32_t)(sf >> 2); L.k1_ok = (sf & 1u) != 0; L.k1_f32 = (sf & 2u) != 0;
+          L.b2 = X[7 * B + tid];
           if (!single) Sp = T.scenes + si;
           K.put(0, k1);
         }
