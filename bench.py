@@ -396,6 +396,8 @@ def main():
                                              "err_le_1e-6_outside_band", "attempts_equal", "psnr_intensity",
                                              "state_bit_identical")},
         "work": {"attempts": st_m["attempts"], "rhs_evals": st_m["rhs_evals"], "rhs_evals_fp32": st_m["rhs_evals_fp32"],
+                 "rhs_evals_fast_fp64": st_m["rhs_evals_fast"],
+                 "rhs_evals_exact_fp64": st_m["rhs_evals"] - st_m["rhs_evals_fp32"] - st_m["rhs_evals_fast"],
                  "attempts_per_ray": st_m["attempts"] / n},
         "cpu_baseline": None,
     }

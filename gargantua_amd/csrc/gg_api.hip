@@ -627,6 +627,7 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
     stats->attempts = cnt.attempts;
     stats->rhs_evals = cnt.rhs;
     stats->rhs_evals_fp32 = cnt.rhs32;
+    stats->rhs_evals_fast = cnt.rhsf;
     stats->attempts_full = cnt.attempts_full;
     stats->grid_blocks = grid;
     stats->block_threads = block;

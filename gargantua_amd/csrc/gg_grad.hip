@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
   const DevScene &S = *G.scene;
   const uint32_t KK = G.K;
   Lane L;
-  L.n_rhs = 0; L.n_rhs32 = 0; L.n_full = 0;
+  L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0;
   bool active = false, exhausted = false;
   uint64_t q = 0, chk_base = 0, tail_base = 0;
   uint32_t pix = 0, acc_target = 0, last0 = 0, att = 0;
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
       camera_ray(S, pix % G.width, pix / G.width, L.pos, L.vel);
       L.lambda = 0.0;
       L.h = S.h0;
-      L.k1_ok = false; L.k1_f32 = false;
+      L.k1_ok = false; L.k1_f32 = false; L.k1_fast = false;
       L.accepted = 0;
       L.err_prev = 1.0;
       L.b2 = 0.0;
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
     double p0[4];
     d4 v0;
     double *recp = L.accepted >= last0 ? G.tail + STEP_REC * (tail_base + (L.accepted - last0)) : nullptr;
-    const int res = attempt<1, 0, true>(S, L, K, acc, p0, v0, false, recp);
+    const int res = attempt<1, 0, true>(S, L, K, acc, p0, v0, false, false, recp);
     ++n_att;
     if (res != RES_CONTINUE || ++att > S.max_steps) {   // the render reached the crossing; a re-trace must too
       mark_bad(G, q, pix, n_bad);
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
   double *const seg = G.seg + gtid * (uint64_t)KK * STEP_REC;
 
   Lane L;
-  L.n_rhs = 0; L.n_rhs32 = 0; L.n_full = 0;
+  L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0;
   enum { IDLE = 0, REV = 1, RECOMP = 2 };
   int phase = IDLE;
   bool exhausted = false, first = false;
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
       double p0[4];
       d4 v0;
       double *recp = seg + STEP_REC * (uint64_t)(L.accepted - c * KK);
-      const int res = attempt<1, 0, true>(S, L, K, acc, p0, v0, false, recp);
+      const int res = attempt<1, 0, true>(S, L, K, acc, p0, v0, false, false, recp);
       ++n_att;
       if (res != RES_CONTINUE || ++att > S.max_steps) {
         mark_bad(G, q, pix, n_bad);
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
       } else if (c > 0) {
         --c;
         get_state(G.chk + 9 * (chk_base + c), L.pos, L.vel, L.h);
-        L.k1_ok = false; L.k1_f32 = false;
+        L.k1_ok = false; L.k1_f32 = false; L.k1_fast = false;
         L.accepted = c * KK;
         att = 0;
         phase = RECOMP;

@@ -25,7 +25,8 @@ struct Counters {
   unsigned long long rhs32;
   unsigned long long attempts_full;
   unsigned long long retired;
-  unsigned long long pad[2];
+  unsigned long long rhsf;           // RHS evaluations of fast-FP64 attempts (mixed)
+  unsigned long long pad[1];
 };
 
 struct TraceLaunch {

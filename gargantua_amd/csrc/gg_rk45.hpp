@@ -61,11 +61,12 @@ struct Lane {
   double h;
   bool k1_ok;        // LDS slot 0 holds k1 = a(pos, vel)
   bool k1_f32;       // ... evaluated by an FP32 attempt (mixed kernel)
+  bool k1_fast;      // ... evaluated by a fast-FP64 attempt (mixed kernel)
   uint32_t attempts, accepted;
   double intensity, transmission;
   double err_prev;   // error ratio of the previous full attempt
   double b2;         // (Q + L_z^2) / E^2 of the ray's start state (mixed kernel)
-  uint32_t n_rhs, n_rhs32, n_full;
+  uint32_t n_rhs, n_rhs32, n_rhsf, n_full;
 };
 
 // Per-lane view of the LDS stage store: slots 0..4 hold k1..k5, structure of
@@ -276,19 +277,27 @@ __device__ inline bool fp32_eligible(const DevScene &S, const Lane &L) {
 }
 
 // One RK45Integrator::step (geodesic.rs:185-328) on lane state L with the
-// stage values in LDS.  PREC 0: exact FP64; 2: fast FP64; 1: mixed: a
-// wavefront runs its stages in FP32 when every live lane's ray is
-// FP32-eligible, and in exact FP64 otherwise (FP64 is always allowed), so a
-// wavefront never executes both loops.  Every accept/reject decision and
-// every h equal those of the FP64 kernel either way; which FP32-eligible
-// steps do run in FP32 depends on the lane grouping, so mixed final states
-// reproduce across launches to FP32 rounding (~1e-12 relative), not bitwise.
+// stage values in LDS.  PREC 0: exact FP64; 2: fast FP64; 1: mixed, three
+// tiers chosen per wavefront (use32, use_fast are wavefront-uniform):
+//   FP32 stages when every live lane's ray is FP32-eligible;
+//   fast-FP64 stages (reciprocal-form RHS, FMA) otherwise, unless a lane's
+//     controller is unsaturated with a tiny previous error;
+//   exact-order FP64 stages otherwise.
+// A lane redoes its attempt one tier down when the result could steer a
+// decision differently from exact FP64: an FP32 error ratio >= 0.4, a fast
+// ratio within 1e-9 of the accept threshold, a fast ratio below 1e-6 with
+// h < max_step (geodesic.rs:318 keeps h when the error rounds to exactly
+// 0, so there the last bit decides the next h), or any invalid stage.
+// Every accept/reject decision is then the FP64 kernel's; h agrees with it
+// to rounding (bitwise while only FP32 and exact steps are taken), and final
+// states reproduce across launches to ~1e-12, not bitwise, since which
+// eligible steps take which tier depends on the lane grouping.
 // Returns RES_*; `acc` reports acceptance; p0/v0 keep the pre-step state.
 // REC (adjoint kernels, PREC 0): rec, when non-null, receives the step
 // record of the attempt if it is accepted.
 template <int METRIC, int PREC, bool REC = false>
 __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool &acc, double p0[4], d4 &v0,
-                              bool use32, double *rec = nullptr) {
+                              bool use32, bool use_fast = false, double *rec = nullptr) {
 #pragma clang fp contract(off)
   constexpr int BASE = (PREC == 2) ? ARITH_FAST : ARITH_EXACT;
   acc = false;
@@ -312,13 +321,25 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
     done = !o.invalid && (o.err / o.thresh < 0.4);
     if (!done) use32 = false;
   }
-  if (!done) {
-    // An FP64 attempt needs an FP64 k1: after an FP32 attempt k7 (the FSAL
-    // k1) carries FP32 rounding, so recompute it.
+  bool fast = false;
+  if (PREC == 1 && !done && use_fast) {
+    // fast FP64 needs an FP64 k1 (either arithmetic); an FP32 one is redone
     const bool need_k1 = !L.k1_ok || L.k1_f32;
+    o = stages<METRIC, ARITH_FAST>(S, K, p0, vel, h, need_k1);
+    L.n_rhs += o.nr;
+    L.n_rhsf += o.nr;
+    if (need_k1) { L.k1_ok = true; L.k1_f32 = false; L.k1_fast = true; }
+    const double ratio = o.err / o.thresh;
+    done = !o.invalid && !(h < S.max_step && ratio < 1e-6) && !(fabs(ratio - 1.0) < 1e-9);
+    fast = done;
+  }
+  if (!done) {
+    // An exact attempt needs an exact k1: after an FP32 or fast attempt k7
+    // (the FSAL k1) carries that arithmetic's rounding, so recompute it.
+    const bool need_k1 = !L.k1_ok || L.k1_f32 || L.k1_fast;
     o = stages<METRIC, BASE, REC>(S, K, p0, vel, h, need_k1, rec);
     L.n_rhs += o.nr;
-    if (need_k1) L.k1_f32 = false;
+    if (need_k1) { L.k1_f32 = false; L.k1_fast = false; }
   }
   L.k1_ok = true;
   if (o.invalid) { L.h = h * 0.5; return RES_CONTINUE; }
@@ -344,6 +365,7 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
     K.put(0, o.k7);                                 // FSAL: k7 is the next k1 ...
     L.k1_ok = !refl;                                // ... unless theta was reflected
     L.k1_f32 = use32;
+    L.k1_fast = fast;
     if (err > 0.0) {
       double hn = h * 0.9 * pow(thresh / err, 0.2);
       if (hn < S.min_step) hn = S.min_step;

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
   extern __shared__ double lds_k[];
   const KStore K{lds_k + threadIdx.x, blockDim.x};
   Lane L;
-  L.n_rhs = 0; L.n_rhs32 = 0; L.n_full = 0;
+  L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0;
   uint32_t n_att = 0, n_retired = 0;
   bool active = false;
   uint64_t ray = 0;
@@ -88,6 +88,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
     L.h = Sp->h0;
     L.k1_ok = false;
     L.k1_f32 = false;
+    L.k1_fast = false;
     L.b2 = 0.0;
     if (PREC == 1) {
       double c[4];
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
           X[4 * B + dst] = __longlong_as_double((long long)out_i);
           X[5 * B + dst] = __longlong_as_double((long long)(((uint64_t)L.attempts << 32) | L.accepted));
           X[6 * B + dst] = __longlong_as_double(
-              (long long)(((uint64_t)si << 2) | (L.k1_f32 ? 2u : 0u) | (L.k1_ok ? 1u : 0u)));
+              (long long)(((uint64_t)si << 3) | (L.k1_fast ? 4u : 0u) | (L.k1_f32 ? 2u : 0u) | (L.k1_ok ? 1u : 0u)));
           X[7 * B + dst] = L.b2;
         }
         __syncthreads();
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
           uint64_t aa = (uint64_t)__double_as_longlong(X[5 * B + tid]);
           L.attempts = (uint32_t)(aa >> 32); L.accepted = (uint32_t)aa;
           uint64_t sf = (uint64_t)__double_as_longlong(X[6 * B + tid]);
-          si = (uint32_t)(sf >> 2); L.k1_ok = (sf & 1u) != 0; L.k1_f32 = (sf & 2u) != 0;
+          si = (uint32_t)(sf >> 3); L.k1_ok = (sf & 1u) != 0; L.k1_f32 = (sf & 2u) != 0; L.k1_fast = (sf & 4u) != 0;
           L.b2 = X[7 * B + tid];
           if (!single) Sp = T.scenes + si;
           K.put(0, k1);
@@ -199,9 +200,13 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
       double p0[4];
       d4 v0;
       double lam0 = L.lambda;
-      bool use32 = false;
-      if (PREC == 1) use32 = __all(fp32_eligible(*Sp, L));   // over the lanes still running
-      int res = attempt<METRIC, PREC>(*Sp, L, K, acc, p0, v0, use32);
+      bool use32 = false, use_fast = false;
+      if (PREC == 1) {
+        // tiers per wavefront, over the lanes still running
+        use32 = __all(fp32_eligible(*Sp, L));
+        use_fast = __all(!(L.h < Sp->max_step && L.err_prev < 1e-6));
+      }
+      int res = attempt<METRIC, PREC>(*Sp, L, K, acc, p0, v0, use32, use_fast);
       L.attempts++;
       n_att++;
       if (acc) L.accepted++;
@@ -286,12 +291,13 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
   }
   // ---- tallies: one atomic per wavefront ----
   unsigned long long a = wave_sum(n_att), r = wave_sum(L.n_rhs), r32 = wave_sum(L.n_rhs32),
-                     f = wave_sum(L.n_full), ret = wave_sum(n_retired);
+                     rf = wave_sum(L.n_rhsf), f = wave_sum(L.n_full), ret = wave_sum(n_retired);
   if (lane == 0) {
     atomicAdd(&T.counters->retired, ret);
     atomicAdd(&T.counters->attempts, a);
     atomicAdd(&T.counters->rhs, r);
     atomicAdd(&T.counters->rhs32, r32);
+    atomicAdd(&T.counters->rhsf, rf);
     atomicAdd(&T.counters->attempts_full, f);
   }
 }

@@ -149,8 +149,9 @@ typedef struct gg_scene {
 typedef enum gg_precision {
   GG_PREC_FP64 = 0,       /* every operation in FP64, reference expression order */
   GG_PREC_MIXED = 1,      /* RHS stages in FP32 where the step-size controller is
-                             saturated and FP32 rounding cannot change a decision
-                             (whole wavefronts); exact FP64 elsewhere             */
+                             saturated and FP32 rounding cannot change a decision,
+                             fast FP64 where its rounding cannot change one, exact
+                             FP64 elsewhere (tiers chosen per wavefront)          */
   GG_PREC_FP64_FAST = 2   /* FP64 with reciprocal-based RHS algebra and FMA      */
 } gg_precision;
 
@@ -187,6 +188,7 @@ typedef struct gg_stats {
   uint64_t attempts;        /* sum of step() calls                            */
   uint64_t rhs_evals;       /* RHS evaluations actually made                  */
   uint64_t rhs_evals_fp32;  /* of which evaluated in FP32 (mixed)             */
+  uint64_t rhs_evals_fast;  /* of which evaluated in fast FP64 (mixed)        */
   uint64_t attempts_full;   /* attempts that reached the error estimate       */
   uint32_t grid_blocks;
   uint32_t block_threads;

@@ -3,15 +3,18 @@ replay and the launch-shape autotuner.  Every ray's arithmetic is the same
 in a batch as alone, so batch outputs equal single-scene outputs bit for bit
 in the FP64 precisions.  The mixed kernel takes FP32 steps only in wavefronts
 whose lanes are all FP32-eligible, so which eligible steps run in FP32 depends
-on the lane grouping: its final states agree across launches to FP32
-rounding, and that rounding can move a late decision of a rare ray (the step
-that crosses the escape radius: one attempt more or less, 1 ray in ~4e5
-observed), so classes must agree and attempt counts on all but 1e-4 of the
-rays, by at most one attempt."""
+on the lane grouping (and its fast-FP64 steps round h differently from exact
+FP64): final states agree across launches to ~1e-12 except where a photon-orbit
+approach amplifies the difference, and a rare ray can move a late decision (the
+step that crosses the escape radius: one attempt more or less, 1 ray in ~4e5
+observed).  So classes must agree, attempt counts on all but 1e-4 of the rays
+(by at most one attempt), and final states within 1e-6 (the mixed-mode bar)
+on all but 1e-4."""
 import numpy as np
 import pytest
 
 import gargantua_amd as gg
+from tests.parity import state_rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -28,8 +31,10 @@ def assert_same(a, b, precision, what=""):
     assert np.mean(~same) <= 1e-4, f"{what}attempts differ on {np.count_nonzero(~same)} rays"
     d = np.abs(a["attempts"].astype(np.int64) - b["attempts"].astype(np.int64))
     assert d.max() <= 1, f"{what}attempts differ by {d.max()}"
-    for k in ("r", "theta", "phi", "intensity"):
-        np.testing.assert_allclose(a[k][same], b[k][same], rtol=1e-7, atol=1e-12, err_msg=f"{what}{k}")
+    err = state_rel_err(a, b)[same]
+    ierr = np.abs(a["intensity"] - b["intensity"])[same] / np.maximum(np.abs(b["intensity"][same]), 1e-300)
+    ok = (err <= 1e-6) & ((ierr <= 1e-6) | (b["intensity"][same] == 0))
+    assert np.mean(~ok) <= 1e-4, f"{what}final states differ by more than 1e-6 on {np.count_nonzero(~ok)} rays"
 
 
 def _scenes():
