@@ -132,13 +132,17 @@ def bench_batch64(args):
     n = sum(s.n_rays for s in scenes)
     rend = gg.Renderer(0, stream=torch.cuda.current_stream().cuda_stream)
     outs = [_ptrs(_dev_outputs(torch, s.n_rays)) for s in scenes]
-    # tune on every 8th scene (same distribution, an eighth of the rays)
-    t0 = time.perf_counter()
-    best, best_ms = rend.autotune(scenes[::8], "mixed")
-    tune_s = time.perf_counter() - t0
-    _log(f"autotune: {best} ({best_ms:.1f} ms on {len(scenes[::8])} scenes) in {tune_s:.1f} s")
+    # tune each precision on every 8th scene (same distribution, an eighth of the rays)
+    tuned, tune_s = {}, {}
     res = {}
     for prec in ("mixed", "fp64"):
+        t0 = time.perf_counter()
+        tuned[prec] = rend.autotune(scenes[::8], prec)
+        tune_s[prec] = time.perf_counter() - t0
+        _log(f"autotune {prec}: {tuned[prec][0]} ({tuned[prec][1]:.1f} ms on {len(scenes[::8])} scenes) "
+             f"in {tune_s[prec]:.1f} s")
+        L = tuned[prec][0]
+        rend.set_launch(L["block_threads"], L["blocks_per_cu"], L["refill_interval"], 1, L["order"])
         for _ in range(args.warmup):
             rend.render_batch(scenes, prec, outputs=[dict(o) for o in outs])
         torch.cuda.synchronize()
@@ -157,8 +161,9 @@ def bench_batch64(args):
             "unit": "geodesics/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": 1e3 * n / res["mixed"]["value"], "higher_is_better": True, "dtype": "f32+f64",
             "config": {"workload": "batch of 64 scenes, generator seed 2026", "rays": n,
-                       "resolutions": [s.width for s in scenes], "autotuned_launch": best,
-                       "autotune_best_ms": best_ms, "autotune_seconds": tune_s},
+                       "resolutions": [s.width for s in scenes],
+                       "autotuned_launch": {p: tuned[p][0] for p in tuned},
+                       "autotune_best_ms": {p: tuned[p][1] for p in tuned}, "autotune_seconds": tune_s},
             "mixed": res["mixed"], "fp64": res["fp64"], "mixed_speedup": res["fp64"]["kernel_ms"] / res["mixed"]["kernel_ms"]}
 
 
