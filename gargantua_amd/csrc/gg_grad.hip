@@ -506,7 +506,7 @@ hipError_t launch_adjoint_cam(const GradLaunch &g, hipStream_t s) {
 hipError_t sort_rays_desc(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
                           uint64_t n, int key_bits, void *temp, size_t &temp_bytes, hipStream_t s) {
   return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out,
-                                                      (int)n, 0, key_bits, s);
+                                                      n, 0, key_bits, s);
 }
 
 hipError_t launch_grad_select(const GradSelect &p, hipStream_t s) {
