@@ -356,8 +356,16 @@ class Renderer:
                "gg_geodesic_acceleration_vjp")
         return {"pos_bar": pb, "vel_bar": vb, "spin_bar": sb}
 
-    def grad(self, gscene, target: np.ndarray, checkpoint_every: int = 32, want_dI: bool = False) -> dict:
-        """Loss, dL/d(spin, inclination, q, r_inner) and the image of a gg_grad_scene (see configs.grad_512_params)."""
+    def set_grad_memory(self, nbytes: int = 0) -> None:
+        """Device memory grad()'s automatic mode may use for step records (0: a quarter of the free memory)."""
+        _check(self.lib, self.lib.gg_set_grad_memory(self.ctx, int(nbytes)), "gg_set_grad_memory")
+
+    def grad(self, gscene, target: np.ndarray, checkpoint_every: int = 0, want_dI: bool = False) -> dict:
+        """Loss, dL/d(spin, inclination, q, r_inner) and the image of a gg_grad_scene (see configs.grad_512_params).
+
+        checkpoint_every = 0: the render records every accepted step and nothing is re-traced (stats mode 2)
+        when the records fit the memory of set_grad_memory; else a re-trace stores the disk rays' steps (mode 1)
+        or checkpoints every 32 accepted steps (mode 0).  The gradient is the same in every mode."""
         n = gscene.width * gscene.height
         target = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
         if target.size != n:

@@ -69,7 +69,8 @@ class gg_stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("rays", C.c_uint64),
                 ("attempts", C.c_uint64), ("rhs_evals", C.c_uint64), ("rhs_evals_fp32", C.c_uint64),
                 ("rhs_evals_fast", C.c_uint64),
-                ("attempts_full", C.c_uint64), ("grid_blocks", C.c_uint32), ("block_threads", C.c_uint32)]
+                ("attempts_full", C.c_uint64), ("grid_blocks", C.c_uint32), ("block_threads", C.c_uint32),
+                ("checkpoint_every", C.c_uint32), ("mode", C.c_uint32)]
 
 
 class gg_launch(C.Structure):
@@ -87,8 +88,9 @@ class gg_grad_scene(C.Structure):
 class gg_grad_stats(C.Structure):
     _fields_ = [("forward_ms", C.c_double), ("adjoint_ms", C.c_double), ("total_ms", C.c_double),
                 ("rays", C.c_uint64), ("disk_rays", C.c_uint64), ("steps", C.c_uint64), ("attempts", C.c_uint64),
-                ("checkpoints", C.c_uint64), ("checkpoint_bytes", C.c_uint64), ("segment_bytes", C.c_uint64), ("last_segment_bytes", C.c_uint64),
-                ("grid_blocks", C.c_uint32), ("block_threads", C.c_uint32)]
+                ("checkpoints", C.c_uint64), ("checkpoint_bytes", C.c_uint64), ("segment_bytes", C.c_uint64), ("record_bytes", C.c_uint64),
+                ("grid_blocks", C.c_uint32), ("block_threads", C.c_uint32),
+                ("checkpoint_every", C.c_uint32), ("mode", C.c_uint32)]
 
 
 # Entry points declared in include/gargantua.h: (name, restype, argtypes).
@@ -100,6 +102,7 @@ EXPORTS = [
     ("gg_context_create", C.c_int, [C.c_int, _P, C.POINTER(_P)]),
     ("gg_context_destroy", C.c_int, [_P]),
     ("gg_set_launch", C.c_int, [_P, C.POINTER(gg_launch)]),
+    ("gg_set_grad_memory", C.c_int, [_P, C.c_uint64]),
     ("gg_get_launch", C.c_int, [_P, C.POINTER(gg_launch)]),
     ("gg_scene_rays", C.c_int, [C.POINTER(gg_scene), C.POINTER(C.c_uint64)]),
     ("gg_render", C.c_int, [_P, C.POINTER(gg_scene), C.c_int, C.POINTER(gg_outputs), C.POINTER(gg_stats)]),

@@ -98,8 +98,19 @@ struct gg_context {
   std::vector<DevBuf> batch_stage;
   // gg_grad: cls, intensity, accepted, target, resid, dI, list, counters,
   // results, scene, checkpoint pool, segment buffers, keys, sorted list,
-  // sorted keys, ray records, last-segment records, sort scratch
-  DevBuf grad[18];
+  // sorted keys, ray records, last-segment records, sort scratch,
+  DevBuf grad[21];
+  // record pool, last chunk per pixel and final states of the recording
+  // render (grad[18..20]); chunks the last recording needed
+  uint64_t grad_chunks = 0;
+  uint64_t grad_budget = 0;   // gg_set_grad_memory (0: a quarter of the free memory)
+  // the recording render gg_grad asks render_impl for (pool null: none)
+  struct {
+    double *pool;
+    unsigned long long cap;
+    unsigned long long *cursor;
+    uint32_t *last;
+  } rec_req{};
 };
 
 namespace {
@@ -428,6 +439,12 @@ int gg_set_launch(gg_context *ctx, const gg_launch *l) {
   return GG_OK;
 }
 
+int gg_set_grad_memory(gg_context *ctx, uint64_t bytes) {
+  if (!ctx) return fail(GG_ERR_INVALID_ARG, "NULL context");
+  ctx->grad_budget = bytes;
+  return GG_OK;
+}
+
 int gg_get_launch(gg_context *ctx, gg_launch *l) {
   if (!ctx || !l) return fail(GG_ERR_INVALID_ARG, "NULL argument");
   *l = ctx->launch;
@@ -547,6 +564,10 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
   T.counters = (Counters *)ctx->counters.p;
   T.refill_interval = ctx->launch.refill_interval;
   T.order = (learn_order && ctx->order_sig == sig) ? (const uint32_t *)ctx->order.p : nullptr;
+  T.rec_pool = ctx->rec_req.pool;
+  T.rec_cap = ctx->rec_req.cap;
+  T.rec_cursor = ctx->rec_req.cursor;
+  T.rec_last = ctx->rec_req.last;
   uint32_t grid = grid_for(ctx, total);
   uint32_t block = ctx->launch.block_threads;
   const bool use_graph = (n_scenes > 1) && ctx->launch.graph == 1 && total > 0;
@@ -841,14 +862,12 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
             double *loss, double grad[4], double *intensity, double *d_intensity, gg_grad_stats *stats) {
   auto t_start = std::chrono::steady_clock::now();
   if (!ctx || !scene || !target) return fail(GG_ERR_INVALID_ARG, "NULL context, scene or target");
-  if (checkpoint_every < 1 || checkpoint_every > 65536)
-    return fail(GG_ERR_INVALID_ARG, "checkpoint_every must be in [1, 65536]");
+  if (checkpoint_every > 65536) return fail(GG_ERR_INVALID_ARG, "checkpoint_every must be in [0, 65536]");
   gg_scene sc;
   int rc = gg_grad_scene_to_scene(scene, &sc);
   if (rc) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
   const uint64_t n = (uint64_t)scene->width * scene->height;
-  const uint32_t K = checkpoint_every;
   DevBuf *B = ctx->grad;
   if ((rc = B[0].ensure(n)) || (rc = B[1].ensure(8 * n)) || (rc = B[2].ensure(4 * n)) ||
       (rc = B[3].ensure(8 * n)) || (rc = B[4].ensure(8 * n)) || (rc = B[5].ensure(32 * n)) ||
@@ -864,32 +883,97 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
   GradCounters *d_cnt = (GradCounters *)B[7].p;
   hipStream_t s = ctx->stream;
 
-  // forward render (FP64): classes, intensities, accepted counts
-  gg_outputs o;
-  std::memset(&o, 0, sizeof(o));
-  o.cls = d_cls;
-  o.intensity = d_int;
-  o.accepted = d_acc;
+  // Modes.  2 (checkpoint_every = 0, when the records fit): the render
+  // records every accepted step and the reverse sweep reads those records.
+  // 1: a re-trace of the disk rays stores every step record (checkpoint_every
+  // at least the longest ray's accepted count, or automatic when the render's
+  // records did not fit but the disk rays' do).  0: the re-trace stores a
+  // checkpoint every K steps and the reverse sweep recomputes segments.
+  // Budget of the automatic choice: gg_set_grad_memory, else a quarter of the
+  // free device memory, counting the record pool this context already holds.
+  const bool automatic = (checkpoint_every == 0);
+  uint32_t K = automatic ? 32 : checkpoint_every;
+  size_t free_b = 0, total_b = 0;
+  if (automatic && !ctx->grad_budget) HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+  const uint64_t budget = ctx->grad_budget ? ctx->grad_budget : (uint64_t)(free_b + B[18].cap) / 4;
+  const uint64_t chunk_bytes = 8ull * GRAD_CHUNK_DOUBLES;
+  int mode = 0;
+  uint64_t pool_chunks = 0;
   gg_stats rs;
-  if ((rc = render_impl(ctx, &sc, 1, GG_PREC_FP64, &o, nullptr, 0, &rs))) return rc;
-
-  DevScene D;
-  build_dev_scene(sc, D);
-  HIP_TRY(hipMemcpyAsync(B[9].p, &D, sizeof(D), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(d_target, target, 8 * n, hipMemcpyDefault, s));
-  HIP_TRY(launch_grad_reset(d_cnt, s));
-  GradSelect sel{n, d_cls, d_int, d_target, d_acc, K, d_resid, d_dI, d_list_in, d_keys_in, d_cnt};
-  HIP_TRY(launch_grad_select(sel, s));
   GradCounters cnt;
-  HIP_TRY(hipMemcpyAsync(&cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  const uint64_t n_list = cnt.n_list, chk_total = cnt.chk_total, tail_total = cnt.tail_total;
+  for (int pass = 0;; ++pass) {
+    bool record = false;
+    if (automatic) {
+      // pool size: the last recording's need with a margin, else 8 chunks
+      // (128 steps) per ray; at least 2 chunks per ray, within the budget
+      uint64_t want = ctx->grad_chunks ? ctx->grad_chunks + ctx->grad_chunks / 4 + 64 : 8 * n;
+      if (want < 2 * n) want = 2 * n;
+      if (want > budget / chunk_bytes) want = budget / chunk_bytes;
+      if (want > 0xFFFFFFFEull) want = 0xFFFFFFFEull;
+      if (want >= n) {
+        if ((rc = B[18].ensure(want * chunk_bytes)) || (rc = B[19].ensure(4 * n)) || (rc = B[20].ensure(72 * n)))
+          return rc;
+        pool_chunks = want;
+        record = true;
+      }
+    }
+    HIP_TRY(launch_grad_reset(d_cnt, s));
+    // the FP64 render: classes, intensities, accepted counts (and, recording,
+    // the step records and final states)
+    gg_outputs o;
+    std::memset(&o, 0, sizeof(o));
+    o.cls = d_cls;
+    o.intensity = d_int;
+    o.accepted = d_acc;
+    if (record) {
+      o.state = (double *)B[20].p;
+      ctx->rec_req = {(double *)B[18].p, pool_chunks, &d_cnt->rec_cursor, (uint32_t *)B[19].p};
+    }
+    rc = render_impl(ctx, &sc, 1, GG_PREC_FP64, &o, nullptr, 0, &rs);
+    ctx->rec_req = {};
+    if (rc) return rc;
+
+    DevScene D;
+    build_dev_scene(sc, D);
+    HIP_TRY(hipMemcpyAsync(B[9].p, &D, sizeof(D), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_target, target, 8 * n, hipMemcpyDefault, s));
+    GradSelect sel{n, d_cls, d_int, d_target, d_acc, K, d_resid, d_dI, d_list_in, d_keys_in, d_cnt};
+    HIP_TRY(launch_grad_select(sel, s));
+    HIP_TRY(hipMemcpyAsync(&cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (!record) break;
+    if (cnt.rec_cursor <= pool_chunks) {
+      ctx->grad_chunks = cnt.rec_cursor;
+      mode = 2;
+      break;
+    }
+    // the pool was too small: once more with the measured size if it fits
+    ctx->grad_chunks = cnt.rec_cursor;
+    if (pass > 0 || cnt.rec_cursor * chunk_bytes > budget) {
+      pool_chunks = 0;
+      break;
+    }
+  }
+  const uint64_t n_list = cnt.n_list;
+  uint64_t chk_total = cnt.chk_total, tail_total = cnt.tail_total;
+  if (mode == 2) {
+    chk_total = 0;
+    tail_total = 0;
+  } else if (K >= cnt.max_acc) {
+    mode = 1;
+  } else if (automatic && 8 * GRAD_STEP_REC * cnt.acc_total + 72 * n_list <= budget) {
+    K = (uint32_t)cnt.max_acc;
+    chk_total = n_list;
+    tail_total = cnt.acc_total;
+    mode = 1;
+  }
 
   // launch shapes: forward 256-thread blocks, 2 per CU; reverse 64-thread
-  // blocks (its LDS and registers allow one wavefront per SIMD), 4 per CU;
-  // never more wavefronts than disk rays
+  // blocks, 4 per CU when it recomputes segments (its LDS and registers allow
+  // one wavefront per SIMD), else 8 (two wavefronts per SIMD); never more
+  // wavefronts than disk rays
   const uint64_t waves = (n_list + 63) / 64;
-  uint64_t grid_f = (uint64_t)ctx->num_cus * 2, grid_r = (uint64_t)ctx->num_cus * 4;
+  uint64_t grid_f = (uint64_t)ctx->num_cus * 2, grid_r = (uint64_t)ctx->num_cus * (mode == 0 ? 4 : 8);
   if ((waves + 3) / 4 < grid_f) grid_f = (waves + 3) / 4;
   if (waves < grid_r) grid_r = waves;
   if (grid_f < 1) grid_f = 1;
@@ -897,9 +981,9 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
   const uint64_t n_threads = grid_r * 64;
   const uint64_t chk_bytes = 72 * (chk_total ? chk_total : 1);
   const uint64_t tail_bytes = 8 * GRAD_STEP_REC * (tail_total ? tail_total : 1);
-  const uint64_t seg_bytes = 8 * GRAD_STEP_REC * (uint64_t)K * n_threads;
-  if ((rc = B[10].ensure(chk_bytes)) || (rc = B[11].ensure(seg_bytes)) || (rc = B[15].ensure(8 * GRAD_REC * (n_list + 1))) ||
-      (rc = B[16].ensure(tail_bytes)))
+  const uint64_t seg_bytes = mode == 0 ? 8 * GRAD_STEP_REC * (uint64_t)K * n_threads : 0;
+  if ((rc = B[10].ensure(chk_bytes)) || (rc = B[11].ensure(seg_bytes ? seg_bytes : 1)) ||
+      (rc = B[15].ensure(8 * GRAD_REC * (n_list + 1))) || (rc = B[16].ensure(tail_bytes)))
     return rc;
 
   float adj_ms = 0.0f;
@@ -927,10 +1011,18 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
     G.r_cam = scene->r_camera;
     G.dI = d_dI;
     G.cnt = d_cnt;
+    G.pool = mode == 2 ? (const double *)B[18].p : nullptr;
+    G.pool_chunks = mode == 2 ? pool_chunks : 0;
+    G.last_chunk = mode == 2 ? (const uint32_t *)B[19].p : nullptr;
+    G.final_state = mode == 2 ? (const double *)B[20].p : nullptr;
+    G.n_pix = n;
     HIP_TRY(hipEventRecord(ctx->ev0, s));
     HIP_TRY(sort_rays_desc(d_keys_in, d_keys, d_list_in, d_list, n_list, key_bits, B[17].p, temp_bytes, s));
-    HIP_TRY(launch_adjoint_fwd(G, (uint32_t)grid_f, 256, s));
-    HIP_TRY(launch_adjoint_rev(G, (uint32_t)grid_r, 64, s));
+    if (mode == 2)
+      HIP_TRY(launch_adjoint_seed(G, s));
+    else
+      HIP_TRY(launch_adjoint_fwd(G, (uint32_t)grid_f, 256, s));
+    HIP_TRY(launch_adjoint_rev(G, mode, (uint32_t)grid_r, 64, s));
     HIP_TRY(launch_adjoint_cam(G, s));
     HIP_TRY(hipEventRecord(ctx->ev1, s));
   }
@@ -942,11 +1034,15 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
   if (d_intensity) HIP_TRY(hipMemcpyAsync(d_intensity, d_dI, 32 * n, hipMemcpyDefault, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (n_list > 0) HIP_TRY(hipEventElapsedTime(&adj_ms, ctx->ev0, ctx->ev1));
-  if (cnt.bad != 0 || cnt.done != n_list || cnt.chk_cursor != chk_total || cnt.tail_cursor != tail_total)
+  // every disk ray finished, and every accepted step of each was
+  // back-propagated exactly once
+  if (cnt.bad != 0 || cnt.done != n_list || cnt.steps != cnt.acc_total || cnt.chk_cursor != chk_total ||
+      cnt.tail_cursor != tail_total)
     return fail(GG_ERR_INTERNAL,
-                "adjoint: %llu of %llu rays done, %llu left the rendered path, %llu/%llu checkpoints, %llu/%llu "
-                "step records",
+                "adjoint: %llu of %llu rays done, %llu left the rendered path, %llu of %llu steps, %llu/%llu "
+                "checkpoints, %llu/%llu step records",
                 (unsigned long long)cnt.done, (unsigned long long)n_list, (unsigned long long)cnt.bad,
+                (unsigned long long)cnt.steps, (unsigned long long)cnt.acc_total,
                 (unsigned long long)cnt.chk_cursor, (unsigned long long)chk_total,
                 (unsigned long long)cnt.tail_cursor, (unsigned long long)tail_total);
   if (loss) *loss = out5[0];
@@ -963,9 +1059,11 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
     stats->checkpoints = chk_total;
     stats->checkpoint_bytes = 72 * chk_total;
     stats->segment_bytes = seg_bytes;
-    stats->last_segment_bytes = 8 * GRAD_STEP_REC * tail_total;
+    stats->record_bytes = mode == 2 ? chunk_bytes * cnt.rec_cursor : 8 * GRAD_STEP_REC * tail_total;
     stats->grid_blocks = (uint32_t)grid_r;
     stats->block_threads = 64;
+    stats->checkpoint_every = mode == 2 ? 0 : K;
+    stats->mode = (uint32_t)mode;
   }
   return GG_OK;
 }

@@ -21,6 +21,10 @@
 //       each earlier segment it re-runs that segment from its checkpoint into
 //       a per-lane segment buffer (rejections included, so h repeats exactly,
 //       and the end state must equal the next checkpoint bit for bit).
+//       When K is at least every ray's accepted count there is one segment
+//       per ray, the re-trace stores every step record, and
+//       adjoint_rev_stored_kernel (the same sweep without recomputation,
+//       two wavefronts per SIMD) reads them.
 //   adjoint_cam_kernel  adds the camera ray's dependence on (a, i) and
 //       writes dI/d(a, i, q, r_in) per pixel.
 #include <hip/hip_runtime.h>
@@ -156,6 +160,40 @@ __device__ inline void mark_bad(const GradLaunch &G, uint64_t q, uint32_t pix, u
   ++n_bad;
 }
 
+// The disk crossing of a ray's last accepted step (p0, v0) -> (p1, v1),
+// the trace kernel's test, and the forward-mode derivatives of its intensity
+// over its 12 inputs: the reverse sweep's seeds in the ray record.  False
+// when the step does not cross the disk (the ray is then not the rendered one).
+__device__ inline bool crossing_seeds(const GradLaunch &G, const DevScene &S, double *rec, double p0r, double p0th,
+                                      double v0t, double v0p, double p1r, double p1th, double v1t, double v1p) {
+#pragma clang fp contract(off)
+  const bool side0 = p0th < HALF_PI, side1 = p1th < HALF_PI;
+  double rc = 0.0;
+  if (side0 != side1) {
+    double s = (HALF_PI - p0th) / (p1th - p0th);
+    rc = p0r + s * (p1r - p0r);
+  }
+  const double w = S.edge_width;
+  const bool hit = (side0 != side1) &&
+                   ((w > 0.0) ? (rc <= S.r_out + 8.0 * w && rc > 0.0) : (rc >= S.r_in && rc <= S.r_out));
+  if (!hit) return false;
+  const CrossVals x{S.spin, S.q, S.r_in, S.sqrt_neg_gtt_obs, p0r, p0th, p1r, p1th, v0t, v0p, v1t, v1p};
+  double d[CX_N];
+#pragma unroll 1
+  for (int k = 0; k < CX_N; ++k) d[k] = crossing_intensity_d(S.mass, S.r_out, w, x, k).d;
+  const Tn so_a = kerr_sqrt_obs_d(S.mass, Tn{S.spin, 1.0}, G.r_cam, dconst(G.incl));
+  const Tn so_i = kerr_sqrt_obs_d(S.mass, dconst(S.spin), G.r_cam, Tn{G.incl, 1.0});
+  rec[R_PBR] = d[CX_R1];
+  rec[R_PBTH] = d[CX_TH1];
+  rec[R_VB0] = d[CX_V1T]; rec[R_VB1] = 0.0; rec[R_VB2] = 0.0; rec[R_VB3] = d[CX_V1P];
+  rec[R_GA] = d[CX_A] + d[CX_SOBS] * so_a.d;
+  rec[R_GI] = d[CX_SOBS] * so_i.d;
+  rec[R_GQ] = d[CX_Q];
+  rec[R_GRIN] = d[CX_RIN];
+  rec[R_XR0] = d[CX_R0]; rec[R_XTH0] = d[CX_TH0]; rec[R_XV0T] = d[CX_V0T]; rec[R_XV0P] = d[CX_V0P];
+  return true;
+}
+
 __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
 #pragma clang fp contract(off)
   extern __shared__ double lds_k[];
@@ -217,35 +255,11 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
     }
     // the disk crossing of the last accepted step (same test as the trace kernel)
     active = false;
-    const bool side0 = p0[2] < HALF_PI, side1 = L.pos[2] < HALF_PI;
-    double rc = 0.0;
-    if (side0 != side1) {
-      double s = (HALF_PI - p0[2]) / (L.pos[2] - p0[2]);
-      rc = p0[1] + s * (L.pos[1] - p0[1]);
-    }
-    const double w = S.edge_width;
-    const bool hit = (side0 != side1) &&
-                     ((w > 0.0) ? (rc <= S.r_out + 8.0 * w && rc > 0.0) : (rc >= S.r_in && rc <= S.r_out));
-    if (!hit) {
+    double *rec = G.rec + REC * q;
+    if (!crossing_seeds(G, S, rec, p0[1], p0[2], v0.x, v0.w, L.pos[1], L.pos[2], L.vel.x, L.vel.w)) {
       mark_bad(G, q, pix, n_bad);
       continue;
     }
-    const CrossVals x{S.spin, S.q, S.r_in, S.sqrt_neg_gtt_obs, p0[1], p0[2], L.pos[1], L.pos[2],
-                      v0.x, v0.w, L.vel.x, L.vel.w};
-    double *rec = G.rec + REC * q;
-    double d[CX_N];
-#pragma unroll 1
-    for (int k = 0; k < CX_N; ++k) d[k] = crossing_intensity_d(S.mass, S.r_out, w, x, k).d;
-    const Tn so_a = kerr_sqrt_obs_d(S.mass, Tn{S.spin, 1.0}, G.r_cam, dconst(G.incl));
-    const Tn so_i = kerr_sqrt_obs_d(S.mass, dconst(S.spin), G.r_cam, Tn{G.incl, 1.0});
-    rec[R_PBR] = d[CX_R1];
-    rec[R_PBTH] = d[CX_TH1];
-    rec[R_VB0] = d[CX_V1T]; rec[R_VB1] = 0.0; rec[R_VB2] = 0.0; rec[R_VB3] = d[CX_V1P];
-    rec[R_GA] = d[CX_A] + d[CX_SOBS] * so_a.d;
-    rec[R_GI] = d[CX_SOBS] * so_i.d;
-    rec[R_GQ] = d[CX_Q];
-    rec[R_GRIN] = d[CX_RIN];
-    rec[R_XR0] = d[CX_R0]; rec[R_XTH0] = d[CX_TH0]; rec[R_XV0T] = d[CX_V0T]; rec[R_XV0P] = d[CX_V0P];
     rec[R_CHK] = as_f64(chk_base);
     rec[R_TAIL] = as_f64(tail_base);
   }
@@ -257,15 +271,23 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
   }
 }
 
-__global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
+// The reverse sweep.  MODE 0: some rays have more than one segment, so the
+// earlier segments are recomputed from their checkpoints.  MODE 1: every
+// step record of every ray is in the last-segment pool (one segment per
+// ray).  MODE 2: the records are the recording render's chunks, walked back
+// through their back-pointers.  Modes 1 and 2 only read records (fewer live
+// registers: two wavefronts per SIMD).
+template <int MODE>
+__device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
 #pragma clang fp contract(off)
+  constexpr bool RECOMPUTE = (MODE == 0);
   extern __shared__ double lds_k[];
   const KStore K{lds_k + threadIdx.x, blockDim.x};
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const DevScene &S = *G.scene;
   const uint32_t KK = G.K;
-  double *const seg = G.seg + gtid * (uint64_t)KK * STEP_REC;
+  double *const seg = RECOMPUTE ? G.seg + gtid * (uint64_t)KK * STEP_REC : nullptr;
 
   Lane L;
   L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0;
@@ -273,7 +295,7 @@ __global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
   int phase = IDLE;
   bool exhausted = false, first = false;
   uint64_t q = 0, chk_base = 0;
-  uint32_t pix = 0, c = 0, j = 0, att = 0;
+  uint32_t pix = 0, c = 0, j = 0, att = 0, chunk = 0;
   const double *src = nullptr;
   double pb_r = 0, pb_th = 0, ga = 0;
   d4 vb{0, 0, 0, 0};
@@ -289,11 +311,16 @@ __global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
       if (!(rec[R_PBR] == rec[R_PBR])) {   // flagged by the forward pass
         phase = IDLE;
       } else {
-        const uint32_t nchk = (acc_target + KK - 1) / KK;
+        // without recomputation the whole ray is one segment
+        const uint32_t nchk = RECOMPUTE ? (acc_target + KK - 1) / KK : 1;
         c = nchk - 1;
         j = acc_target - c * KK - 1;
-        chk_base = as_u64(rec[R_CHK]);
-        src = G.tail + STEP_REC * as_u64(rec[R_TAIL]);
+        if (MODE == 2) {
+          chunk = (uint32_t)as_u64(rec[R_CHK]);
+        } else {
+          chk_base = as_u64(rec[R_CHK]);
+          src = G.tail + STEP_REC * as_u64(rec[R_TAIL]);
+        }
         pb_r = rec[R_PBR]; pb_th = rec[R_PBTH];
         vb = d4{rec[R_VB0], rec[R_VB1], rec[R_VB2], rec[R_VB3]};
         ga = rec[R_GA];
@@ -304,7 +331,7 @@ __global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
     if (exhausted && __all(phase == IDLE)) break;
 
     // ---- RECOMP: one attempt of the segment before ----
-    if (phase == RECOMP) {
+    if (RECOMPUTE && phase == RECOMP) {
       bool acc;
       double p0[4];
       d4 v0;
@@ -336,7 +363,9 @@ __global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
 
     // ---- REV: one accepted step backwards ----
     if (phase == REV) {
-      step_adjoint(K, src + STEP_REC * (uint64_t)j, S.mass, S.spin, pb_r, pb_th, vb, ga);
+      const double *sp = MODE == 2 ? G.pool + (uint64_t)chunk * GRAD_CHUNK_DOUBLES + 1 + STEP_REC * (j % GRAD_CHUNK)
+                                   : src + STEP_REC * (uint64_t)j;
+      step_adjoint(K, sp, S.mass, S.spin, pb_r, pb_th, vb, ga);
       ++n_adj;
       if (first) {
         const double *rec = G.rec + REC * q;
@@ -344,8 +373,9 @@ __global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
         first = false;
       }
       if (j > 0) {
+        if (MODE == 2 && j % GRAD_CHUNK == 0) chunk = (uint32_t)as_u64(G.pool[(uint64_t)chunk * GRAD_CHUNK_DOUBLES]);
         --j;
-      } else if (c > 0) {
+      } else if (RECOMPUTE && c > 0) {
         --c;
         get_state(G.chk + 9 * (chk_base + c), L.pos, L.vel, L.h);
         L.k1_ok = false; L.k1_f32 = false; L.k1_fast = false;
@@ -373,6 +403,36 @@ __global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) {
   }
 }
 
+__global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) { adjoint_rev_body<0>(G); }
+__global__ __launch_bounds__(64, 2) void adjoint_rev_stored_kernel(GradLaunch G) { adjoint_rev_body<1>(G); }
+__global__ __launch_bounds__(64, 2) void adjoint_rev_chunked_kernel(GradLaunch G) { adjoint_rev_body<2>(G); }
+
+// Seeds of the recorded rays (the recording render replaces the re-trace):
+// the crossing step is the ray's last record, its end state the render's
+// final state.
+__global__ __launch_bounds__(256) void adjoint_seed_kernel(GradLaunch G) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long n_bad = 0;
+  if (q < G.n_list) {
+    const uint32_t pix = G.list[q];
+    const uint32_t acc = G.keys[q];
+    const uint32_t chunk = G.last_chunk[pix];
+    double *rec = G.rec + REC * q;
+    bool ok = acc > 0 && chunk != 0xFFFFFFFFu && chunk < G.pool_chunks;
+    if (ok) {
+      const double *r = G.pool + (uint64_t)chunk * GRAD_CHUNK_DOUBLES + 1 + STEP_REC * ((acc - 1) % GRAD_CHUNK);
+      const uint64_t n = G.n_pix;
+      const double *fs = G.final_state;
+      ok = crossing_seeds(G, *G.scene, rec, r[1], r[2], r[4], r[7], fs[n + pix], fs[2 * n + pix], fs[4 * n + pix],
+                          fs[7 * n + pix]);
+      rec[R_CHK] = as_f64(chunk);
+    }
+    if (!ok) mark_bad(G, q, pix, n_bad);
+  }
+  const unsigned long long bd = wave_sum(n_bad);
+  if ((threadIdx.x & 63) == 0 && bd) atomicAdd(&G.cnt->bad, bd);
+}
+
 // The camera ray's dependence on (a, i): pos0 = (0, r_cam, i, 0), vel0 from
 // Camera::pixel_direction and GeodesicRay::from_camera; dI per pixel.
 __global__ __launch_bounds__(256) void adjoint_cam_kernel(GradLaunch G) {
@@ -397,8 +457,9 @@ __global__ __launch_bounds__(256) void adjoint_cam_kernel(GradLaunch G) {
 }
 
 // Residuals, zeroed gradients and the list of disk rays with their
-// accepted counts (sort keys), checkpoint and last-segment totals.  One
-// atomic per wavefront for the list and the totals.
+// accepted counts (sort keys), checkpoint and last-segment totals, the total
+// and the largest accepted count.  One atomic per wavefront for the list and
+// each total.
 __global__ void select_kernel(GradSelect P) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63;
@@ -416,7 +477,9 @@ __global__ void select_kernel(GradSelect P) {
     }
   }
   const unsigned long long m = __ballot(disk);
-  const unsigned long long tot = wave_sum(nchk), tl = wave_sum(last);
+  const unsigned long long tot = wave_sum(nchk), tl = wave_sum(last), all = wave_sum(acc);
+  uint32_t mx = acc;
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off, 64));
   if (m) {
     const int leader = __ffsll((long long)m) - 1;
     unsigned long long base = 0;
@@ -424,6 +487,8 @@ __global__ void select_kernel(GradSelect P) {
       base = atomicAdd(&P.cnt->n_list, (unsigned long long)__popcll(m));
       atomicAdd(&P.cnt->chk_total, tot);
       atomicAdd(&P.cnt->tail_total, tl);
+      atomicAdd(&P.cnt->acc_total, all);
+      atomicMax(&P.cnt->max_acc, (unsigned long long)mx);
     }
     base = __shfl(base, leader, 64);
     if (disk) {
@@ -491,8 +556,19 @@ hipError_t launch_adjoint_fwd(const GradLaunch &g, uint32_t grid, uint32_t block
   return hipGetLastError();
 }
 
-hipError_t launch_adjoint_rev(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s) {
-  hipLaunchKernelGGL(adjoint_rev_kernel, dim3(grid), dim3(block), adjoint_lds_bytes(block), s, g);
+hipError_t launch_adjoint_rev(const GradLaunch &g, int mode, uint32_t grid, uint32_t block, hipStream_t s) {
+  if (mode == 0)
+    hipLaunchKernelGGL(adjoint_rev_kernel, dim3(grid), dim3(block), adjoint_lds_bytes(block), s, g);
+  else if (mode == 1)
+    hipLaunchKernelGGL(adjoint_rev_stored_kernel, dim3(grid), dim3(block), adjoint_lds_bytes(block), s, g);
+  else
+    hipLaunchKernelGGL(adjoint_rev_chunked_kernel, dim3(grid), dim3(block), adjoint_lds_bytes(block), s, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_seed(const GradLaunch &g, hipStream_t s) {
+  if (g.n_list == 0) return hipSuccess;
+  hipLaunchKernelGGL(adjoint_seed_kernel, dim3((uint32_t)((g.n_list + 255) / 256)), dim3(256), 0, s, g);
   return hipGetLastError();
 }
 

@@ -38,6 +38,14 @@ struct TraceLaunch {
   Counters *counters;
   uint32_t refill_interval;
   const uint32_t *order;      // camera renders: queue position -> global ray, or null (ray order)
+  // Recording render of gg_grad (Kerr thin disk, FP64, one scene), else
+  // null: every accepted step's record in chunks (GRAD_CHUNK_DOUBLES each:
+  // the index of the ray's previous chunk, then GRAD_CHUNK step records);
+  // rec_last[output index] = the ray's last chunk.
+  double *rec_pool;
+  unsigned long long rec_cap;       // chunks in the pool
+  unsigned long long *rec_cursor;   // chunks claimed; beyond rec_cap nothing is written
+  uint32_t *rec_last;
 };
 
 // Tallies of gg_grad (zeroed by launch_grad_reset).
@@ -54,7 +62,10 @@ struct GradCounters {
   unsigned long long n_list;       // disk rays found by select
   unsigned long long chk_total;    // checkpoints they need
   unsigned long long tail_total;   // step records of their last segments
-  unsigned long long pad[4];
+  unsigned long long acc_total;    // accepted steps of all disk rays
+  unsigned long long max_acc;      // the largest accepted count
+  unsigned long long rec_cursor;   // chunks claimed by the recording render
+  unsigned long long pad[1];
 };
 
 struct GradSelect {
@@ -84,14 +95,23 @@ struct GradLaunch {
   double incl, r_cam;
   double *dI;                      // [pixels][4]
   GradCounters *cnt;
+  // records of the recording render (reverse mode 2), else null
+  const double *pool;              // chunks, GRAD_CHUNK_DOUBLES each
+  unsigned long long pool_chunks;
+  const uint32_t *last_chunk;      // per pixel
+  const double *final_state;       // the render's final states: 9 planes of n_pix doubles
+  uint64_t n_pix;
 };
 
 constexpr uint32_t GRAD_REC = 16;       // doubles per ray record
 constexpr uint32_t GRAD_STEP_REC = 29;  // doubles per step record: pos, vel, h, v_2..v_6
+constexpr uint32_t GRAD_CHUNK = 16;     // step records per chunk of the recording render
+constexpr uint32_t GRAD_CHUNK_DOUBLES = 1 + GRAD_CHUNK * GRAD_STEP_REC;
 size_t adjoint_lds_bytes(uint32_t block);
 size_t adjoint_fwd_lds_bytes(uint32_t block);
 hipError_t launch_adjoint_fwd(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s);
-hipError_t launch_adjoint_rev(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s);
+hipError_t launch_adjoint_rev(const GradLaunch &g, int mode, uint32_t grid, uint32_t block, hipStream_t s);
+hipError_t launch_adjoint_seed(const GradLaunch &g, hipStream_t s);
 hipError_t launch_adjoint_cam(const GradLaunch &g, hipStream_t s);
 hipError_t sort_rays_desc(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
                           uint64_t n, int key_bits, void *temp, size_t &temp_bytes, hipStream_t s);

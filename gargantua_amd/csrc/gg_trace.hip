@@ -38,7 +38,11 @@ __device__ inline uint32_t find_scene(const DevScene *scenes, uint32_t n, uint64
   return lo;
 }
 
-// Persistent trace kernel.  SRC 0: rays from the camera; 1: states_in.
+// Persistent trace kernel.  SRC 0: rays from the camera; 1: states_in;
+// 2: rays from the camera, recording every accepted step (the record the
+// adjoint reads: start state, h, stage velocities) into chunks of
+// GRAD_CHUNK records, each chunk headed by the index of the ray's previous
+// chunk, and each ray's last chunk into rec_last.
 // BATCH 0: one scene, its constants wavefront-uniform (scalar registers);
 // BATCH 1: each lane looks up the scene of its ray.
 template <int METRIC, int DISK, int PREC, int SRC, int BATCH>
@@ -61,6 +65,10 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
   uint32_t si = 0;        // scene of this lane's ray
   uint64_t out_i = 0;     // output slot of this lane's ray
   bool exhausted = false;
+  constexpr bool record = (SRC == 2);
+  constexpr uint32_t NO_CHUNK = 0xFFFFFFFFu;
+  uint32_t r_chunk = NO_CHUNK;   // record chunk holding step `accepted`
+  bool r_need = false;           // a new chunk is due before the next attempt
   // Scene constants are read through the pointer where they are used, so
   // the compiler reloads them (scalar loads for a single scene) instead of
   // holding ~75 doubles live across the persistent loop.
@@ -68,11 +76,11 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
 
   // Start ray `ray` in this lane.
   auto init_ray = [&](uint64_t rid) {
-    ray = (SRC == 0 && T.order) ? (uint64_t)T.order[rid] : rid;
+    ray = (SRC != 1 && T.order) ? (uint64_t)T.order[rid] : rid;
     si = single ? 0u : find_scene(T.scenes, T.n_scenes, ray);
     if (!single) Sp = T.scenes + si;
     uint64_t local = ray - Sp->ray_begin;
-    if (SRC == 0) {
+    if (SRC != 1) {
       uint32_t px, py;
       ray_pixel(*Sp, local, px, py);
       out_i = (uint64_t)py * Sp->sub_w + px;
@@ -98,6 +106,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
     L.attempts = 0; L.accepted = 0;
     L.intensity = 0.0; L.transmission = 1.0;
     L.err_prev = 1.0;
+    if (record) { r_chunk = NO_CHUNK; r_need = true; }
     active = true;
   };
 
@@ -155,6 +164,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
           X[6 * B + dst] = __longlong_as_double(
               (long long)(((uint64_t)si << 3) | (L.k1_fast ? 4u : 0u) | (L.k1_f32 ? 2u : 0u) | (L.k1_ok ? 1u : 0u)));
           X[7 * B + dst] = L.b2;
+          if (record) X[8 * B + dst] = __longlong_as_double((long long)(((uint64_t)r_need << 32) | r_chunk));
         }
         __syncthreads();
         if (now_active) {
@@ -166,6 +176,11 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
           uint64_t sf = (uint64_t)__double_as_longlong(X[6 * B + tid]);
           si = (uint32_t)(sf >> 3); L.k1_ok = (sf & 1u) != 0; L.k1_f32 = (sf & 2u) != 0; L.k1_fast = (sf & 4u) != 0;
           L.b2 = X[7 * B + tid];
+          if (record) {
+            const uint64_t cw = (uint64_t)__double_as_longlong(X[8 * B + tid]);
+            r_chunk = (uint32_t)cw;
+            r_need = (cw >> 32) != 0;
+          }
           if (!single) Sp = T.scenes + si;
           K.put(0, k1);
         }
@@ -206,10 +221,35 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
         use32 = __all(fp32_eligible(*Sp, L));
         use_fast = __all(!(L.h < Sp->max_step && L.err_prev < 1e-6));
       }
-      int res = attempt<METRIC, PREC>(*Sp, L, K, acc, p0, v0, use32, use_fast);
+      double *recp = nullptr;
+      if (record) {
+        // a new chunk every GRAD_CHUNK accepted steps, one atomicAdd per
+        // wavefront; chunks beyond the pool are counted, not written
+        const unsigned long long m = __ballot(r_need);
+        if (m) {
+          const int leader = __ffsll((long long)m) - 1;
+          unsigned long long base = 0;
+          if ((int)lane == leader) base = atomicAdd(T.rec_cursor, (unsigned long long)__popcll(m));
+          base = __shfl(base, leader, 64);
+          if (r_need) {
+            const unsigned long long mine = base + __popcll(m & ((1ull << lane) - 1ull));
+            const uint32_t prev = r_chunk;
+            r_chunk = mine < T.rec_cap ? (uint32_t)mine : NO_CHUNK;
+            if (r_chunk != NO_CHUNK)
+              T.rec_pool[(uint64_t)r_chunk * GRAD_CHUNK_DOUBLES] = __longlong_as_double((long long)prev);
+            r_need = false;
+          }
+        }
+        if (r_chunk != NO_CHUNK)
+          recp = T.rec_pool + (uint64_t)r_chunk * GRAD_CHUNK_DOUBLES + 1 + GRAD_STEP_REC * (L.accepted % GRAD_CHUNK);
+      }
+      int res = attempt<METRIC, PREC, record>(*Sp, L, K, acc, p0, v0, use32, use_fast, recp);
       L.attempts++;
       n_att++;
-      if (acc) L.accepted++;
+      if (acc) {
+        L.accepted++;
+        if (record && L.accepted % GRAD_CHUNK == 0) r_need = true;
+      }
       if (DISK == 1) {
         // trace_ray_with_disk (tracer.rs:96-145)
         double dl = L.lambda - lam0;
@@ -273,6 +313,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
       if (O.intensity) O.intensity[out_i] = L.intensity;
       if (O.attempts) O.attempts[out_i] = L.attempts;
       if (O.accepted) O.accepted[out_i] = L.accepted;
+      if (record) T.rec_last[out_i] = r_chunk;
       if (O.state) {
         const uint64_t p = O.n_rays;
         O.state[out_i] = L.pos[0];
@@ -425,6 +466,14 @@ inline size_t lds_bytes(uint32_t block) { return (size_t)block * 5 * 4 * sizeof(
 
 template <int M, int D, int P>
 hipError_t launch3(const TraceLaunch &t, uint32_t grid, uint32_t block, hipStream_t s) {
+  if constexpr (M == 1 && D == 2 && P == 0) {
+    // the recording render of the adjoint: Kerr thin disk, FP64, one scene
+    if (t.rec_pool) {
+      hipLaunchKernelGGL((trace_kernel<M, D, P, 2, 0>), dim3(grid), dim3(block), lds_bytes(block), s, t);
+      return hipGetLastError();
+    }
+  }
+  if (t.rec_pool) return hipErrorInvalidValue;
   if (t.states_in)
     hipLaunchKernelGGL((trace_kernel<M, D, P, 1, 0>), dim3(grid), dim3(block), lds_bytes(block), s, t);
   else if (t.n_scenes > 1)

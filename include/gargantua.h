@@ -307,10 +307,17 @@ typedef struct gg_grad_stats {
   uint64_t checkpoints;     /* checkpoints stored                             */
   uint64_t checkpoint_bytes;
   uint64_t segment_bytes;   /* per-thread segment buffers of the reverse pass */
-  uint64_t last_segment_bytes; /* step records of each ray's last segment,
-                               stored by the re-trace                         */
-  uint32_t grid_blocks;
+  uint64_t record_bytes;    /* step records stored (232 bytes each): each disk
+                               ray's last segment by the re-trace (modes 0, 1),
+                               or every ray's steps in 3720-byte chunks of 16
+                               by the render (mode 2)                         */
+  uint32_t grid_blocks;      /* reverse kernel launch                        */
   uint32_t block_threads;
+  uint32_t checkpoint_every; /* interval used (mode 1: at least the longest
+                               ray's accepted count; mode 2: 0)               */
+  uint32_t mode;             /* 0: checkpoints, segments recomputed; 1: every
+                               step record stored by a re-trace of the disk
+                               rays; 2: recorded by the render itself         */
 } gg_grad_stats;
 
 /* Loss L = mean_p (I_p - target_p)^2 over the width*height pixel-centre rays
@@ -319,14 +326,25 @@ typedef struct gg_grad_stats {
  * with each ray's step sizes, accept/reject pattern and disk-crossing step
  * held fixed (reverse mode; checkpoints every checkpoint_every accepted
  * steps, segments recomputed between them; the result does not depend on
- * checkpoint_every).  Only rays of class GG_CLASS_DISK carry a gradient.
+ * checkpoint_every).  checkpoint_every = 0: the render records every
+ * accepted step of every ray (232 bytes each, in chunks of 16) and nothing
+ * is re-traced or recomputed, when the records fit in the memory of
+ * gg_set_grad_memory (default a quarter of the free device memory); else a re-trace stores every disk ray's step records when
+ * those fit, else checkpoints every 32 steps.  An interval at least the
+ * longest ray's accepted count also stores every step record (re-trace).
+ * Only rays of class GG_CLASS_DISK carry a gradient.
  * target: [N] host or device.  Optional outputs: intensity [N] (the image),
  * d_intensity [4*N] (dI_p/d(spin, inclination, q, r_inner), AoS).
  * Requires |spin| < 0.9999 mass (the clamp of Kerr::new has no derivative),
- * 0 < inclination < pi, 1 <= checkpoint_every <= 65536. */
+ * 0 < inclination < pi, checkpoint_every <= 65536. */
 int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target,
             uint32_t checkpoint_every, double *loss, double grad[4],
             double *intensity, double *d_intensity, gg_grad_stats *stats);
+
+/* Device memory gg_grad's automatic mode (checkpoint_every = 0) may use for
+ * step records; 0 (the default): a quarter of the free device memory.  The
+ * choice of mode follows it; the gradient does not depend on it. */
+int gg_set_grad_memory(gg_context *ctx, uint64_t bytes);
 
 /* Vector-Jacobian product of gg_geodesic_acceleration for Kerr (FP64):
  * for each point, cot[4*i..] = w, pos_bar[4*i..] = w^T da/dpos (t and phi

@@ -98,13 +98,77 @@ def test_grad_independent_of_checkpoint_interval(renderer, oracle):
     target = _target(oracle)
     g0 = _scene()
     ref = renderer.grad(g0, target, checkpoint_every=1, want_dI=True)
-    for K in (3, 8, 32, 128, 100000 // 2):
+    assert ref["stats"]["mode"] == 0
+    for K in (3, 8, 32, 128, 100000 // 2, 0):
         a = renderer.grad(g0, target, checkpoint_every=K, want_dI=True)
-        assert a["loss"] == ref["loss"]
-        assert np.array_equal(a["grad"], ref["grad"]), K
-        assert np.array_equal(a["dI"], ref["dI"]), K
-        assert a["stats"]["steps"] == ref["stats"]["steps"]
-        assert a["stats"]["checkpoints"] <= ref["stats"]["checkpoints"]
+        _assert_same(a, ref, K)
+        st = a["stats"]
+        if K == 0:
+            # the render recorded every step: no checkpoint, no re-trace
+            assert st["mode"] == 2
+            assert st["checkpoints"] == 0 and st["attempts"] == 0 and st["segment_bytes"] == 0
+            assert st["record_bytes"] % 3720 == 0 and st["record_bytes"] >= 232 * st["steps"]
+        elif K == 100000 // 2:
+            # one segment per ray: the re-trace stores every step record
+            assert st["mode"] == 1
+            assert st["checkpoints"] == st["disk_rays"]
+            assert st["record_bytes"] == 232 * st["steps"]
+            assert st["segment_bytes"] == 0
+        else:
+            assert st["checkpoint_every"] == K
+            assert st["checkpoints"] <= ref["stats"]["checkpoints"]
+
+
+def _assert_same(a, ref, label):
+    assert a["loss"] == ref["loss"], label
+    assert np.array_equal(a["grad"], ref["grad"]), label
+    assert np.array_equal(a["dI"], ref["dI"]), label
+    assert a["stats"]["steps"] == ref["stats"]["steps"], label
+
+
+def test_grad_automatic_mode_follows_the_memory_budget(renderer, oracle):
+    """checkpoint_every = 0 under budgets that fit the render's records (mode
+    2), only the disk rays' records (mode 1), or neither (mode 0, K = 32):
+    the same gradient bit for bit."""
+    target = _target(oracle)
+    g0 = _scene()
+    ref = renderer.grad(g0, target, checkpoint_every=1, want_dI=True)
+    try:
+        a2 = renderer.grad(g0, target, checkpoint_every=0, want_dI=True)
+        assert a2["stats"]["mode"] == 2
+        need1 = 232 * ref["stats"]["steps"] + 72 * ref["stats"]["disk_rays"]
+        assert a2["stats"]["record_bytes"] > need1
+        renderer.set_grad_memory(need1)
+        a1 = renderer.grad(g0, target, checkpoint_every=0, want_dI=True)
+        assert a1["stats"]["mode"] == 1
+        renderer.set_grad_memory(1)
+        a0 = renderer.grad(g0, target, checkpoint_every=0, want_dI=True)
+        assert a0["stats"]["mode"] == 0 and a0["stats"]["checkpoint_every"] == 32
+        for lab, a in (("mode 2", a2), ("mode 1", a1), ("mode 0", a0)):
+            _assert_same(a, ref, lab)
+    finally:
+        renderer.set_grad_memory(0)
+
+
+def test_grad_record_pool_grows(oracle):
+    """A fresh context sized by a small scene meets a larger one: the record
+    pool overflows, is resized from the count of chunks the render asked for,
+    and the render runs again; the gradient equals a fresh context's."""
+    import gargantua_amd as gg
+    target = _target(oracle)
+    g0 = _scene()
+    r1 = gg.Renderer(0)
+    r2 = gg.Renderer(0)
+    try:
+        small = configs.grad_512_params(8, 8)
+        r1.grad(small, np.zeros(64), checkpoint_every=0)
+        a = r1.grad(g0, target, checkpoint_every=0, want_dI=True)
+        b = r2.grad(g0, target, checkpoint_every=0, want_dI=True)
+        assert a["stats"]["mode"] == 2 and b["stats"]["mode"] == 2
+        _assert_same(a, b, "grown pool")
+    finally:
+        r1.close()
+        r2.close()
 
 
 def test_grad_image_is_the_fp64_render(renderer, oracle):
@@ -126,7 +190,7 @@ def test_grad_zero_at_target(renderer):
 def test_grad_rejects_bad_arguments(renderer):
     g0 = _scene()
     with pytest.raises(gg.GargantuaError):
-        renderer.grad(g0, np.zeros(N * N), checkpoint_every=0)
+        renderer.grad(g0, np.zeros(N * N), checkpoint_every=65537)
     bad = _scene()
     bad.spin = 1.0
     with pytest.raises(gg.GargantuaError):
