@@ -225,16 +225,21 @@ def bench_grad512(args):
     rend = gg.Renderer(0, stream=torch.cuda.current_stream().cuda_stream)
     tgt_scene = cfg.grad_512_params(size, size, spin=0.7, inclination_deg=60.0, q=2.5, r_in=cfg.r_isco(1.0, 0.7))
     target = rend.grad_image(tgt_scene)
+    target_dev = torch.from_numpy(target).to("cuda")   # resident in HBM, read in place
     g0 = cfg.grad_512_params(size, size)
     per_k = {}
-    for K in (0, 65536, 8, 32, 128):
+    # (label, K, learned ray order for the render)
+    runs = [("auto+order", 0, 1), ("auto", 0, 0), ("K65536", 65536, 0), ("K8", 8, 0), ("K32", 32, 0),
+            ("K128", 128, 0)]
+    for K, (label, kk, order) in [(r[0], r) for r in runs]:
+        rend.set_launch(order=order)
         for _ in range(args.warmup):
-            rend.grad(g0, target, checkpoint_every=K)
+            rend.grad(g0, target_dev, checkpoint_every=kk, want_image=False)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         sts = []
         for _ in range(args.steps):
-            out = rend.grad(g0, target, checkpoint_every=K)
+            out = rend.grad(g0, target_dev, checkpoint_every=kk, want_image=False)
             sts.append(out["stats"])
         wall = time.perf_counter() - t0
         st = sts[-1]
@@ -248,11 +253,13 @@ def bench_grad512(args):
                     "attempts_retraced": st["attempts"], "checkpoints": st["checkpoints"],
                     "checkpoint_MiB": st["checkpoint_bytes"] / 2**20, "segment_MiB": st["segment_bytes"] / 2**20,
                     "record_MiB": st["record_bytes"] / 2**20, "checkpoint_every": st["checkpoint_every"],
-                    "mode": st["mode"],
+                    "mode": st["mode"], "learned_order": bool(order), "total_ms_in_library":
+                        float(np.mean([x["total_ms"] for x in sts])),
                     "grid_blocks": st["grid_blocks"], "loss": out["loss"], "grad": out["grad"].tolist()}
         _log(f"grad512 K={K}: {per_k[K]['ms_per_grad']:.1f} ms/grad (adjoint {adj_ms:.1f} ms, "
              f"forward {per_k[K]['forward_ms']:.1f} ms, mode {st['mode']})")
-    same = all(per_k[K]["grad"] == per_k[8]["grad"] for K in per_k)
+    rend.set_launch(order=1)
+    same = all(per_k[K]["grad"] == per_k["K8"]["grad"] for K in per_k)
     # gradient-descent recovery (Adam on the raw parameters)
     gd = None
     if args.gd_steps > 0:
@@ -266,7 +273,7 @@ def bench_grad512(args):
         for it in range(1, args.gd_steps + 1):
             g = cfg.grad_512_params(size, size, spin=theta[0], inclination_deg=math.degrees(theta[1]), q=theta[2],
                                     r_in=theta[3])
-            out = rend.grad(g, target, checkpoint_every=0)
+            out = rend.grad(g, target_dev, checkpoint_every=0, want_image=False)
             grad = out["grad"]
             m = b1 * m + (1 - b1) * grad
             v = b2 * v + (1 - b2) * grad * grad
@@ -283,15 +290,15 @@ def bench_grad512(args):
               "theta_true": truth.tolist(), "abs_err": np.abs(theta - truth).tolist(),
               "loss_first": hist[0], "loss_last": hist[-1]}
     rend.close()
-    best = per_k[0]
+    best = per_k["auto+order"]
     return {"metric": f"adjoint gradient evaluations/s, {size}x{size} Kerr thin disk, dL/d(a, i, q, r_in)",
             "value": best["grads_per_s"], "unit": "gradients/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": best["ms_per_grad"], "higher_is_better": True, "dtype": "f64",
             "data": "synthetic (target image rendered at the true parameters)",
             "config": {"workload": f"{size}x{size} Kerr thin disk r_out 20M edge 0.25M, camera r=50M FOV 30deg, "
                                    "rtol 1e-8 atol 1e-10; theta0 (0.5, 45deg, 3.0, r_ISCO+0.5), target "
-                                   "(0.7, 60deg, 2.5, r_ISCO(0.7)); value at K=0 (the render records every step)",
-                       "K": 0},
+                                   "(0.7, 60deg, 2.5, r_ISCO(0.7)); value at K=0 (the render records every "
+                                   "step), learned ray order, target resident on the device", "K": 0},
             "roofline": {"bound": "valu", "achieved": best["adjoint_tflops"], "peak": PEAK_FP64, "unit": "TFLOP/s",
                          "frac": best["adjoint_frac_fp64_peak"], "traffic": None,
                          "kernel": "adjoint_seed_kernel + adjoint_rev_chunked_kernel + adjoint_cam_kernel"},

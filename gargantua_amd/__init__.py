@@ -360,23 +360,33 @@ class Renderer:
         """Device memory grad()'s automatic mode may use for step records (0: a quarter of the free memory)."""
         _check(self.lib, self.lib.gg_set_grad_memory(self.ctx, int(nbytes)), "gg_set_grad_memory")
 
-    def grad(self, gscene, target: np.ndarray, checkpoint_every: int = 0, want_dI: bool = False) -> dict:
+    def grad(self, gscene, target, checkpoint_every: int = 0, want_dI: bool = False,
+             want_image: bool = True) -> dict:
         """Loss, dL/d(spin, inclination, q, r_inner) and the image of a gg_grad_scene (see configs.grad_512_params).
 
+        target: [width*height] float64, a numpy array or a contiguous device tensor (anything with
+        data_ptr(), numel() and a float64 dtype, e.g. a torch CUDA tensor), which is read in place.
         checkpoint_every = 0: the render records every accepted step and nothing is re-traced (stats mode 2)
         when the records fit the memory of set_grad_memory; else a re-trace stores the disk rays' steps (mode 1)
         or checkpoints every 32 accepted steps (mode 0).  The gradient is the same in every mode."""
         n = gscene.width * gscene.height
-        target = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
-        if target.size != n:
-            raise ValueError(f"target has {target.size} pixels, scene {n}")
+        if hasattr(target, "data_ptr"):
+            if str(target.dtype) not in ("torch.float64", "float64") or target.numel() != n or \
+                    not target.is_contiguous():
+                raise ValueError(f"device target must be {n} contiguous float64 values")
+            tptr = target.data_ptr()
+        else:
+            target = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
+            if target.size != n:
+                raise ValueError(f"target has {target.size} pixels, scene {n}")
+            tptr = target.ctypes.data
         loss = C.c_double()
         g = np.zeros(4)
-        inten = np.zeros(n)
+        inten = np.zeros(n) if want_image else None
         dI = np.zeros((n, 4)) if want_dI else None
         st = _abi.gg_grad_stats()
-        _check(self.lib, self.lib.gg_grad(self.ctx, C.byref(gscene), target.ctypes.data, checkpoint_every,
-                                          C.byref(loss), g.ctypes.data, inten.ctypes.data,
+        _check(self.lib, self.lib.gg_grad(self.ctx, C.byref(gscene), tptr, checkpoint_every,
+                                          C.byref(loss), g.ctypes.data, inten.ctypes.data if want_image else None,
                                           dI.ctypes.data if want_dI else None, C.byref(st)), "gg_grad")
         out = {"loss": loss.value, "grad": g, "intensity": inten,
                "stats": {k: getattr(st, k) for k, _ in _abi.gg_grad_stats._fields_}}

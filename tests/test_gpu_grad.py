@@ -150,6 +150,51 @@ def test_grad_automatic_mode_follows_the_memory_budget(renderer, oracle):
         renderer.set_grad_memory(0)
 
 
+class _DeviceArray:
+    """float64 values copied to device memory with the HIP runtime the library uses."""
+
+    dtype = "float64"
+
+    def __init__(self, a):
+        import ctypes as C
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.n = a.size
+        self.p = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(self.p), C.c_size_t(a.nbytes)) == 0
+        assert self.hip.hipMemcpy(self.p, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), 1) == 0
+
+    def data_ptr(self):
+        return self.p.value
+
+    def numel(self):
+        return self.n
+
+    def is_contiguous(self):
+        return True
+
+    def free(self):
+        self.hip.hipFree(self.p)
+
+
+def test_grad_learned_order_and_device_target(renderer, oracle):
+    """The render under the learned ray order (queued by the previous call's
+    attempt counts) and a target read in place from device memory give the
+    same gradient bit for bit."""
+    target = _target(oracle)
+    g0 = _scene()
+    ref = renderer.grad(g0, target, checkpoint_every=1, want_dI=True)
+    t_dev = _DeviceArray(np.ascontiguousarray(target, dtype=np.float64))
+    try:
+        renderer.set_launch(order=1)
+        for _ in range(2):   # the second call runs in the learned order
+            a = renderer.grad(g0, t_dev, checkpoint_every=0, want_dI=True, want_image=False)
+            assert a["stats"]["mode"] == 2 and a["intensity"] is None
+            _assert_same(a, ref, "learned order")
+    finally:
+        renderer.set_launch()
+        t_dev.free()
+
+
 def test_grad_record_pool_grows(oracle):
     """A fresh context sized by a small scene meets a larger one: the record
     pool overflows, is resized from the count of chunks the render asked for,
