@@ -871,7 +871,7 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
   DevBuf *B = ctx->grad;
   if ((rc = B[0].ensure(n)) || (rc = B[1].ensure(8 * n)) || (rc = B[2].ensure(4 * n)) ||
       (rc = B[3].ensure(8 * n)) || (rc = B[4].ensure(8 * n)) || (rc = B[5].ensure(32 * n)) ||
-      (rc = B[6].ensure(4 * n)) || (rc = B[7].ensure(sizeof(GradCounters))) || (rc = B[8].ensure(64)) ||
+      (rc = B[6].ensure(4 * n)) || (rc = B[7].ensure(sizeof(GradCounters))) || (rc = B[8].ensure(64 + grad_reduce_scratch_bytes())) ||
       (rc = B[9].ensure(sizeof(DevScene))) || (rc = B[12].ensure(4 * n)) || (rc = B[13].ensure(4 * n)) ||
       (rc = B[14].ensure(4 * n)))
     return rc;
@@ -1026,7 +1026,7 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
     HIP_TRY(launch_adjoint_cam(G, s));
     HIP_TRY(hipEventRecord(ctx->ev1, s));
   }
-  HIP_TRY(launch_grad_reduce(n, d_resid, d_dI, d_out5, s));
+  HIP_TRY(launch_grad_reduce(n, d_resid, d_dI, d_out5 + 8, d_out5, s));
   double out5[5];
   HIP_TRY(hipMemcpyAsync(out5, d_out5, sizeof(out5), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(&cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, s));

@@ -44,16 +44,17 @@ static_assert(STEP_REC == GRAD_STEP_REC, "step record layout");
 
 // Adjoint of one accepted Kerr step (p, v) -> (p + h v normalised,
 // v + h sum_i c_i k_i), k_i = a(p + h a_i v, v_i), from its step record
-// (p, v, h, v_2..v_6).  On entry (pb_r, pb_th, vb) is the cotangent of the
+// (p, v, v_2..v_6, h).  On entry (pb_r, pb_th, vb) is the cotangent of the
 // step's end state (t and phi carry none: nothing depends on them); on exit
 // that of its start.  ga accumulates the spin cotangent.  LDS slots 0..5
 // hold the stage cotangents kbar_1..kbar_6.
-__device__ inline void step_adjoint(const KStore &K, const double *rec, double m, double a, double &pb_r,
+__device__ inline void step_adjoint(const KStore &K, const double *rec_in, double m, double a, double &pb_r,
                                     double &pb_th, d4 &vb, double &ga) {
 #pragma clang fp contract(off)
+  const double *rec = (const double *)__builtin_assume_aligned(rec_in, 32);
   const double r0 = rec[1], th0 = rec[2];
   const d4 v{rec[4], rec[5], rec[6], rec[7]};
-  const double h = rec[8];
+  const double h = rec[REC_H];
   // normalize_spherical_coords (geodesic.rs:337-352): theta -> -theta or
   // 2 pi - theta flips the sign of its cotangent
   double th1 = h * v.z + th0;
@@ -81,7 +82,7 @@ __device__ inline void step_adjoint(const KStore &K, const double *rec, double m
       ha = h * kA[i];
       qr = ha * v.y + r0;
       qth = ha * v.z + th0;
-      const double *o = rec + 9 + 4 * (i - 2);
+      const double *o = rec + REC_VS + 4 * (i - 2);
       vs = d4{o[0], o[1], o[2], o[3]};
     }
     const RhsBar jb = kerr_rhs_vjp(m, a, qr, qth, vs, w);
@@ -363,7 +364,7 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
 
     // ---- REV: one accepted step backwards ----
     if (phase == REV) {
-      const double *sp = MODE == 2 ? G.pool + (uint64_t)chunk * GRAD_CHUNK_DOUBLES + 1 + STEP_REC * (j % GRAD_CHUNK)
+      const double *sp = MODE == 2 ? G.pool + (uint64_t)chunk * GRAD_CHUNK_DOUBLES + GRAD_CHUNK_HEAD + STEP_REC * (j % GRAD_CHUNK)
                                    : src + STEP_REC * (uint64_t)j;
       step_adjoint(K, sp, S.mass, S.spin, pb_r, pb_th, vb, ga);
       ++n_adj;
@@ -420,7 +421,7 @@ __global__ __launch_bounds__(256) void adjoint_seed_kernel(GradLaunch G) {
     double *rec = G.rec + REC * q;
     bool ok = acc > 0 && chunk != 0xFFFFFFFFu && chunk < G.pool_chunks;
     if (ok) {
-      const double *r = G.pool + (uint64_t)chunk * GRAD_CHUNK_DOUBLES + 1 + STEP_REC * ((acc - 1) % GRAD_CHUNK);
+      const double *r = G.pool + (uint64_t)chunk * GRAD_CHUNK_DOUBLES + GRAD_CHUNK_HEAD + STEP_REC * ((acc - 1) % GRAD_CHUNK);
       const uint64_t n = G.n_pix;
       const double *fs = G.final_state;
       ok = crossing_seeds(G, *G.scene, rec, r[1], r[2], r[4], r[7], fs[n + pix], fs[2 * n + pix], fs[4 * n + pix],
@@ -458,11 +459,12 @@ __global__ __launch_bounds__(256) void adjoint_cam_kernel(GradLaunch G) {
 
 // Residuals, zeroed gradients and the list of disk rays with their
 // accepted counts (sort keys), checkpoint and last-segment totals, the total
-// and the largest accepted count.  One atomic per wavefront for the list and
-// each total.
-__global__ void select_kernel(GradSelect P) {
+// and the largest accepted count.  One atomic per block for the list and
+// each total (the list order does not matter: it is sorted next).
+constexpr uint32_t SELECT_BLOCK = 1024;
+__global__ __launch_bounds__(SELECT_BLOCK) void select_kernel(GradSelect P) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   bool disk = false;
   uint32_t nchk = 0, last = 0, acc = 0;
   if (i < P.n) {
@@ -480,46 +482,79 @@ __global__ void select_kernel(GradSelect P) {
   const unsigned long long tot = wave_sum(nchk), tl = wave_sum(last), all = wave_sum(acc);
   uint32_t mx = acc;
   for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off, 64));
-  if (m) {
-    const int leader = __ffsll((long long)m) - 1;
+  constexpr uint32_t NW = SELECT_BLOCK / 64;
+  __shared__ unsigned long long s_cnt[NW], s_tot[NW], s_tl[NW], s_all[NW];
+  __shared__ uint32_t s_mx[NW];
+  __shared__ unsigned long long s_base;
+  if (lane == 0) { s_cnt[wave] = __popcll(m); s_tot[wave] = tot; s_tl[wave] = tl; s_all[wave] = all; s_mx[wave] = mx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long c = 0, t = 0, l = 0, al = 0;
+    uint32_t x = 0;
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
+      c += s_cnt[w]; t += s_tot[w]; l += s_tl[w]; al += s_all[w]; x = max(x, s_mx[w]);
+    }
     unsigned long long base = 0;
-    if ((int)lane == leader) {
-      base = atomicAdd(&P.cnt->n_list, (unsigned long long)__popcll(m));
-      atomicAdd(&P.cnt->chk_total, tot);
-      atomicAdd(&P.cnt->tail_total, tl);
-      atomicAdd(&P.cnt->acc_total, all);
-      atomicMax(&P.cnt->max_acc, (unsigned long long)mx);
+    if (c) {
+      base = atomicAdd(&P.cnt->n_list, c);
+      atomicAdd(&P.cnt->chk_total, t);
+      atomicAdd(&P.cnt->tail_total, l);
+      atomicAdd(&P.cnt->acc_total, al);
+      atomicMax(&P.cnt->max_acc, (unsigned long long)x);
     }
-    base = __shfl(base, leader, 64);
-    if (disk) {
-      const uint64_t at = base + __popcll(m & ((1ull << lane) - 1ull));
-      P.list[at] = (uint32_t)i;
-      P.keys[at] = acc;
-    }
+    s_base = base;
+  }
+  __syncthreads();
+  if (disk) {
+    unsigned long long at = s_base + __popcll(m & ((1ull << lane) - 1ull));
+    for (uint32_t w = 0; w < wave; ++w) at += s_cnt[w];
+    P.list[at] = (uint32_t)i;
+    P.keys[at] = acc;
   }
 }
 
-// loss = mean r^2, grad = 2 mean r dI: one block, fixed summation order.
-__global__ __launch_bounds__(1024) void reduce_kernel(uint64_t n, const double *resid, const double *dI,
-                                                      double *out5) {
-  __shared__ double part[5][1024];
+// loss = mean r^2, grad = 2 mean r dI in two deterministic passes: block b
+// sums a fixed contiguous range of pixels (fixed per-thread order, fixed
+// tree), then one block sums the block partials in order.
+constexpr uint32_t REDUCE_BLOCKS = 256, REDUCE_THREADS = 256;
+__device__ inline void block_tree5(double acc[5], double (*part)[REDUCE_THREADS]) {
   const uint32_t t = threadIdx.x;
-  double acc[5] = {0, 0, 0, 0, 0};
-  for (uint64_t i = t; i < n; i += 1024) {
-    double r = resid[i];
-    acc[0] += r * r;
-    const double *d = dI + 4 * i;
-    acc[1] += r * d[0]; acc[2] += r * d[1]; acc[3] += r * d[2]; acc[4] += r * d[3];
-  }
   for (int k = 0; k < 5; ++k) part[k][t] = acc[k];
   __syncthreads();
-  for (uint32_t w = 512; w > 0; w >>= 1) {
+  for (uint32_t w = REDUCE_THREADS / 2; w > 0; w >>= 1) {
     if (t < w)
       for (int k = 0; k < 5; ++k) part[k][t] += part[k][t + w];
     __syncthreads();
   }
-  if (t == 0) {
-    double inv = 1.0 / (double)n;
+}
+
+__global__ __launch_bounds__(REDUCE_THREADS) void reduce_partial_kernel(uint64_t n, const double *resid,
+                                                                        const double *dI, double *partials) {
+  __shared__ double part[5][REDUCE_THREADS];
+  const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const uint64_t begin = (uint64_t)blockIdx.x * per;
+  const uint64_t end = begin + per < n ? begin + per : n;
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (uint64_t i = begin + threadIdx.x; i < end; i += REDUCE_THREADS) {
+    const double r = resid[i];
+    acc[0] += r * r;
+    const double *d = dI + 4 * i;
+    acc[1] += r * d[0]; acc[2] += r * d[1]; acc[3] += r * d[2]; acc[4] += r * d[3];
+  }
+  block_tree5(acc, part);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 5; ++k) partials[5 * blockIdx.x + k] = part[k][0];
+}
+
+__global__ __launch_bounds__(REDUCE_THREADS) void reduce_final_kernel(uint64_t n, uint32_t nb, const double *partials,
+                                                                      double *out5) {
+  __shared__ double part[5][REDUCE_THREADS];
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (uint32_t b = threadIdx.x; b < nb; b += REDUCE_THREADS)
+    for (int k = 0; k < 5; ++k) acc[k] += partials[5 * b + k];
+  block_tree5(acc, part);
+  if (threadIdx.x == 0) {
+    const double inv = 1.0 / (double)n;
     out5[0] = part[0][0] * inv;
     for (int k = 1; k < 5; ++k) out5[k] = 2.0 * part[k][0] * inv;
   }
@@ -587,12 +622,18 @@ hipError_t sort_rays_desc(const uint32_t *keys_in, uint32_t *keys_out, const uin
 
 hipError_t launch_grad_select(const GradSelect &p, hipStream_t s) {
   if (p.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((p.n + 255) / 256)), dim3(256), 0, s, p);
+  hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((p.n + SELECT_BLOCK - 1) / SELECT_BLOCK)), dim3(SELECT_BLOCK), 0,
+                     s, p);
   return hipGetLastError();
 }
 
-hipError_t launch_grad_reduce(uint64_t n, const double *resid, const double *dI, double *out5, hipStream_t s) {
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, s, n, resid, dI, out5);
+size_t grad_reduce_scratch_bytes() { return sizeof(double) * 5 * REDUCE_BLOCKS; }
+
+hipError_t launch_grad_reduce(uint64_t n, const double *resid, const double *dI, double *partials, double *out5,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(reduce_partial_kernel, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, s, n, resid, dI, partials);
+  hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(REDUCE_THREADS), 0, s, n, REDUCE_BLOCKS,
+                     (const double *)partials, out5);
   return hipGetLastError();
 }
 

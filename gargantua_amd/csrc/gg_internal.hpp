@@ -40,7 +40,8 @@ struct TraceLaunch {
   const uint32_t *order;      // camera renders: queue position -> global ray, or null (ray order)
   // Recording render of gg_grad (Kerr thin disk, FP64, one scene), else
   // null: every accepted step's record in chunks (GRAD_CHUNK_DOUBLES each:
-  // the index of the ray's previous chunk, then GRAD_CHUNK step records);
+  // the index of the ray's previous chunk in a 128-byte line of its own,
+  // then GRAD_CHUNK step records);
   // rec_last[output index] = the ray's last chunk.
   double *rec_pool;
   unsigned long long rec_cap;       // chunks in the pool
@@ -104,9 +105,10 @@ struct GradLaunch {
 };
 
 constexpr uint32_t GRAD_REC = 16;       // doubles per ray record
-constexpr uint32_t GRAD_STEP_REC = 29;  // doubles per step record: pos, vel, h, v_2..v_6
+constexpr uint32_t GRAD_STEP_REC = 32;  // doubles per step record: pos, vel, v_2..v_6, h, padding
 constexpr uint32_t GRAD_CHUNK = 16;     // step records per chunk of the recording render
-constexpr uint32_t GRAD_CHUNK_DOUBLES = 1 + GRAD_CHUNK * GRAD_STEP_REC;
+constexpr uint32_t GRAD_CHUNK_HEAD = 16;   // the back-pointer's 128-byte line
+constexpr uint32_t GRAD_CHUNK_DOUBLES = GRAD_CHUNK_HEAD + GRAD_CHUNK * GRAD_STEP_REC;
 size_t adjoint_lds_bytes(uint32_t block);
 size_t adjoint_fwd_lds_bytes(uint32_t block);
 hipError_t launch_adjoint_fwd(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s);
@@ -116,7 +118,9 @@ hipError_t launch_adjoint_cam(const GradLaunch &g, hipStream_t s);
 hipError_t sort_rays_desc(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
                           uint64_t n, int key_bits, void *temp, size_t &temp_bytes, hipStream_t s);
 hipError_t launch_grad_select(const GradSelect &p, hipStream_t s);
-hipError_t launch_grad_reduce(uint64_t n, const double *resid, const double *dI, double *out5, hipStream_t s);
+size_t grad_reduce_scratch_bytes();
+hipError_t launch_grad_reduce(uint64_t n, const double *resid, const double *dI, double *partials, double *out5,
+                              hipStream_t s);
 hipError_t launch_grad_reset(GradCounters *c, hipStream_t s);
 hipError_t launch_rhs_vjp(const DevScene *S, uint64_t n, const double *pos, const double *vel, const double *cot,
                           double *pos_bar, double *vel_bar, double *spin_bar, hipStream_t s);

@@ -143,9 +143,13 @@ struct StageOut {
   bool invalid;
 };
 
-// Step record of an accepted step for the adjoint: start position (4),
-// velocity (4), h, then the stage velocities v_2..v_6 (4 each).
-constexpr int STEP_REC = 29;
+// Step record of an accepted step for the adjoint, 32 doubles (256 bytes,
+// two 128-byte lines; records are 128-byte aligned): start position (0..3),
+// velocity (4..7), the stage velocities v_2..v_6 (REC_VS + 4 (s - 2)), h
+// (REC_H), 3 doubles of padding.  Every 4-vector is 32-byte aligned.
+constexpr int STEP_REC = 32;
+constexpr int REC_VS = 8;
+constexpr int REC_H = 28;
 
 // The stages of one attempt (geodesic.rs:244-308): k1 when the lane has no
 // valid one (s = 1), then k2..k7, the 5th-order velocity and the error.  One
@@ -197,7 +201,7 @@ __device__ inline StageOut stages(const DevScene &S, const KStore &K, const doub
       qr = q1;
       qth = q2;
       if (REC && rec) {
-        double *o = rec + 9 + 4 * (s - 2);
+        double *o = (double *)__builtin_assume_aligned(rec + REC_VS + 4 * (s - 2), 32);
         o[0] = vs.x; o[1] = vs.y; o[2] = vs.z; o[3] = vs.w;
       }
     } else {
@@ -350,9 +354,10 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
     L.pos[0] = madd<BASE>(h, vel.x, p0[0]); L.pos[1] = madd<BASE>(h, vel.y, p0[1]);
     L.pos[2] = madd<BASE>(h, vel.z, p0[2]); L.pos[3] = madd<BASE>(h, vel.w, p0[3]);
     if (REC && rec) {
-      rec[0] = p0[0]; rec[1] = p0[1]; rec[2] = p0[2]; rec[3] = p0[3];
-      rec[4] = vel.x; rec[5] = vel.y; rec[6] = vel.z; rec[7] = vel.w;
-      rec[8] = h;
+      double *r = (double *)__builtin_assume_aligned(rec, 32);
+      r[0] = p0[0]; r[1] = p0[1]; r[2] = p0[2]; r[3] = p0[3];
+      r[4] = vel.x; r[5] = vel.y; r[6] = vel.z; r[7] = vel.w;
+      r[REC_H] = h;
     }
     L.vel = o.nv;
     L.lambda = L.lambda + h;

@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
           }
         }
         if (r_chunk != NO_CHUNK)
-          recp = T.rec_pool + (uint64_t)r_chunk * GRAD_CHUNK_DOUBLES + 1 + GRAD_STEP_REC * (L.accepted % GRAD_CHUNK);
+          recp = T.rec_pool + (uint64_t)r_chunk * GRAD_CHUNK_DOUBLES + GRAD_CHUNK_HEAD + GRAD_STEP_REC * (L.accepted % GRAD_CHUNK);
       }
       int res = attempt<METRIC, PREC, record>(*Sp, L, K, acc, p0, v0, use32, use_fast, recp);
       L.attempts++;

@@ -307,9 +307,9 @@ typedef struct gg_grad_stats {
   uint64_t checkpoints;     /* checkpoints stored                             */
   uint64_t checkpoint_bytes;
   uint64_t segment_bytes;   /* per-thread segment buffers of the reverse pass */
-  uint64_t record_bytes;    /* step records stored (232 bytes each): each disk
+  uint64_t record_bytes;    /* step records stored (256 bytes each): each disk
                                ray's last segment by the re-trace (modes 0, 1),
-                               or every ray's steps in 3720-byte chunks of 16
+                               or every ray's steps in 4224-byte chunks of 16
                                by the render (mode 2)                         */
   uint32_t grid_blocks;      /* reverse kernel launch                        */
   uint32_t block_threads;
@@ -327,7 +327,7 @@ typedef struct gg_grad_stats {
  * held fixed (reverse mode; checkpoints every checkpoint_every accepted
  * steps, segments recomputed between them; the result does not depend on
  * checkpoint_every).  checkpoint_every = 0: the render records every
- * accepted step of every ray (232 bytes each, in chunks of 16) and nothing
+ * accepted step of every ray (256 bytes each, in chunks of 16) and nothing
  * is re-traced or recomputed, when the records fit in the memory of
  * gg_set_grad_memory (default a quarter of the free device memory); else a re-trace stores every disk ray's step records when
  * those fit, else checkpoints every 32 steps.  An interval at least the

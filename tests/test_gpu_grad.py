@@ -107,12 +107,12 @@ def test_grad_independent_of_checkpoint_interval(renderer, oracle):
             # the render recorded every step: no checkpoint, no re-trace
             assert st["mode"] == 2
             assert st["checkpoints"] == 0 and st["attempts"] == 0 and st["segment_bytes"] == 0
-            assert st["record_bytes"] % 3720 == 0 and st["record_bytes"] >= 232 * st["steps"]
+            assert st["record_bytes"] % 4224 == 0 and st["record_bytes"] >= 256 * st["steps"]
         elif K == 100000 // 2:
             # one segment per ray: the re-trace stores every step record
             assert st["mode"] == 1
             assert st["checkpoints"] == st["disk_rays"]
-            assert st["record_bytes"] == 232 * st["steps"]
+            assert st["record_bytes"] == 256 * st["steps"]
             assert st["segment_bytes"] == 0
         else:
             assert st["checkpoint_every"] == K
@@ -136,7 +136,7 @@ def test_grad_automatic_mode_follows_the_memory_budget(renderer, oracle):
     try:
         a2 = renderer.grad(g0, target, checkpoint_every=0, want_dI=True)
         assert a2["stats"]["mode"] == 2
-        need1 = 232 * ref["stats"]["steps"] + 72 * ref["stats"]["disk_rays"]
+        need1 = 256 * ref["stats"]["steps"] + 72 * ref["stats"]["disk_rays"]
         assert a2["stats"]["record_bytes"] > need1
         renderer.set_grad_memory(need1)
         a1 = renderer.grad(g0, target, checkpoint_every=0, want_dI=True)
