@@ -50,7 +50,7 @@ static_assert(STEP_REC == GRAD_STEP_REC, "step record layout");
 // hold the stage cotangents kbar_1..kbar_6.
 __device__ inline void step_adjoint(const KStore &K, const double *rec_in, double m, double a, double &pb_r,
                                     double &pb_th, d4 &vb, double &ga) {
-#pragma clang fp contract(off)
+  GG_CONTRACT_FAST
   const double *rec = (const double *)__builtin_assume_aligned(rec_in, 32);
   const double r0 = rec[1], th0 = rec[2];
   const d4 v{rec[4], rec[5], rec[6], rec[7]};

@@ -8,6 +8,16 @@
 #define GG_HD
 #endif
 
+// Multiply-add contraction allowed in the enclosing block (the library is
+// built with it off, which the forward integration needs to round like the
+// reference); for the adjoint's own algebra, whose results are checked to a
+// tolerance.  Other compilers of the host checks ignore it.
+#ifdef __clang__
+#define GG_CONTRACT_FAST _Pragma("clang fp contract(fast)")
+#else
+#define GG_CONTRACT_FAST
+#endif
+
 namespace gg {
 
 struct d4 { double x, y, z, w; };

@@ -22,6 +22,7 @@ namespace gg {
 // Coefficient of symbol i in w . a.  Symbol order:
 // 001 002 013 023 100 111 122 133 112 103 200 211 222 233 212 203 301 302 313 323
 GG_HD inline double gamma_coef(int i, d4 v, d4 w) {
+  GG_CONTRACT_FAST
   switch (i) {
     case 0: return -2.0 * w.x * v.x * v.y;
     case 1: return -2.0 * w.x * v.x * v.z;
@@ -48,6 +49,7 @@ GG_HD inline double gamma_coef(int i, d4 v, d4 w) {
 
 // gv += Gamma_i dC_i/dv.
 GG_HD inline void gamma_coef_dv(int i, double g, d4 v, d4 w, d4 &gv) {
+  GG_CONTRACT_FAST
   switch (i) {
     case 0: { double t = -2.0 * w.x * g; gv.x += t * v.y; gv.y += t * v.x; break; }
     case 1: { double t = -2.0 * w.x * g; gv.x += t * v.z; gv.z += t * v.x; break; }
@@ -78,6 +80,7 @@ struct RhsBar {
 };
 
 GG_HD inline RhsBar kerr_rhs_vjp(double m, double a, double r, double th, d4 v, d4 w) {
+  GG_CONTRACT_FAST
   // ---- forward: the symbol algebra of rhs64f_kerr, intermediates kept ----
   const double st = sin(th), ct = cos(th);
   const double tm = 2.0 * m;
