@@ -258,6 +258,7 @@ void build_dev_scene(const gg_scene &s, DevScene &D) {
   D.h0 = s.integrator.initial_step;
   D.min_step = s.integrator.min_step;
   D.max_step = s.integrator.max_step;
+  D.max_step5 = D.max_step * D.max_step * D.max_step * D.max_step * D.max_step;
   D.tol = s.integrator.tolerance;
   D.rtol = s.integrator.rtol;
   D.atol = s.integrator.atol;

@@ -34,6 +34,7 @@ struct DevScene {
   double max_radius;
   // integrator
   double h0, min_step, max_step, tol, rtol, atol;
+  double max_step5;       // max_step^5: the controller's clamp test
   // camera
   double cam_pos[4];
   double fwd[3], right[3], up[3];
