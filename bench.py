@@ -36,6 +36,20 @@ F_COMB = 318
 PEAK_FP64 = 78.6    # TFLOP/s, MI355X vector FP64 (published)
 PEAK_FP32 = 157.3   # TFLOP/s, MI355X vector FP32 (published; needs packed math)
 
+
+def profiled_traffic(profile: str, column: int):
+    """HBM bytes per launch of a kernel from a committed counter summary (FETCH_SIZE and WRITE_SIZE passes of
+    rocprofv3, scaled by a 512 MiB copy), or None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", profile)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("| HBM traffic per launch (calibrated, MB)"):
+                    return float(line.split("|")[2 + column]) * 1e6
+    except (OSError, ValueError, IndexError):
+        return None
+    return None
+
 METRIC = "geodesics/s Kerr a=0.9 1024²: mixed FP32 vs FP64 at ≤1e-6 err; adjoint grads/s"
 
 
@@ -301,7 +315,10 @@ def bench_grad512(args):
                                    "(0.7, 60deg, 2.5, r_ISCO(0.7)); value at K=0 (the render records every "
                                    "step), learned ray order, target resident on the device", "K": 0},
             "roofline": {"bound": "valu", "achieved": best["adjoint_tflops"], "peak": PEAK_FP64, "unit": "TFLOP/s",
-                         "frac": best["adjoint_frac_fp64_peak"], "traffic": None,
+                         "frac": best["adjoint_frac_fp64_peak"],
+                         "traffic": profiled_traffic("adjoint_grad512_counters.md", 1),
+                         "traffic_source": "profiles/adjoint_grad512_counters.md, adjoint_rev_chunked_kernel "
+                                           "(dominant kernel; reads 4.4 GB of step records)",
                          "kernel": "adjoint_seed_kernel + adjoint_rev_chunked_kernel + adjoint_cam_kernel"},
             "per_K": per_k, "grad_identical_across_K": same, "gradient_descent": gd}
 
@@ -402,7 +419,10 @@ def main():
                                "attempt counts (learned in the warm-up), every ray traced in full",
                    "first_frame_tile_order_ms": tile_ms},
         "roofline": {"bound": "valu", "achieved": tflops, "peak": peak, "unit": "TFLOP/s", "frac": tflops / peak,
-                     "traffic": None, "kernel": "trace_kernel<Kerr, thin, mixed>",
+                     "traffic": profiled_traffic("trace_kernel_kerr1024_counters.md", 0),
+                     "traffic_source": "profiles/trace_kernel_kerr1024_counters.md (rocprofv3 FETCH_SIZE and "
+                                       "WRITE_SIZE, copy-calibrated); algorithmic outputs 47 MB",
+                     "kernel": "trace_kernel<Kerr, thin, mixed>",
                      "kernel_ms": k_ms, "flop_per_launch": f32 + f64, "fp32_flop_share": f32 / (f32 + f64)},
         "fp64": {"value": n * args.steps / wall_d, "kernel_ms": k_ms_d,
                  "roofline_frac": (d32 + d64) / (k_ms_d * 1e-3) / 1e12 / PEAK_FP64,
