@@ -371,7 +371,7 @@ def main():
     # Launch shape per precision from the autotuner: block size, blocks per CU,
     # and whether rays are queued longest first by the previous render's
     # attempt counts (every ray is traced in full either way).
-    cands = [(256, 2, 16, 2, 0), (256, 2, 16, 2, 1), (64, 8, 16, 2, 0), (64, 8, 16, 2, 1)]
+    cands = [(256, 2, 16, 2, 1), (64, 8, 16, 2, 1), (64, 8, 64, 2, 0), (64, 8, 64, 2, 1)]
     tuned = {}
     for prec in ("mixed", "fp64", "fp64fast"):
         tuned[prec] = rend.autotune([scene], prec, candidates=cands)[0]
