@@ -42,6 +42,31 @@ def test_struct_sizes_match_c_layout():
     assert C.sizeof(_abi.gg_outputs) == 72
 
 
+def test_python_mirror_matches_c_field_offsets(tmp_path):
+    """Every struct the ABI mirrors: size and each field's offset as gcc lays
+    out include/gargantua.h, against the ctypes mirror."""
+    import subprocess
+    structs = [v for v in vars(_abi).values()
+               if isinstance(v, type) and issubclass(v, C.Structure) and v.__name__.startswith("gg_")]
+    assert structs
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gargantua.h"', "int main(void) {"]
+    for st in structs:
+        lines.append(f'  printf("{st.__name__} sizeof %zu\\n", sizeof({st.__name__}));')
+        for f in st._fields_:
+            lines.append(f'  printf("{st.__name__} {f[0]} %zu\\n", offsetof({st.__name__}, {f[0]}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.dirname(HEADER), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {(a, b): int(c) for a, b, c in (l.split() for l in out if l)}
+    for st in structs:
+        assert got[(st.__name__, "sizeof")] == C.sizeof(st), st.__name__
+        for f in st._fields_:
+            assert got[(st.__name__, f[0])] == getattr(st, f[0]).offset, (st.__name__, f[0])
+
+
 def _rays(scene):
     lib = _abi.load()
     n = C.c_uint64()
