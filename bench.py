@@ -229,9 +229,9 @@ F_ADJ_STEP = 6 * F_VJP + 250
 def bench_grad512(args):
     """Differentiable render: 512x512 Kerr thin disk at theta0 = (a=0.5, i=45 deg, q=3, r_in=r_ISCO(0.5)+0.5),
     loss = MSE against the image at (0.7, 60 deg, 2.5, r_ISCO(0.7)); dL/dtheta by the adjoint kernels with
-    the render recording every step (K = 0, automatic), with the disk rays re-traced and every step stored
-    (K = 65536), and with checkpoints every K in {8, 32, 128} accepted steps; then gradient-descent (Adam)
-    recovery of theta."""
+    the render recording every step (K = 0, automatic; FP64, and mixed FP32/FP64 as the headline), with the
+    disk rays re-traced and every step stored (K = 65536), and with checkpoints every K in {8, 32, 128}
+    accepted steps; then gradient-descent (Adam, mixed forward) recovery of theta."""
     import torch
     import gargantua_amd as gg
     cfg = gg.configs
@@ -242,19 +242,20 @@ def bench_grad512(args):
     target_dev = torch.from_numpy(target).to("cuda")   # resident in HBM, read in place
     g0 = cfg.grad_512_params(size, size)
     per_k = {}
-    # (label, K, learned ray order for the render)
+    g0m = cfg.grad_512_params(size, size, precision="mixed")
+    # (label, K, learned ray order for the render, forward precision)
     # the headline run last, on a warm GPU
-    runs = [("auto", 0, 0), ("K65536", 65536, 0), ("K8", 8, 0), ("K32", 32, 0), ("K128", 128, 0),
-            ("auto+order", 0, 1)]
-    for K, (label, kk, order) in [(r[0], r) for r in runs]:
+    runs = [("auto", 0, 0, g0), ("K65536", 65536, 0, g0), ("K8", 8, 0, g0), ("K32", 32, 0, g0), ("K128", 128, 0, g0),
+            ("auto+order", 0, 1, g0), ("auto+order mixed", 0, 1, g0m)]
+    for K, (label, kk, order, gs) in [(r[0], r) for r in runs]:
         rend.set_launch(order=order)
         for _ in range(args.warmup):
-            rend.grad(g0, target_dev, checkpoint_every=kk, want_image=False)
+            rend.grad(gs, target_dev, checkpoint_every=kk, want_image=False)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         sts = []
         for _ in range(args.steps):
-            out = rend.grad(g0, target_dev, checkpoint_every=kk, want_image=False)
+            out = rend.grad(gs, target_dev, checkpoint_every=kk, want_image=False)
             sts.append(out["stats"])
         wall = time.perf_counter() - t0
         st = sts[-1]
@@ -274,7 +275,9 @@ def bench_grad512(args):
         _log(f"grad512 K={K}: {per_k[K]['ms_per_grad']:.1f} ms/grad (adjoint {adj_ms:.1f} ms, "
              f"forward {per_k[K]['forward_ms']:.1f} ms, mode {st['mode']})")
     rend.set_launch(order=1)
-    same = all(per_k[K]["grad"] == per_k["K8"]["grad"] for K in per_k)
+    same = all(per_k[K]["grad"] == per_k["K8"]["grad"] for K in per_k if "mixed" not in K)
+    mixed_rel = (np.abs(np.array(per_k["auto+order mixed"]["grad"]) - np.array(per_k["K8"]["grad"])) /
+                 np.abs(np.array(per_k["K8"]["grad"]))).max()
     # gradient-descent recovery (Adam on the raw parameters)
     gd = None
     if args.gd_steps > 0:
@@ -287,7 +290,7 @@ def bench_grad512(args):
         t0 = time.perf_counter()
         for it in range(1, args.gd_steps + 1):
             g = cfg.grad_512_params(size, size, spin=theta[0], inclination_deg=math.degrees(theta[1]), q=theta[2],
-                                    r_in=theta[3])
+                                    r_in=theta[3], precision="mixed")
             out = rend.grad(g, target_dev, checkpoint_every=0, want_image=False)
             grad = out["grad"]
             m = b1 * m + (1 - b1) * grad
@@ -305,22 +308,24 @@ def bench_grad512(args):
               "theta_true": truth.tolist(), "abs_err": np.abs(theta - truth).tolist(),
               "loss_first": hist[0], "loss_last": hist[-1]}
     rend.close()
-    best = per_k["auto+order"]
+    best = per_k["auto+order mixed"]
     return {"metric": f"adjoint gradient evaluations/s, {size}x{size} Kerr thin disk, dL/d(a, i, q, r_in)",
             "value": best["grads_per_s"], "unit": "gradients/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": best["ms_per_grad"], "higher_is_better": True, "dtype": "f64",
             "data": "synthetic (target image rendered at the true parameters)",
             "config": {"workload": f"{size}x{size} Kerr thin disk r_out 20M edge 0.25M, camera r=50M FOV 30deg, "
                                    "rtol 1e-8 atol 1e-10; theta0 (0.5, 45deg, 3.0, r_ISCO+0.5), target "
-                                   "(0.7, 60deg, 2.5, r_ISCO(0.7)); value at K=0 (the render records every "
-                                   "step), learned ray order, target resident on the device", "K": 0},
+                                   "(0.7, 60deg, 2.5, r_ISCO(0.7)); value: mixed FP32/FP64 forward render "
+                                   "recording every step (K=0), learned ray order, target resident on the device",
+                       "K": 0, "forward_precision": "mixed"},
             "roofline": {"bound": "valu", "achieved": best["adjoint_tflops"], "peak": PEAK_FP64, "unit": "TFLOP/s",
                          "frac": best["adjoint_frac_fp64_peak"],
                          "traffic": profiled_traffic("adjoint_grad512_counters.md", 1),
                          "traffic_source": "profiles/adjoint_grad512_counters.md, adjoint_rev_chunked_kernel "
                                            "(dominant kernel; reads 4.4 GB of step records)",
                          "kernel": "adjoint_seed_kernel + adjoint_rev_chunked_kernel + adjoint_cam_kernel"},
-            "per_K": per_k, "grad_identical_across_K": same, "gradient_descent": gd}
+            "per_K": per_k, "grad_identical_across_K": same, "mixed_forward_grad_max_rel_diff": float(mixed_rel),
+            "gradient_descent": gd}
 
 
 def main():

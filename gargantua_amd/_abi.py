@@ -81,7 +81,8 @@ class gg_grad_scene(C.Structure):
     _fields_ = [("mass", C.c_double), ("spin", C.c_double), ("inclination", C.c_double),
                 ("emissivity_q", C.c_double), ("r_inner", C.c_double), ("r_outer", C.c_double),
                 ("edge_width", C.c_double), ("r_camera", C.c_double), ("fov_y_radians", C.c_double),
-                ("integrator", gg_integrator), ("width", C.c_uint32), ("height", C.c_uint32)]
+                ("integrator", gg_integrator), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("precision", C.c_int32), ("_pad", C.c_uint32)]
 
 
 class gg_grad_stats(C.Structure):

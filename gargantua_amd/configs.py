@@ -98,8 +98,10 @@ def batch_64(seed: int = 2026, n: int = 64, resolutions=(128, 256, 512, 1024)):
     return scenes
 
 
-def grad_512_params(width: int = 512, height: int = 512, spin=0.5, inclination_deg=45.0, q=3.0, r_in=None):
-    """gg_grad_scene fields for the differentiable render (r_in defaults to r_ISCO(a) + 0.5M)."""
+def grad_512_params(width: int = 512, height: int = 512, spin=0.5, inclination_deg=45.0, q=3.0, r_in=None,
+                    precision: str = "fp64"):
+    """gg_grad_scene fields for the differentiable render (r_in defaults to r_ISCO(a) + 0.5M); precision of the
+    forward render, "fp64" or "mixed"."""
     from . import _abi
     if r_in is None:
         r_in = r_isco(1.0, spin) + 0.5
@@ -116,4 +118,5 @@ def grad_512_params(width: int = 512, height: int = 512, spin=0.5, inclination_d
     g.integrator = rtol_integrator(200.0, 20000).to_c()
     g.width = width
     g.height = height
+    g.precision = {"fp64": _abi.GG_PREC_FP64, "mixed": _abi.GG_PREC_MIXED}[precision]
     return g

@@ -864,6 +864,12 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
   auto t_start = std::chrono::steady_clock::now();
   if (!ctx || !scene || !target) return fail(GG_ERR_INVALID_ARG, "NULL context, scene or target");
   if (checkpoint_every > 65536) return fail(GG_ERR_INVALID_ARG, "checkpoint_every must be in [0, 65536]");
+  if (scene->precision != GG_PREC_FP64 && scene->precision != GG_PREC_MIXED)
+    return fail(GG_ERR_INVALID_ARG, "gg_grad_scene.precision must be GG_PREC_FP64 or GG_PREC_MIXED");
+  const bool mixed = scene->precision == GG_PREC_MIXED;
+  if (mixed && checkpoint_every != 0)
+    return fail(GG_ERR_UNSUPPORTED, "the mixed forward render needs checkpoint_every = 0 (recorded steps): its FP32 "
+                                    "steps depend on the lane grouping, so checkpoints cannot be recomputed");
   gg_scene sc;
   int rc = gg_grad_scene_to_scene(scene, &sc);
   if (rc) return rc;
@@ -930,7 +936,7 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
       o.state = (double *)B[20].p;
       ctx->rec_req = {(double *)B[18].p, pool_chunks, &d_cnt->rec_cursor, (uint32_t *)B[19].p};
     }
-    rc = render_impl(ctx, &sc, 1, GG_PREC_FP64, &o, nullptr, 0, &rs);
+    rc = render_impl(ctx, &sc, 1, record && mixed ? GG_PREC_MIXED : GG_PREC_FP64, &o, nullptr, 0, &rs);
     ctx->rec_req = {};
     if (rc) return rc;
 
@@ -955,6 +961,10 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
       break;
     }
   }
+  if (mixed && mode != 2)
+    return fail(GG_ERR_UNSUPPORTED, "the mixed forward render needs room for the recorded steps (%llu bytes of "
+                                    "chunks; budget %llu): raise gg_set_grad_memory or use GG_PREC_FP64",
+                (unsigned long long)(chunk_bytes * cnt.rec_cursor), (unsigned long long)budget);
   const uint64_t n_list = cnt.n_list;
   uint64_t chk_total = cnt.chk_total, tail_total = cnt.tail_total;
   if (mode == 2) {

@@ -320,7 +320,7 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
   bool done = false;
   if (PREC == 1 && use32) {
     const bool need_k1 = !L.k1_ok;
-    o = stages<METRIC, ARITH_F32>(S, K, p0, vel, h, need_k1);
+    o = stages<METRIC, ARITH_F32, REC>(S, K, p0, vel, h, need_k1, rec);
     L.n_rhs += o.nr;
     L.n_rhs32 += o.nr;
     if (need_k1) { L.k1_ok = true; L.k1_f32 = true; }   // slot 0 now holds an FP32 k1
@@ -332,7 +332,7 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
   if (PREC == 1 && !done && use_fast) {
     // fast FP64 needs an FP64 k1 (either arithmetic); an FP32 one is redone
     const bool need_k1 = !L.k1_ok || L.k1_f32;
-    o = stages<METRIC, ARITH_FAST>(S, K, p0, vel, h, need_k1);
+    o = stages<METRIC, ARITH_FAST, REC>(S, K, p0, vel, h, need_k1, rec);
     L.n_rhs += o.nr;
     L.n_rhsf += o.nr;
     if (need_k1) { L.k1_ok = true; L.k1_f32 = false; L.k1_fast = true; }

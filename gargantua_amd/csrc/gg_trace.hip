@@ -466,7 +466,7 @@ inline size_t lds_bytes(uint32_t block) { return (size_t)block * 5 * 4 * sizeof(
 
 template <int M, int D, int P>
 hipError_t launch3(const TraceLaunch &t, uint32_t grid, uint32_t block, hipStream_t s) {
-  if constexpr (M == 1 && D == 2 && P == 0) {
+  if constexpr (M == 1 && D == 2 && (P == 0 || P == 1)) {
     // the recording render of the adjoint: Kerr thin disk, FP64, one scene
     if (t.rec_pool) {
       hipLaunchKernelGGL((trace_kernel<M, D, P, 2, 0>), dim3(grid), dim3(block), lds_bytes(block), s, t);

@@ -285,6 +285,12 @@ typedef struct gg_grad_scene {
   gg_integrator integrator;
   uint32_t width;
   uint32_t height;
+  int32_t precision;       /* forward render: GG_PREC_FP64 (0, default) or
+                              GG_PREC_MIXED, which needs checkpoint_every = 0
+                              and room for the recorded steps (the mixed
+                              kernel's steps depend on the lane grouping, so
+                              only the render's own records are its path) */
+  uint32_t _pad;
 } gg_grad_scene;
 
 /* The gg_scene gg_grad renders: Kerr(mass, spin), camera at

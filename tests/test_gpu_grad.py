@@ -195,6 +195,23 @@ def test_grad_learned_order_and_device_target(renderer, oracle):
         t_dev.free()
 
 
+def test_grad_mixed_forward(renderer, oracle):
+    """The mixed-precision forward render (FP32 stages where the trace
+    kernel's policy allows, the render's own records for the reverse sweep):
+    loss and gradient within 1e-6 of the FP64 forward's; it needs the
+    recorded mode, so an explicit checkpoint interval is rejected."""
+    target = _target(oracle)
+    a = renderer.grad(_scene(), target, want_dI=True)
+    b = renderer.grad(_scene(precision="mixed"), target, want_dI=True)
+    assert b["stats"]["mode"] == 2
+    rel = np.abs(b["grad"] - a["grad"]) / np.abs(a["grad"])
+    print("fp64", a["grad"], "mixed", b["grad"], "rel", rel)
+    assert np.all(rel < 1e-6)
+    assert abs(b["loss"] - a["loss"]) <= 1e-6 * a["loss"]
+    with pytest.raises(gg.GargantuaError):
+        renderer.grad(_scene(precision="mixed"), target, checkpoint_every=8)
+
+
 def test_grad_record_pool_grows(oracle):
     """A fresh context sized by a small scene meets a larger one: the record
     pool overflows, is resized from the count of chunks the render asked for,
