@@ -37,17 +37,22 @@ PEAK_FP64 = 78.6    # TFLOP/s, MI355X vector FP64 (published)
 PEAK_FP32 = 157.3   # TFLOP/s, MI355X vector FP32 (published; needs packed math)
 
 
-def profiled_traffic(profile: str, column: int):
-    """HBM bytes per launch of a kernel from a committed counter summary (FETCH_SIZE and WRITE_SIZE passes of
-    rocprofv3, scaled by a 512 MiB copy), or None when the summary is absent."""
+def profiled_traffic(profile: str, kernel: str):
+    """HBM bytes per launch of `kernel` from a committed counter summary (FETCH_SIZE and WRITE_SIZE passes of
+    rocprofv3, scaled by a 512 MiB copy), or None when the summary or the kernel is absent."""
     path = os.path.join(ROOT, "profiles", profile)
     try:
+        col = None
         with open(path) as f:
             for line in f:
-                if line.startswith("| HBM traffic per launch (calibrated, MB)"):
-                    return float(line.split("|")[2 + column]) * 1e6
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if cells and cells[0].startswith("derived"):
+                    col = next((i for i, c in enumerate(cells) if c == f"`{kernel}`"), None)
+                elif col is not None and cells[0] == "HBM traffic per launch (calibrated, MB)":
+                    return float(cells[col]) * 1e6
     except (OSError, ValueError, IndexError):
         return None
+    return None
     return None
 
 METRIC = "geodesics/s Kerr a=0.9 1024²: mixed FP32 vs FP64 at ≤1e-6 err; adjoint grads/s"
@@ -320,7 +325,7 @@ def bench_grad512(args):
                        "K": 0, "forward_precision": "mixed"},
             "roofline": {"bound": "valu", "achieved": best["adjoint_tflops"], "peak": PEAK_FP64, "unit": "TFLOP/s",
                          "frac": best["adjoint_frac_fp64_peak"],
-                         "traffic": profiled_traffic("adjoint_grad512_counters.md", 1),
+                         "traffic": profiled_traffic("adjoint_grad512_counters.md", "gg::adjoint_rev_chunked_kernel"),
                          "traffic_source": "profiles/adjoint_grad512_counters.md, adjoint_rev_chunked_kernel "
                                            "(dominant kernel; reads 4.4 GB of step records)",
                          "kernel": "adjoint_seed_kernel + adjoint_rev_chunked_kernel + adjoint_cam_kernel"},
@@ -424,7 +429,7 @@ def main():
                                "attempt counts (learned in the warm-up), every ray traced in full",
                    "first_frame_tile_order_ms": tile_ms},
         "roofline": {"bound": "valu", "achieved": tflops, "peak": peak, "unit": "TFLOP/s", "frac": tflops / peak,
-                     "traffic": profiled_traffic("trace_kernel_kerr1024_counters.md", 0),
+                     "traffic": profiled_traffic("trace_kernel_kerr1024_counters.md", "gg::trace_kernel<1, 2, 1, 0, 0>"),
                      "traffic_source": "profiles/trace_kernel_kerr1024_counters.md (rocprofv3 FETCH_SIZE and "
                                        "WRITE_SIZE, copy-calibrated); algorithmic outputs 47 MB",
                      "kernel": "trace_kernel<Kerr, thin, mixed>",
