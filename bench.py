@@ -173,7 +173,7 @@ def bench_batch64(args):
             _log(f"batch64 {prec}: {st['kernel_ms']:.1f} ms")
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        res[prec] = {"value": n * args.steps / wall, "kernel_ms": float(np.mean(kms)),
+        res[prec] = {"value": n * args.steps / wall, "kernel_ms": float(np.median(kms)),
                      "attempts_per_ray": st["attempts"] / n, "fp32_rhs_share": st["rhs_evals_fp32"] / st["rhs_evals"]}
     rend.close()
     return {"metric": "geodesics/s, 64 seeded Kerr scenes in one graph-captured launch", "value": res["mixed"]["value"],
@@ -410,12 +410,12 @@ def main():
 
     value = n * args.steps / wall_m
     ms_step = 1e3 * wall_m / args.steps
-    k_ms = float(np.mean(kms_m))
+    k_ms = float(np.median(kms_m))   # median of the timed launches
     f32, f64 = flops(st_m, "kerr")
     tflops = (f32 + f64) / (k_ms * 1e-3) / 1e12
     peak = (f32 + f64) / (f32 / PEAK_FP32 + f64 / PEAK_FP64) if (f32 + f64) else PEAK_FP64
     d32, d64 = flops(st_d, "kerr")
-    k_ms_d = float(np.mean(kms_d))
+    k_ms_d = float(np.median(kms_d))
 
     line = {
         "metric": METRIC, "value": value, "unit": "geodesics/s", "n_gpus": 1, "steps": args.steps,
@@ -437,7 +437,7 @@ def main():
         "fp64": {"value": n * args.steps / wall_d, "kernel_ms": k_ms_d,
                  "roofline_frac": (d32 + d64) / (k_ms_d * 1e-3) / 1e12 / PEAK_FP64,
                  "mixed_speedup": k_ms_d / k_ms},
-        "fp64fast": {"value": n * args.steps / wall_f, "kernel_ms": float(np.mean(kms_f)),
+        "fp64fast": {"value": n * args.steps / wall_f, "kernel_ms": float(np.median(kms_f)),
                      "vs_fp64": {k: cmp_f[k] for k in ("class_agree", "err_le_1e-6", "attempts_equal")}},
         "mixed_vs_fp64": {k: cmp[k] for k in ("class_agree", "class_agree_outside_band", "err_le_1e-6",
                                              "err_le_1e-6_outside_band", "attempts_equal", "psnr_intensity",

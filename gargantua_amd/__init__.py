@@ -8,6 +8,7 @@ C ABI of include/gargantua.h):
   Camera                         crates/raytracer/src/curved/camera.rs:9-15
   AccretionDisk                  crates/raytracer/src/curved/disk.rs:8-31
   RenderOptions                  crates/raytracer/src/curved/renderer.rs:17-26
+  HdriSky                        crates/raytracer/src/curved/sky.rs:43-122
   render, render_with_disk       crates/raytracer/src/curved/renderer.rs:82-101
   Renderer.trace_states          trace_ray / trace_ray_with_disk (tracer.rs:54-152)
   Renderer.geodesic_acceleration Metric::geodesic_acceleration (metric.rs:37-52)
@@ -133,11 +134,55 @@ class AccretionDisk:
                             emissivity_q=self.emissivity_q, edge_width=self.edge_width)
 
 
+class HdriSky:
+    """HdriSky (sky.rs:43-122): equirectangular linear-RGB environment map.
+
+    Texel (0, 0) is the +z pole; rows run to the -z pole and columns wrap in
+    azimuth.  The map is sampled on the GPU (bilinear in f32, wrap-x /
+    clamp-y, times intensity) for every escaped ray of a render with rgb
+    output once it is set on a Renderer or passed in RenderOptions.sky.
+    """
+
+    def __init__(self, pixels: np.ndarray, intensity: float = 1.0):
+        px = np.ascontiguousarray(pixels, dtype=np.float32)
+        if px.ndim != 3 or px.shape[2] != 3 or px.shape[0] == 0 or px.shape[1] == 0:
+            raise ValueError(f"HdriSky pixels must be (height, width, 3), got {px.shape}")
+        self.pixels = px
+        self.intensity = float(intensity)
+
+    @property
+    def width(self) -> int:
+        return self.pixels.shape[1]
+
+    @property
+    def height(self) -> int:
+        return self.pixels.shape[0]
+
+    @staticmethod
+    def from_pixels(width: int, height: int, pixels) -> "HdriSky":
+        """HdriSky::from_pixels (sky.rs:72-77): row-major [r, g, b] texels, row 0 = +z pole."""
+        px = np.asarray(pixels, dtype=np.float32)
+        if px.size != width * height * 3:
+            raise ValueError("pixel buffer size mismatch")
+        return HdriSky(px.reshape(height, width, 3))
+
+    @staticmethod
+    def from_file(path) -> "HdriSky":
+        """HdriSky::from_file (sky.rs:57-70) for Radiance .hdr files."""
+        from . import hdr
+        return HdriSky(hdr.read(path))
+
+    def with_intensity(self, intensity: float) -> "HdriSky":
+        """HdriSky::with_intensity (sky.rs:79-83)."""
+        return HdriSky(self.pixels, intensity)
+
+
 @dataclass
 class RenderOptions:
-    """RenderOptions (renderer.rs:17-26); the sky is the procedural starfield."""
+    """RenderOptions (renderer.rs:17-26); sky None is the procedural starfield."""
     samples_per_axis: int = 1
     show_progress: bool = False
+    sky: Optional[HdriSky] = None
 
 
 @dataclass
@@ -356,6 +401,15 @@ class Renderer:
                "gg_geodesic_acceleration_vjp")
         return {"pos_bar": pb, "vel_bar": vb, "spin_bar": sb}
 
+    def set_sky_hdri(self, sky: Optional[HdriSky]) -> None:
+        """Sky of every following render with rgb output (None: the procedural starfield)."""
+        if sky is None:
+            _check(self.lib, self.lib.gg_set_sky_hdri(self.ctx, 0, 0, None, 1.0), "gg_set_sky_hdri")
+            return
+        px = np.ascontiguousarray(sky.pixels, dtype=np.float32)
+        _check(self.lib, self.lib.gg_set_sky_hdri(self.ctx, px.shape[1], px.shape[0], px.ctypes.data,
+                                                  sky.intensity), "gg_set_sky_hdri")
+
     def set_grad_memory(self, nbytes: int = 0) -> None:
         """Device memory grad()'s automatic mode may use for step records (0: a quarter of the free memory)."""
         _check(self.lib, self.lib.gg_set_grad_memory(self.ctx, int(nbytes)), "gg_set_grad_memory")
@@ -425,14 +479,20 @@ def _reference_render_scene(metric, camera, disk, width, height, opts) -> Scene:
 
 def render(metric, camera: Camera, width: int, height: int, opts: Optional[RenderOptions] = None) -> np.ndarray:
     """raytracer::curved::render (renderer.rs:82-90): returns an (height, width, 3) uint8 image."""
-    sc = _reference_render_scene(metric, camera, None, width, height, opts or RenderOptions())
-    o = _renderer().render(sc, "fp64", rgb=True)
+    opts = opts or RenderOptions()
+    sc = _reference_render_scene(metric, camera, None, width, height, opts)
+    r = _renderer()
+    r.set_sky_hdri(opts.sky)
+    o = r.render(sc, "fp64", rgb=True)
     return o["rgb"].reshape(height, width, 3)
 
 
 def render_with_disk(metric, camera: Camera, disk: AccretionDisk, width: int, height: int,
                      opts: Optional[RenderOptions] = None) -> np.ndarray:
     """raytracer::curved::render_with_disk (renderer.rs:92-101)."""
-    sc = _reference_render_scene(metric, camera, disk, width, height, opts or RenderOptions())
-    o = _renderer().render(sc, "fp64", rgb=True)
+    opts = opts or RenderOptions()
+    sc = _reference_render_scene(metric, camera, disk, width, height, opts)
+    r = _renderer()
+    r.set_sky_hdri(opts.sky)
+    o = r.render(sc, "fp64", rgb=True)
     return o["rgb"].reshape(height, width, 3)

@@ -103,6 +103,7 @@ EXPORTS = [
     ("gg_context_destroy", C.c_int, [_P]),
     ("gg_set_launch", C.c_int, [_P, C.POINTER(gg_launch)]),
     ("gg_set_grad_memory", C.c_int, [_P, C.c_uint64]),
+    ("gg_set_sky_hdri", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_float]),
     ("gg_get_launch", C.c_int, [_P, C.POINTER(gg_launch)]),
     ("gg_scene_rays", C.c_int, [C.POINTER(gg_scene), C.POINTER(C.c_uint64)]),
     ("gg_render", C.c_int, [_P, C.POINTER(gg_scene), C.c_int, C.POINTER(gg_outputs), C.POINTER(gg_stats)]),

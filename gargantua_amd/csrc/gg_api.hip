@@ -104,6 +104,11 @@ struct gg_context {
   // render (grad[18..20]); chunks the last recording needed
   uint64_t grad_chunks = 0;
   uint64_t grad_budget = 0;   // gg_set_grad_memory (0: a quarter of the free memory)
+  // gg_set_sky_hdri: equirectangular environment map for rgb outputs
+  // (sky_w = 0: the procedural starfield)
+  DevBuf sky;
+  uint32_t sky_w = 0, sky_h = 0;
+  float sky_intensity = 1.0f;
   // the recording render gg_grad asks render_impl for (pool null: none)
   struct {
     double *pool;
@@ -440,6 +445,27 @@ int gg_set_launch(gg_context *ctx, const gg_launch *l) {
   return GG_OK;
 }
 
+int gg_set_sky_hdri(gg_context *ctx, uint32_t width, uint32_t height, const float *pixels, float intensity) {
+  if (!ctx) return fail(GG_ERR_INVALID_ARG, "NULL context");
+  if (width == 0) {
+    ctx->sky_w = ctx->sky_h = 0;
+    return GG_OK;
+  }
+  if (height == 0 || !pixels) return fail(GG_ERR_INVALID_ARG, "HDRI sky needs height > 0 and pixels");
+  if ((uint64_t)width * height > (1ull << 28)) return fail(GG_ERR_INVALID_ARG, "HDRI sky larger than 2^28 texels");
+  if (!std::isfinite(intensity)) return fail(GG_ERR_INVALID_ARG, "HDRI sky intensity must be finite");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t bytes = 12ull * width * height;
+  int rc = ctx->sky.ensure(bytes);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(ctx->sky.p, pixels, bytes, hipMemcpyDefault, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->sky_w = width;
+  ctx->sky_h = height;
+  ctx->sky_intensity = intensity;
+  return GG_OK;
+}
+
 int gg_set_grad_memory(gg_context *ctx, uint64_t bytes) {
   if (!ctx) return fail(GG_ERR_INVALID_ARG, "NULL context");
   ctx->grad_budget = bytes;
@@ -607,8 +633,9 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
     if (plans[i][8].dev) {
       const DevOut *dop = (const DevOut *)ctx->outs.p + i;
       const DevScene *dsp = (const DevScene *)ctx->scenes.p + i;
+      const SkyDesc sky{ctx->sky_w ? (const float *)ctx->sky.p : nullptr, ctx->sky_w, ctx->sky_h, ctx->sky_intensity};
       HIP_TRY(launch_shade(dsp, dop, scenes[i].image.width, scenes[i].image.height,
-                           scenes[i].image.samples_per_axis, (uint8_t *)plans[i][8].dev, s));
+                           scenes[i].image.samples_per_axis, sky, (uint8_t *)plans[i][8].dev, s));
     }
   }
   if (learn_order) {

@@ -138,8 +138,16 @@ hipError_t launch_camera_rays(const DevScene *scene, uint64_t n, double *states,
 // ids[i] = i: input of the cost-order sort.
 hipError_t launch_order_keys(const DevScene *scenes, const DevOut *outs, uint32_t n_scenes, uint64_t n,
                              uint32_t *keys, uint32_t *ids, hipStream_t stream);
+// Equirectangular environment map on the device (HdriSky, sky.rs:43-122),
+// or pixels null for the procedural starfield.
+struct SkyDesc {
+  const float *pixels;             // width x height x 3, row 0 = +z pole
+  uint32_t width, height;
+  float intensity;
+};
+
 // RGB image (renderer.rs:44-80 shading + sRGB) from per-ray outputs.
 hipError_t launch_shade(const DevScene *scene, const DevOut *out, uint32_t width, uint32_t height, uint32_t spa,
-                        uint8_t *rgb, hipStream_t stream);
+                        const SkyDesc &sky, uint8_t *rgb, hipStream_t stream);
 
 }  // namespace gg

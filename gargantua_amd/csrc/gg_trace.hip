@@ -412,6 +412,37 @@ __device__ inline void sky(double d0, double d1, double d2, double rgb[3]) {
   }
 }
 
+// HdriSky::sample + fetch (sky.rs:88-122): equirectangular map with row 0
+// at the +z pole, theta = acos(z), phi = atan2(y, x), u = phi/(2 pi) + 1/2,
+// v = theta/pi; bilinear in f32 with wrap-x / clamp-y, times intensity.
+__device__ inline void sky_hdri(const SkyDesc &K, double d0, double d1, double d2, double rgb[3]) {
+#pragma clang fp contract(off)
+  const double FRAC_1_PI = 0.31830988618379067154;
+  double z = d2 < -1.0 ? -1.0 : (d2 > 1.0 ? 1.0 : d2);
+  const double theta = acos(z);
+  const double phi = atan2(d1, d0);
+  const double u = phi * FRAC_1_PI * 0.5 + 0.5;
+  const double v = theta * FRAC_1_PI;
+  const double w = (double)K.width, h = (double)K.height;
+  double fx = fmod(u * w, w);                          // rem_euclid
+  if (fx < 0.0) fx = fx + fabs(w);
+  double fy = v * h;
+  fy = fy < 0.0 ? 0.0 : (fy > h - 1.0 ? h - 1.0 : fy);
+  const uint32_t x0 = (uint32_t)floor(fx) % K.width;
+  const uint32_t y0 = (uint32_t)floor(fy);
+  const uint32_t x1 = (x0 + 1) % K.width;
+  const uint32_t y1 = y0 + 1 < K.height ? y0 + 1 : K.height - 1;
+  const float tx = (float)(fx - floor(fx)), ty = (float)(fy - floor(fy));
+  const float *p00 = K.pixels + 3 * ((uint64_t)y0 * K.width + x0), *p10 = K.pixels + 3 * ((uint64_t)y0 * K.width + x1);
+  const float *p01 = K.pixels + 3 * ((uint64_t)y1 * K.width + x0), *p11 = K.pixels + 3 * ((uint64_t)y1 * K.width + x1);
+  for (int c = 0; c < 3; ++c) {
+    const float a = p00[c] + (p10[c] - p00[c]) * tx;
+    const float b = p01[c] + (p11[c] - p01[c]) * tx;
+    const float px = a + (b - a) * ty;
+    rgb[c] = (double)(px * K.intensity);
+  }
+}
+
 __device__ inline uint8_t to_u8(double c) {
 #pragma clang fp contract(off)
   if (c < 0.0) c = 0.0;
@@ -426,7 +457,7 @@ __device__ inline uint8_t to_u8(double c) {
 // render_row (renderer.rs:309-345) shading: average of the spa^2 linear
 // colours (shade_outcome_with, disk_color) then encode_srgb.
 __global__ void shade_kernel(const DevScene *Sp, const DevOut *Op, uint32_t width, uint32_t height, uint32_t spa,
-                             uint8_t *rgb) {
+                             SkyDesc skyd, uint8_t *rgb) {
 #pragma clang fp contract(off)
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= width * height) return;
@@ -447,7 +478,7 @@ __global__ void shade_kernel(const DevScene *Sp, const DevOut *Op, uint32_t widt
         double nn = sqrt(v1 * v1 + v2 * v2 + v3 * v3);
         double d0 = 1.0, d1 = 0.0, d2 = 0.0;
         if (nn > 0.0) { d0 = v1 / nn; d1 = v2 / nn; d2 = v3 / nn; }
-        sky(d0, d1, d2, lin);
+        if (skyd.pixels) sky_hdri(skyd, d0, d1, d2, lin); else sky(d0, d1, d2, lin);
       } else if (c == GG_CLASS_DISK) {
         double I = O.intensity[k];
         double x = I / (1.0 + I);
@@ -552,9 +583,10 @@ hipError_t launch_order_keys(const DevScene *scenes, const DevOut *outs, uint32_
 }
 
 hipError_t launch_shade(const DevScene *scene, const DevOut *out, uint32_t width, uint32_t height, uint32_t spa,
-                        uint8_t *rgb, hipStream_t stream) {
+                        const SkyDesc &sky, uint8_t *rgb, hipStream_t stream) {
   uint32_t n = width * height;
-  hipLaunchKernelGGL(shade_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, scene, out, width, height, spa, rgb);
+  hipLaunchKernelGGL(shade_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, scene, out, width, height, spa, sky,
+                     rgb);
   return hipGetLastError();
 }
 

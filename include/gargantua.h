@@ -347,6 +347,16 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target,
             uint32_t checkpoint_every, double *loss, double grad[4],
             double *intensity, double *d_intensity, gg_grad_stats *stats);
 
+/* Equirectangular environment map for the escaped rays of every following
+ * render with rgb output on this context (RenderOptions.sky = HdriSky,
+ * sky.rs:43-122): width x height linear RGB f32 texels, row-major, row 0 the
+ * +z pole, columns wrapping in azimuth; bilinear with wrap-x / clamp-y,
+ * scaled by intensity.  pixels (host or device) are copied.  width = 0
+ * restores the procedural starfield (ProceduralSky, the reference's
+ * default); this call is the library's counterpart of RenderOptions.sky. */
+int gg_set_sky_hdri(gg_context *ctx, uint32_t width, uint32_t height,
+                    const float *pixels, float intensity);
+
 /* Device memory gg_grad's automatic mode (checkpoint_every = 0) may use for
  * step records; 0 (the default): a quarter of the free device memory.  The
  * choice of mode follows it; the gradient does not depend on it. */

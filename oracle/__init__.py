@@ -36,6 +36,8 @@ _SIGS = [
     ("oracle_disk_color", None, [_D, _P]),
     ("oracle_encode_srgb", None, [_P, _P]),
     ("oracle_sky", None, [_P, _P]),
+    ("oracle_set_sky_hdri", C.c_int, [C.c_uint32, C.c_uint32, _P, C.c_float]),
+    ("oracle_hdri_sample", None, [C.c_uint32, C.c_uint32, _P, C.c_float, _P, _P]),
     ("oracle_splitmix64", C.c_uint64, [C.c_uint64, C.c_uint64]),
     ("oracle_render_rays", C.c_int, [C.POINTER(_abi.gg_scene), C.c_uint64, _P, C.POINTER(_abi.gg_outputs), C.c_int]),
     ("oracle_render", C.c_int, [C.POINTER(_abi.gg_scene), C.POINTER(_abi.gg_outputs), C.c_int]),
@@ -174,6 +176,23 @@ class Oracle:
         out = np.zeros(3)
         self.lib.oracle_sky(d.ctypes.data, out.ctypes.data)
         return out
+
+    def hdri_sample(self, pixels, d, intensity=1.0):
+        """HdriSky::sample for pixels [height, width, 3] (f32, row 0 = +z pole)."""
+        px = np.ascontiguousarray(pixels, dtype=np.float32)
+        d = _f64(d, 3)
+        out = np.zeros(3)
+        self.lib.oracle_hdri_sample(px.shape[1], px.shape[0], px.ctypes.data, intensity, d.ctypes.data,
+                                    out.ctypes.data)
+        return out
+
+    def set_sky_hdri(self, pixels=None, intensity=1.0):
+        """Sky of render() (None: the procedural sky)."""
+        if pixels is None:
+            self.lib.oracle_set_sky_hdri(0, 0, None, 1.0)
+            return
+        px = np.ascontiguousarray(pixels, dtype=np.float32)
+        assert self.lib.oracle_set_sky_hdri(px.shape[1], px.shape[0], px.ctypes.data, intensity) == 0
 
     def splitmix64(self, seed, n):
         return self.lib.oracle_splitmix64(seed, n)

@@ -298,6 +298,44 @@ void sky_sample(const double dir[3], double rgb[3]) {
   }
 }
 
+// HdriSky (sky.rs:43-122): equirectangular map, row 0 the +z pole, columns
+// wrapping in azimuth, bilinear in f32 with wrap-x / clamp-y (fetch,
+// :88-106), times intensity; sample (:108-122) maps theta = acos(z) to
+// v = theta / pi and phi = atan2(y, x) to u = phi / (2 pi) + 1/2.  Empty:
+// the procedural sky.
+struct HdriSkyT {
+  uint32_t width = 0, height = 0;
+  std::vector<float> pixels;   // width * height * 3
+  float intensity = 1.0f;
+};
+HdriSkyT g_hdri;
+
+void hdri_sample(const HdriSkyT &k, const double dir[3], double rgb[3]) {
+  const double FRAC_1_PI = 0.31830988618379067154;
+  double z = dir[2] < -1.0 ? -1.0 : (dir[2] > 1.0 ? 1.0 : dir[2]);   // f64::clamp
+  double theta = std::acos(z);
+  double phi = std::atan2(dir[1], dir[0]);
+  double u = phi * FRAC_1_PI * 0.5 + 0.5;
+  double v = theta * FRAC_1_PI;
+  double w = (double)k.width, h = (double)k.height;
+  double fx = std::fmod(u * w, w);                                     // rem_euclid
+  if (fx < 0.0) fx = fx + std::fabs(w);
+  double fy = v * h;
+  fy = fy < 0.0 ? 0.0 : (fy > h - 1.0 ? h - 1.0 : fy);
+  size_t x0 = (size_t)std::floor(fx) % k.width;
+  size_t y0 = (size_t)std::floor(fy);
+  size_t x1 = (x0 + 1) % k.width;
+  size_t y1 = std::min<size_t>(y0 + 1, k.height - 1);
+  float tx = (float)(fx - std::floor(fx)), ty = (float)(fy - std::floor(fy));
+  auto at = [&](size_t y, size_t x, int c) { return k.pixels[3 * (y * k.width + x) + c]; };
+  for (int c = 0; c < 3; ++c) {
+    float a = at(y0, x0, c) + (at(y0, x1, c) - at(y0, x0, c)) * tx;
+    float b = at(y1, x0, c) + (at(y1, x1, c) - at(y1, x0, c)) * tx;
+    float p = a + (b - a) * ty;
+    rgb[c] = (double)(p * k.intensity);
+  }
+}
+
 // spatial_dir (tracer.rs:185-189).
 void spatial_dir(const double vel[4], double out[3]) {
   double s[3] = {vel[1], vel[2], vel[3]};
@@ -323,11 +361,17 @@ uint8_t to_u8(double c) {
   return (uint8_t)r;
 }
 
-// shade_outcome_with (renderer.rs:44-53) with the procedural sky.
+// shade_outcome_with (renderer.rs:44-53) with the HDRI sky when one is set,
+// else the procedural sky.
 void shade(uint8_t cls, double intensity, const double vel[4], double lin[3]) {
   switch (cls) {
     case GG_CLASS_HORIZON: lin[0] = lin[1] = lin[2] = 0.0; break;
-    case GG_CLASS_ESCAPED: { double d[3]; spatial_dir(vel, d); sky_sample(d, lin); break; }
+    case GG_CLASS_ESCAPED: {
+      double d[3];
+      spatial_dir(vel, d);
+      if (g_hdri.width) hdri_sample(g_hdri, d, lin); else sky_sample(d, lin);
+      break;
+    }
     case GG_CLASS_DISK: disk_color(intensity, lin); break;
     default: lin[0] = 0.0; lin[1] = 0.0; lin[2] = 10.0 / 255.0; break;
   }
@@ -530,6 +574,23 @@ void oracle_encode_srgb(const double lin[3], uint8_t out[3]) {
   for (int i = 0; i < 3; ++i) out[i] = to_u8(lin[i]);
 }
 void oracle_sky(const double dir[3], double rgb[3]) { sky_sample(dir, rgb); }
+// The sky of oracle_render (width = 0: the procedural sky).
+int oracle_set_sky_hdri(uint32_t width, uint32_t height, const float *pixels, float intensity) {
+  if (width == 0) { g_hdri = HdriSkyT{}; return 0; }
+  if (height == 0 || !pixels) return GG_ERR_INVALID_ARG;
+  g_hdri.width = width;
+  g_hdri.height = height;
+  g_hdri.pixels.assign(pixels, pixels + 3ull * width * height);
+  g_hdri.intensity = intensity;
+  return 0;
+}
+void oracle_hdri_sample(uint32_t width, uint32_t height, const float *pixels, float intensity, const double dir[3],
+                        double rgb[3]) {
+  HdriSkyT k;
+  k.width = width; k.height = height; k.intensity = intensity;
+  k.pixels.assign(pixels, pixels + 3ull * width * height);
+  hdri_sample(k, dir, rgb);
+}
 uint64_t oracle_splitmix64(uint64_t seed, uint64_t n) { return splitmix64(seed, n); }
 
 // Trace the rays of scene s listed in idx[0..n) (ray index = sub_py *
