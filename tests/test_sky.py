@@ -257,3 +257,55 @@ def test_gpu_hdri_bad_args(renderer):
     assert lib.gg_set_sky_hdri(ctx, 2, 2, None, 1.0) != 0
     assert lib.gg_set_sky_hdri(ctx, 2, 2, px.ctypes.data, float("inf")) != 0
     assert lib.gg_set_sky_hdri(None, 0, 0, None, 1.0) != 0
+
+
+# --- PNG output of the RgbImage ----------------------------------------------------------
+
+def test_png_known_answers():
+    from gargantua_amd import png
+    b = png.encode(np.zeros((1, 1, 3), np.uint8))
+    assert b[:8] == b"\x89PNG\r\n\x1a\n"
+    # The IEND chunk of every PNG: zero length, type, CRC-32 of "IEND" = AE 42 60 82.
+    assert b[-12:] == b"\x00\x00\x00\x00IEND\xaeB`\x82"
+    # IHDR of a 1x1 8-bit RGB image, with its CRC-32 90 77 53 DE.
+    assert b[8:33] == bytes.fromhex("0000000d494844520000000100000001080200000090775" "3de")
+
+
+def test_png_round_trip_and_filters(tmp_path):
+    import struct
+    import zlib
+    from gargantua_amd import png
+    rng = np.random.default_rng(3)
+    img = rng.integers(0, 256, (9, 13, 3), dtype=np.uint8)
+    p = tmp_path / "x.png"
+    png.write(p, img)
+    np.testing.assert_array_equal(png.read(p), img)
+    # Scanlines stored with filters 1..4 (sub, up, average, Paeth) decode to the same image.
+    h, w = img.shape[:2]
+    flat = img.reshape(h, 3 * w).astype(np.int32)
+    rows = []
+    for y in range(h):
+        f = 1 + y % 4
+        prev = flat[y - 1] if y else np.zeros(3 * w, np.int32)
+        enc = np.zeros(3 * w, np.int32)
+        for x in range(3 * w):
+            a = flat[y, x - 3] if x >= 3 else 0
+            c = prev[x - 3] if x >= 3 else 0
+            b = prev[x]
+            if f == 1:
+                pred = a
+            elif f == 2:
+                pred = b
+            elif f == 3:
+                pred = (a + b) // 2
+            else:
+                pp = a + b - c
+                pa, pb, pc = abs(pp - a), abs(pp - b), abs(pp - c)
+                pred = a if (pa <= pb and pa <= pc) else (b if pb <= pc else c)
+            enc[x] = (flat[y, x] - pred) & 0xFF
+        rows.append(bytes([f]) + enc.astype(np.uint8).tobytes())
+    ihdr = struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0)
+    body = png._chunk(b"IHDR", ihdr) + png._chunk(b"IDAT", zlib.compress(b"".join(rows))) + png._chunk(b"IEND", b"")
+    np.testing.assert_array_equal(png.decode(png.SIGNATURE + body), img)
+    with pytest.raises(ValueError):
+        png.decode(png.SIGNATURE + body[:-1] + b"\x00")          # IEND CRC broken
