@@ -9,6 +9,11 @@ C ABI of include/gargantua.h):
   AccretionDisk                  crates/raytracer/src/curved/disk.rs:8-31
   RenderOptions                  crates/raytracer/src/curved/renderer.rs:17-26
   HdriSky                        crates/raytracer/src/curved/sky.rs:43-122
+  Sphere, Quad, World            crates/raytracer/src/{shape/sphere.rs,shape/quad.rs,scene/world.rs}
+  Lambertian, Metal, Dielectric, crates/raytracer/src/material/*.rs
+  DiffuseLight
+  render_with_scene,             crates/raytracer/src/curved/renderer.rs:103-254
+  render_with_disk_and_scene
   render, render_with_disk       crates/raytracer/src/curved/renderer.rs:82-101
   Renderer.trace_states          trace_ray / trace_ray_with_disk (tracer.rs:54-152)
   Renderer.geodesic_acceleration Metric::geodesic_acceleration (metric.rs:37-52)
@@ -27,7 +32,7 @@ import numpy as np
 
 from . import _abi
 from . import configs  # noqa: F401  (scene builders for the benchmark configurations)
-from ._abi import (GG_CLASS_DISK, GG_CLASS_ESCAPED, GG_CLASS_HORIZON, GG_CLASS_MAX_STEPS,  # noqa: F401
+from ._abi import (GG_CLASS_DISK, GG_CLASS_ESCAPED, GG_CLASS_HORIZON, GG_CLASS_MAX_STEPS, GG_CLASS_SCENE,  # noqa: F401
                    GG_DISK_NONE, GG_DISK_THIN, GG_DISK_VOLUMETRIC, GG_ERR_REFERENCE, GG_ERR_RTOL_ATOL,
                    GG_METRIC_KERR, GG_METRIC_SCHWARZSCHILD)
 
@@ -177,6 +182,104 @@ class HdriSky:
         return HdriSky(self.pixels, intensity)
 
 
+# ---- Hittable scene: materials, shapes, world ----
+
+@dataclass
+class Lambertian:
+    """Lambertian (material/lambertian.rs:9-18): scatters with attenuation `albedo`."""
+    albedo: Sequence[float]
+
+
+@dataclass
+class Metal:
+    """Metal (material/metal.rs:7-19); fuzz is clamped to [0, 1] and only 0 is supported."""
+    albedo: Sequence[float]
+    fuzz: float = 0.0
+
+
+@dataclass
+class Dielectric:
+    """Dielectric (material/dielectric.rs): white attenuation; ior does not enter one-bounce shading."""
+    ior: float = 1.5
+
+
+@dataclass
+class DiffuseLight:
+    """DiffuseLight (material/diffuse_light.rs:6-20): emits `emit`, scatters nothing."""
+    emit: Sequence[float]
+
+    @staticmethod
+    def white(intensity: float) -> "DiffuseLight":
+        return DiffuseLight((intensity, intensity, intensity))
+
+
+def _material_fields(m) -> tuple:
+    if isinstance(m, Lambertian):
+        return _abi.GG_MATERIAL_LAMBERTIAN, m.albedo, 0.0, 1.0
+    if isinstance(m, Metal):
+        return _abi.GG_MATERIAL_METAL, m.albedo, m.fuzz, 1.0
+    if isinstance(m, Dielectric):
+        return _abi.GG_MATERIAL_DIELECTRIC, (1.0, 1.0, 1.0), 0.0, m.ior
+    if isinstance(m, DiffuseLight):
+        return _abi.GG_MATERIAL_DIFFUSE_LIGHT, m.emit, 0.0, 1.0
+    raise TypeError(f"unsupported material {type(m).__name__}")
+
+
+@dataclass
+class Sphere:
+    """Sphere::new(center, radius, material) (shape/sphere.rs:17-27)."""
+    center: Sequence[float]
+    radius: float
+    material: object
+
+    def to_c(self) -> _abi.gg_hittable:
+        mk, col, fuzz, ior = _material_fields(self.material)
+        h = _abi.gg_hittable(shape=_abi.GG_SHAPE_SPHERE, material=mk, radius=float(self.radius), fuzz=fuzz, ior=ior)
+        h.q[:] = [float(x) for x in self.center]
+        h.color[:] = [float(x) for x in col]
+        return h
+
+
+@dataclass
+class Quad:
+    """Quad::new(q, u, v, material) (shape/quad.rs:19-72): the parallelogram q + a u + b v, a, b in [0, 1]."""
+    q: Sequence[float]
+    u: Sequence[float]
+    v: Sequence[float]
+    material: object
+
+    def to_c(self) -> _abi.gg_hittable:
+        mk, col, fuzz, ior = _material_fields(self.material)
+        h = _abi.gg_hittable(shape=_abi.GG_SHAPE_QUAD, material=mk, fuzz=fuzz, ior=ior)
+        h.q[:] = [float(x) for x in self.q]
+        h.u[:] = [float(x) for x in self.u]
+        h.v[:] = [float(x) for x in self.v]
+        h.color[:] = [float(x) for x in col]
+        return h
+
+
+class World:
+    """World (scene/world.rs:5-60): a list of primitives.  build_bvh() returns the world itself, since
+    the library builds its own hierarchy when the world is set on a Renderer."""
+
+    def __init__(self, objects=None):
+        self.objects = list(objects or [])
+
+    def add(self, obj) -> "World":
+        self.objects.append(obj)
+        return self
+
+    def __len__(self) -> int:
+        return len(self.objects)
+
+    def build_bvh(self) -> "World":
+        return self
+
+    def to_c(self):
+        arr = (_abi.gg_hittable * len(self.objects))(*[o.to_c() for o in self.objects])
+        return arr
+
+
 @dataclass
 class RenderOptions:
     """RenderOptions (renderer.rs:17-26); sky None is the procedural starfield."""
@@ -243,7 +346,7 @@ def _check(lib, rc: int, what: str):
         raise GargantuaError(f"{what}: {lib.gg_strerror(rc).decode()} ({rc}): {lib.gg_last_error().decode()}")
 
 
-def alloc_outputs(n: int, *, state: bool = True, rgb_pixels: int = 0) -> dict:
+def alloc_outputs(n: int, *, state: bool = True, rgb_pixels: int = 0, color: bool = False) -> dict:
     """Host numpy buffers for one scene's per-ray outputs."""
     o = {
         "cls": np.zeros(n, np.uint8), "r": np.zeros(n), "theta": np.zeros(n), "phi": np.zeros(n),
@@ -253,6 +356,8 @@ def alloc_outputs(n: int, *, state: bool = True, rgb_pixels: int = 0) -> dict:
         o["state"] = np.zeros((9, n))
     if rgb_pixels:
         o["rgb"] = np.zeros((rgb_pixels, 3), np.uint8)
+    if color:
+        o["color"] = np.zeros((n, 3))
     return o
 
 
@@ -267,7 +372,8 @@ def outputs_struct(o: dict) -> _abi.gg_outputs:
         assert v.flags["C_CONTIGUOUS"]
         return v.ctypes.data
     return _abi.gg_outputs(cls=p("cls"), r=p("r"), theta=p("theta"), phi=p("phi"), intensity=p("intensity"),
-                           attempts=p("attempts"), accepted=p("accepted"), state=p("state"), rgb=p("rgb"))
+                           attempts=p("attempts"), accepted=p("accepted"), state=p("state"), rgb=p("rgb"),
+                           color=p("color"))
 
 
 def stats_dict(st: _abi.gg_stats) -> dict:
@@ -283,6 +389,7 @@ class Renderer:
         _check(self.lib, self.lib.gg_context_create(device, C.c_void_p(stream) if stream else None, C.byref(h)),
                "gg_context_create")
         self.ctx = h
+        self.n_hittables = 0
 
     def close(self):
         if self.ctx:
@@ -315,7 +422,8 @@ class Renderer:
         """Trace every ray of `scene`; returns the per-ray arrays and stats."""
         sc = scene.to_c()
         o = outputs if outputs is not None else alloc_outputs(
-            scene.n_rays, state=state, rgb_pixels=scene.width * scene.height if rgb else 0)
+            scene.n_rays, state=state, rgb_pixels=scene.width * scene.height if rgb else 0,
+            color=self.n_hittables > 0)
         st = _abi.gg_stats()
         _check(self.lib, self.lib.gg_render(self.ctx, C.byref(sc), PRECISIONS[precision],
                                             C.byref(outputs_struct(o)), C.byref(st)), "gg_render")
@@ -350,7 +458,7 @@ class Renderer:
         """trace_ray / trace_ray_with_disk from given states [n, 9] (t r th ph vt vr vth vph lambda)."""
         states = np.ascontiguousarray(states, dtype=np.float64)
         n = states.shape[0]
-        o = alloc_outputs(n)
+        o = alloc_outputs(n, color=self.n_hittables > 0)
         st = _abi.gg_stats()
         sc = scene.to_c()
         _check(self.lib, self.lib.gg_trace_states(self.ctx, C.byref(sc), PRECISIONS[precision], n,
@@ -409,6 +517,16 @@ class Renderer:
         px = np.ascontiguousarray(sky.pixels, dtype=np.float32)
         _check(self.lib, self.lib.gg_set_sky_hdri(self.ctx, px.shape[1], px.shape[0], px.ctypes.data,
                                                   sky.intensity), "gg_set_sky_hdri")
+
+    def set_hittables(self, world: Optional[World]) -> None:
+        """Scene geometry of every following render / trace_states (None or empty: none)."""
+        if world is None or len(world) == 0:
+            _check(self.lib, self.lib.gg_set_hittables(self.ctx, 0, None), "gg_set_hittables")
+            self.n_hittables = 0
+            return
+        arr = world.to_c()
+        _check(self.lib, self.lib.gg_set_hittables(self.ctx, len(world), C.cast(arr, C.c_void_p)), "gg_set_hittables")
+        self.n_hittables = len(world)
 
     def set_grad_memory(self, nbytes: int = 0) -> None:
         """Device memory grad()'s automatic mode may use for step records (0: a quarter of the free memory)."""
@@ -495,4 +613,29 @@ def render_with_disk(metric, camera: Camera, disk: AccretionDisk, width: int, he
     r = _renderer()
     r.set_sky_hdri(opts.sky)
     o = r.render(sc, "fp64", rgb=True)
+    return o["rgb"].reshape(height, width, 3)
+
+
+def render_with_scene(metric, camera: Camera, scene: World, width: int, height: int,
+                      opts: Optional[RenderOptions] = None) -> np.ndarray:
+    """raytracer::curved::render_with_scene (renderer.rs:103-174)."""
+    return _render_scene(metric, camera, None, scene, width, height, opts)
+
+
+def render_with_disk_and_scene(metric, camera: Camera, disk: AccretionDisk, scene: World, width: int, height: int,
+                               opts: Optional[RenderOptions] = None) -> np.ndarray:
+    """raytracer::curved::render_with_disk_and_scene (renderer.rs:176-254); the disk is volumetric."""
+    return _render_scene(metric, camera, disk, scene, width, height, opts)
+
+
+def _render_scene(metric, camera, disk, scene, width, height, opts) -> np.ndarray:
+    opts = opts or RenderOptions()
+    sc = _reference_render_scene(metric, camera, disk, width, height, opts)
+    r = _renderer()
+    r.set_sky_hdri(opts.sky)
+    r.set_hittables(scene)
+    try:
+        o = r.render(sc, "fp64", rgb=True)
+    finally:
+        r.set_hittables(None)
     return o["rgb"].reshape(height, width, 3)

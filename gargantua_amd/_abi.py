@@ -25,6 +25,13 @@ GG_CLASS_HORIZON = 0
 GG_CLASS_ESCAPED = 1
 GG_CLASS_DISK = 2
 GG_CLASS_MAX_STEPS = 3
+GG_CLASS_SCENE = 4
+GG_SHAPE_SPHERE = 0
+GG_SHAPE_QUAD = 1
+GG_MATERIAL_LAMBERTIAN = 0
+GG_MATERIAL_METAL = 1
+GG_MATERIAL_DIELECTRIC = 2
+GG_MATERIAL_DIFFUSE_LIGHT = 3
 
 
 class gg_metric(C.Structure):
@@ -62,7 +69,13 @@ class gg_scene(C.Structure):
 class gg_outputs(C.Structure):
     _fields_ = [("cls", C.c_void_p), ("r", C.c_void_p), ("theta", C.c_void_p), ("phi", C.c_void_p),
                 ("intensity", C.c_void_p), ("attempts", C.c_void_p), ("accepted", C.c_void_p),
-                ("state", C.c_void_p), ("rgb", C.c_void_p)]
+                ("state", C.c_void_p), ("rgb", C.c_void_p), ("color", C.c_void_p)]
+
+
+class gg_hittable(C.Structure):
+    _fields_ = [("shape", C.c_int32), ("material", C.c_int32), ("q", C.c_double * 3), ("u", C.c_double * 3),
+                ("v", C.c_double * 3), ("radius", C.c_double), ("color", C.c_double * 3), ("fuzz", C.c_double),
+                ("ior", C.c_double)]
 
 
 class gg_stats(C.Structure):
@@ -104,6 +117,7 @@ EXPORTS = [
     ("gg_set_launch", C.c_int, [_P, C.POINTER(gg_launch)]),
     ("gg_set_grad_memory", C.c_int, [_P, C.c_uint64]),
     ("gg_set_sky_hdri", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_float]),
+    ("gg_set_hittables", C.c_int, [_P, C.c_uint32, _P]),
     ("gg_get_launch", C.c_int, [_P, C.POINTER(gg_launch)]),
     ("gg_scene_rays", C.c_int, [C.POINTER(gg_scene), C.POINTER(C.c_uint64)]),
     ("gg_render", C.c_int, [_P, C.POINTER(gg_scene), C.c_int, C.POINTER(gg_outputs), C.POINTER(gg_stats)]),

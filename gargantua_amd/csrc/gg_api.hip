@@ -20,6 +20,7 @@
 
 #include "gargantua.h"
 #include "gg_internal.hpp"
+#include "gg_scene.hpp"
 
 using namespace gg;
 
@@ -89,7 +90,7 @@ struct gg_context {
   } graph_key{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf scenes, outs, counters, states_in;
-  DevBuf stage[9];       // device staging for host-memory outputs
+  DevBuf stage[10];      // device staging for host-memory outputs
   DevBuf rhs_io, misc_io;
   // learned ray order (gg_launch.order = 1): queue position -> ray of the
   // last camera render, the signature of its grids, the sort's work buffers
@@ -109,6 +110,11 @@ struct gg_context {
   DevBuf sky;
   uint32_t sky_w = 0, sky_h = 0;
   float sky_intensity = 1.0f;
+  // gg_set_hittables: primitives (BVH leaf order) and BVH nodes; hit_off
+  // while gg_grad renders its own scene
+  DevBuf hit_objs, hit_nodes;
+  uint32_t n_hit = 0;
+  bool hit_off = false;
   // the recording render gg_grad asks render_impl for (pool null: none)
   struct {
     double *pool;
@@ -119,6 +125,121 @@ struct gg_context {
 };
 
 namespace {
+
+// ---- Hittable scene: primitive constants and the BVH ----
+
+struct HitBox { double lo[3], hi[3]; };
+
+void cross_h(const double a[3], const double b[3], double r[3]) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+double dot_h(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Sphere::new (sphere.rs:17-27) / Quad::with_arc_material (quad.rs:24-72)
+// and the materials' scatter albedo and emission (lambertian.rs:41-56,
+// metal.rs:13-18,40-57, dielectric.rs:65-90, diffuse_light.rs:22-35).
+int make_scene_obj(const gg_hittable &h, uint32_t i, SceneObj &o, HitBox &b) {
+  std::memset(&o, 0, sizeof(o));
+  auto finite3 = [](const double *v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); };
+  if (!finite3(h.q) || !finite3(h.color)) return fail(GG_ERR_INVALID_ARG, "hittable %u: non-finite position or colour", i);
+  switch (h.material) {
+    case GG_MATERIAL_LAMBERTIAN:
+      for (int k = 0; k < 3; ++k) o.albedo[k] = h.color[k];
+      break;
+    case GG_MATERIAL_METAL: {
+      double fuzz = h.fuzz < 0.0 ? 0.0 : (h.fuzz > 1.0 ? 1.0 : h.fuzz);   // fuzz.clamp(0, 1)
+      if (!(fuzz == 0.0))
+        return fail(GG_ERR_UNSUPPORTED, "hittable %u: a metal with fuzz > 0 shades with a random draw", i);
+      for (int k = 0; k < 3; ++k) o.albedo[k] = h.color[k];
+      o.metal = 1;
+      break;
+    }
+    case GG_MATERIAL_DIELECTRIC:
+      o.albedo[0] = o.albedo[1] = o.albedo[2] = 1.0;
+      break;
+    case GG_MATERIAL_DIFFUSE_LIGHT:
+      for (int k = 0; k < 3; ++k) o.emit[k] = h.color[k];
+      break;
+    default:
+      return fail(GG_ERR_INVALID_ARG, "hittable %u: material %d unknown", i, h.material);
+  }
+  if (h.shape == GG_SHAPE_SPHERE) {
+    if (!std::isfinite(h.radius) || h.radius == 0.0) return fail(GG_ERR_INVALID_ARG, "hittable %u: sphere radius", i);
+    o.shape = SHAPE_SPHERE;
+    for (int k = 0; k < 3; ++k) o.a[k] = h.q[k];
+    o.n[0] = h.radius;
+    const double r = std::fabs(h.radius);
+    for (int k = 0; k < 3; ++k) { b.lo[k] = h.q[k] - r; b.hi[k] = h.q[k] + r; }
+  } else if (h.shape == GG_SHAPE_QUAD) {
+    if (!finite3(h.u) || !finite3(h.v)) return fail(GG_ERR_INVALID_ARG, "hittable %u: non-finite quad edges", i);
+    double nn[3];
+    cross_h(h.u, h.v, nn);
+    const double n2 = dot_h(nn, nn);
+    if (!(n2 > 0.0) || !std::isfinite(n2)) return fail(GG_ERR_INVALID_ARG, "hittable %u: degenerate quad", i);
+    const double len = std::sqrt(n2);
+    o.shape = SHAPE_QUAD;
+    for (int k = 0; k < 3; ++k) { o.a[k] = h.q[k]; o.n[k] = nn[k] / len; }
+    o.d = dot_h(o.n, h.q);
+    double w[3] = {nn[0] / n2, nn[1] / n2, nn[2] / n2};
+    cross_h(h.v, w, o.ea);
+    cross_h(w, h.u, o.eb);
+    for (int k = 0; k < 3; ++k) {
+      const double c[4] = {h.q[k], h.q[k] + h.u[k], h.q[k] + h.v[k], h.q[k] + h.u[k] + h.v[k]};
+      b.lo[k] = std::min(std::min(c[0], c[1]), std::min(c[2], c[3])) - 0.0001;
+      b.hi[k] = std::max(std::max(c[0], c[1]), std::max(c[2], c[3])) + 0.0001;
+    }
+  } else {
+    return fail(GG_ERR_INVALID_ARG, "hittable %u: shape %d unknown", i, h.shape);
+  }
+  // a margin far above rounding, so that the slab test never culls a
+  // primitive the exact test hits
+  for (int k = 0; k < 3; ++k) {
+    const double m = 1e-9 * (1.0 + std::max(std::fabs(b.lo[k]), std::fabs(b.hi[k])));
+    b.lo[k] -= m;
+    b.hi[k] += m;
+  }
+  return GG_OK;
+}
+
+// Median split on the longest axis of the centroid bounds, leaves of at
+// most four primitives; order[] becomes the leaf order.  Node 0 is the root.
+int32_t build_bvh(const std::vector<HitBox> &box, std::vector<uint32_t> &order, uint32_t b, uint32_t e,
+                  std::vector<BvhNode> &nodes, int depth, int &max_depth) {
+  max_depth = std::max(max_depth, depth);
+  const int32_t id = (int32_t)nodes.size();
+  nodes.push_back(BvhNode{});
+  BvhNode nd{};
+  double clo[3], chi[3];
+  for (int k = 0; k < 3; ++k) { nd.lo[k] = clo[k] = INFINITY; nd.hi[k] = chi[k] = -INFINITY; }
+  for (uint32_t i = b; i < e; ++i) {
+    const HitBox &x = box[order[i]];
+    for (int k = 0; k < 3; ++k) {
+      nd.lo[k] = std::min(nd.lo[k], x.lo[k]);
+      nd.hi[k] = std::max(nd.hi[k], x.hi[k]);
+      const double c = 0.5 * (x.lo[k] + x.hi[k]);
+      clo[k] = std::min(clo[k], c);
+      chi[k] = std::max(chi[k], c);
+    }
+  }
+  if (e - b <= 4) {
+    nd.left = (int32_t)b;
+    nd.right = -(int32_t)(e - b);
+  } else {
+    int axis = 0;
+    for (int k = 1; k < 3; ++k)
+      if (chi[k] - clo[k] > chi[axis] - clo[axis]) axis = k;
+    const uint32_t mid = b + (e - b) / 2;
+    std::nth_element(order.begin() + b, order.begin() + mid, order.begin() + e, [&](uint32_t p, uint32_t q) {
+      return box[p].lo[axis] + box[p].hi[axis] < box[q].lo[axis] + box[q].hi[axis];
+    });
+    nd.left = build_bvh(box, order, b, mid, nodes, depth + 1, max_depth);
+    nd.right = build_bvh(box, order, mid, e, nodes, depth + 1, max_depth);
+  }
+  nodes[id] = nd;
+  return id;
+}
 
 // ---- host restatements needed for per-scene constants ----
 
@@ -293,6 +414,7 @@ void build_dev_scene(const gg_scene &s, DevScene &D) {
   D.sqrt_g_phph = std::sqrt(D.g_phph);
   double g_tt_obs = -D.g_tt;                                 // tracer.rs:93-94
   D.sqrt_neg_gtt_obs = std::sqrt(g_tt_obs >= 0.0 ? g_tt_obs : 0.0);
+  D.g_tt_obs = g_tt_obs;                                     // tracer.rs:216
   uint32_t spa = s.image.samples_per_axis ? s.image.samples_per_axis : 1;
   D.sub_w = s.image.width * spa;
   D.sub_h = s.image.height * spa;
@@ -420,6 +542,7 @@ int gg_context_destroy(gg_context *ctx) {
   for (auto &b : ctx->grad) b.release();
   for (auto &b : ctx->stage) b.release();
   for (auto &b : ctx->batch_stage) b.release();
+  ctx->hit_objs.release(); ctx->hit_nodes.release(); ctx->sky.release();
   if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -466,6 +589,42 @@ int gg_set_sky_hdri(gg_context *ctx, uint32_t width, uint32_t height, const floa
   return GG_OK;
 }
 
+int gg_set_hittables(gg_context *ctx, uint32_t n, const gg_hittable *objects) {
+  if (!ctx) return fail(GG_ERR_INVALID_ARG, "NULL context");
+  if (n == 0) {
+    ctx->n_hit = 0;
+    return GG_OK;
+  }
+  if (!objects) return fail(GG_ERR_INVALID_ARG, "hittables: NULL objects");
+  if (n > (1u << 24)) return fail(GG_ERR_INVALID_ARG, "hittables: more than 2^24 primitives");
+  std::vector<SceneObj> prim(n);
+  std::vector<HitBox> boxes(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    int rc = make_scene_obj(objects[i], i, prim[i], boxes[i]);
+    if (rc) return rc;
+  }
+  std::vector<BvhNode> nodes;
+  std::vector<uint32_t> order(n);
+  for (uint32_t i = 0; i < n; ++i) order[i] = i;
+  nodes.reserve(2 * (size_t)n);
+  int depth = 0;
+  build_bvh(boxes, order, 0, n, nodes, 1, depth);
+  if (depth + 2 > BVH_STACK) return fail(GG_ERR_INTERNAL, "hittables: BVH depth %d exceeds the traversal stack", depth);
+  std::vector<SceneObj> leaf_order(n);
+  for (uint32_t i = 0; i < n; ++i) leaf_order[i] = prim[order[i]];
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc;
+  if ((rc = ctx->hit_objs.ensure(sizeof(SceneObj) * n))) return rc;
+  if ((rc = ctx->hit_nodes.ensure(sizeof(BvhNode) * nodes.size()))) return rc;
+  HIP_TRY(hipMemcpyAsync(ctx->hit_objs.p, leaf_order.data(), sizeof(SceneObj) * n, hipMemcpyHostToDevice,
+                         ctx->stream));
+  HIP_TRY(hipMemcpyAsync(ctx->hit_nodes.p, nodes.data(), sizeof(BvhNode) * nodes.size(), hipMemcpyHostToDevice,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->n_hit = n;
+  return GG_OK;
+}
+
 int gg_set_grad_memory(gg_context *ctx, uint64_t bytes) {
   if (!ctx) return fail(GG_ERR_INVALID_ARG, "NULL context");
   ctx->grad_budget = bytes;
@@ -506,13 +665,22 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
   for (uint32_t i = 1; i < n_scenes; ++i)
     if (scenes[i].metric.kind != scenes[0].metric.kind || scenes[i].disk.kind != scenes[0].disk.kind)
       return fail(GG_ERR_UNSUPPORTED, "a batch must share metric kind and disk kind (scene %u differs)", i);
+  // Hittables: render_with_scene (no disk) or render_with_disk_and_scene
+  // (volumetric disk), one scene per call as the reference's drivers.
+  const bool with_hits = ctx->n_hit > 0 && !ctx->hit_off;
+  int mode = scenes[0].disk.kind;
+  if (with_hits) {
+    if (n_scenes > 1) return fail(GG_ERR_UNSUPPORTED, "hittables are set: render one scene per call");
+    if (mode == GG_DISK_THIN) return fail(GG_ERR_UNSUPPORTED, "hittables are set: the thin disk has no scene mode");
+    mode = mode == GG_DISK_NONE ? 3 : 4;
+  }
   HIP_TRY(hipSetDevice(ctx->device));
 
   std::vector<DevScene> ds(n_scenes);
   std::vector<DevOut> dout(n_scenes);
   std::vector<std::vector<OutPlan>> plans(n_scenes);
   uint64_t total = 0;
-  if (ctx->batch_stage.size() < 9ull * n_scenes) ctx->batch_stage.resize(9ull * n_scenes);
+  if (ctx->batch_stage.size() < 10ull * n_scenes) ctx->batch_stage.resize(10ull * n_scenes);
   for (uint32_t i = 0; i < n_scenes; ++i) {
     build_dev_scene(scenes[i], ds[i]);
     uint64_t n = from_states ? n_states : (uint64_t)ds[i].sub_w * ds[i].sub_h;
@@ -520,8 +688,12 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
     ds[i].out_index = i;
     total += n;
     const gg_outputs &o = outs[i];
-    plans[i].resize(9);
-    DevBuf *st = n_scenes == 1 ? ctx->stage : &ctx->batch_stage[9ull * i];
+    plans[i].resize(10);
+    DevBuf *st = n_scenes == 1 ? ctx->stage : &ctx->batch_stage[10ull * i];
+    if (with_hits) {
+      ds[i].objs = ctx->hit_objs.p;
+      ds[i].nodes = ctx->hit_nodes.p;
+    }
     bool need_rgb = (o.rgb != nullptr) && !from_states;
     // rgb shading reads cls, intensity and the velocity planes: stage them
     // internally when the caller did not ask for them.
@@ -534,7 +706,9 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
     if ((rc = plan_out(st[5], o.attempts, 4 * n, plans[i][5]))) return rc;
     if ((rc = plan_out(st[6], o.accepted, 4 * n, plans[i][6]))) return rc;
     if ((rc = plan_out(st[7], state_p, 72 * n, plans[i][7]))) return rc;
+    if (with_hits && (rc = plan_out(st[9], o.color, 24 * n, plans[i][9]))) return rc;
     if (need_rgb) {
+      if (with_hits && !plans[i][9].dev) { if ((rc = st[9].ensure(24 * n))) return rc; plans[i][9].dev = st[9].p; }
       if (!plans[i][0].dev) { if ((rc = st[0].ensure(n))) return rc; plans[i][0].dev = st[0].p; }
       if (!plans[i][4].dev) { if ((rc = st[4].ensure(8 * n))) return rc; plans[i][4].dev = st[4].p; }
       if (!plans[i][7].dev) { if ((rc = st[7].ensure(72 * n))) return rc; plans[i][7].dev = st[7].p; }
@@ -551,6 +725,7 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
     d.attempts = (uint32_t *)plans[i][5].dev;
     d.accepted = (uint32_t *)plans[i][6].dev;
     d.state = (double *)plans[i][7].dev;
+    d.color = (double *)plans[i][9].dev;
     d.n_rays = n;
   }
   // learned ray order: attempts are the sort keys of the next render's queue
@@ -626,7 +801,10 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
   } else {
     HIP_TRY(launch_reset(T.counters, s));
     HIP_TRY(hipEventRecord(ctx->ev0, s));
-    if (total > 0) HIP_TRY(launch_trace(scenes[0].metric.kind, scenes[0].disk.kind, precision, T, grid, block, s));
+    if (total > 0) {
+      if (mode >= 3) HIP_TRY(launch_trace_scene(scenes[0].metric.kind, mode, precision, T, grid, block, s));
+      else HIP_TRY(launch_trace(scenes[0].metric.kind, mode, precision, T, grid, block, s));
+    }
     HIP_TRY(hipEventRecord(ctx->ev1, s));
   }
   for (uint32_t i = 0; i < n_scenes; ++i) {
@@ -963,7 +1141,9 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
       o.state = (double *)B[20].p;
       ctx->rec_req = {(double *)B[18].p, pool_chunks, &d_cnt->rec_cursor, (uint32_t *)B[19].p};
     }
+    ctx->hit_off = true;      // the gradient's scene is its own: no hittables
     rc = render_impl(ctx, &sc, 1, record && mixed ? GG_PREC_MIXED : GG_PREC_FP64, &o, nullptr, 0, &rs);
+    ctx->hit_off = false;
     ctx->rec_req = {};
     if (rc) return rc;
 

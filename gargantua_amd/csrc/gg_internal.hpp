@@ -14,6 +14,7 @@ struct DevOut {
   double *r, *theta, *phi, *intensity;
   uint32_t *attempts, *accepted;
   double *state;        // 9 planes of n_rays each
+  double *color;        // scene modes: linear RGB of each ray's scene hit (3 per ray, 0 when none)
   uint64_t n_rays;      // rays of this scene (plane stride of `state`)
 };
 
@@ -125,7 +126,10 @@ hipError_t launch_grad_reset(GradCounters *c, hipStream_t s);
 hipError_t launch_rhs_vjp(const DevScene *S, uint64_t n, const double *pos, const double *vel, const double *cot,
                           double *pos_bar, double *vel_bar, double *spin_bar, hipStream_t s);
 
-// Launch the persistent trace kernel for (metric, disk, precision).
+// Launch the persistent trace kernel for (metric, disk, precision); disk
+// 3 and 4 are the scene modes (gg_trace_scene.hip, one scene per launch).
+hipError_t launch_trace_scene(int metric_kind, int mode, int precision, const TraceLaunch &t, uint32_t grid,
+                              uint32_t block, hipStream_t stream);
 hipError_t launch_trace(int metric_kind, int disk_kind, int precision, const TraceLaunch &t, uint32_t grid,
                         uint32_t block, hipStream_t stream);
 hipError_t launch_reset(Counters *c, hipStream_t stream);

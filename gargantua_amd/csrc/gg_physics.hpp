@@ -53,6 +53,11 @@ struct DevScene {
   // on inbound rays beyond fp32_far_r, and on inbound rays beyond fp32_min_r
   // whose squared impact parameter (Q + L_z^2)/E^2 is at least fp32_b2_min
   double fp32_min_r, fp32_far_r, fp32_b2_min;
+  // Hittable scene (gg_scene.hpp): primitives, BVH nodes, and -g_tt at
+  // the camera for the static-emitter redshift (tracer.rs:216)
+  const void *objs;
+  const void *nodes;
+  double g_tt_obs;
   // batch bookkeeping
   uint64_t ray_begin;     // first global ray index of this scene
   uint64_t out_index;     // which output set (batch)

@@ -1,0 +1,196 @@
+// gg_scene.hpp -- Hittable geometry met by the chords of a curved photon path.
+//
+// trace_ray_with_scene / trace_ray_with_disk_and_scene (tracer.rs:207-364)
+// map every accepted step to the straight Cartesian chord between its end
+// points and ask the scene for the closest hit with chord parameter in
+// (0, 1).  Primitives are the reference's Sphere (shape/sphere.rs:53-88)
+// and Quad (shape/quad.rs:75-127); materials enter only through the albedo
+// of one `scatter` call and their emission (shade_hit, tracer.rs:369-384).
+// The host builds a bounding-volume hierarchy over the primitives (flat
+// array, leaves of up to four primitives); the closest hit it returns is
+// the minimum over primitives of each primitive's own first root in the
+// open interval, so it equals the reference's BVH answer whatever the tree.
+#pragma once
+
+#include "gg_physics.hpp"
+
+namespace gg {
+
+enum : uint32_t { SHAPE_SPHERE = 0, SHAPE_QUAD = 1 };
+
+// One primitive as the kernels read it (host-precomputed, 160 bytes).
+struct SceneObj {
+  double a[3];        // sphere centre | quad corner Q
+  double n[3];        // quad unit normal; sphere: n[0] = radius
+  double ea[3], eb[3];// quad e_alpha = v x w, e_beta = w x u (quad.rs:29-31)
+  double d;           // quad plane offset normal . Q
+  double albedo[3];   // attenuation of the material's scatter
+  double emit[3];     // material emission
+  uint32_t shape;
+  uint32_t metal;     // mirror: the albedo only when the reflection leaves the surface (metal.rs:40-57)
+};
+
+// BVH node (64 bytes).  Internal: left/right child indices.  Leaf:
+// left = first primitive of its range, right = -(primitive count).
+struct BvhNode {
+  double lo[3], hi[3];
+  int32_t left, right;
+  uint32_t pad[2];
+};
+
+constexpr int BVH_STACK = 32;       // the host rejects trees deeper than this (median splits of 2^24 primitives: 23)
+
+namespace {
+
+__device__ inline double sdot3(const double a[3], const double b[3]) {
+#pragma clang fp contract(off)
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];           // nalgebra dot, U3: (a + b) + c
+}
+
+// spherical_to_cartesian (tracer.rs:12-18).
+__device__ inline void sph2cart(const double p[4], double c[3]) {
+#pragma clang fp contract(off)
+  const double r = p[1], st = sin(p[2]);
+  c[0] = r * st * cos(p[3]);
+  c[1] = r * st * sin(p[3]);
+  c[2] = r * cos(p[2]);
+}
+
+// Slab test of [lo, hi] against origin + t dir, t in (0, t_max).  The host
+// pads every box, so an origin on a slab plane (0 * inf) cannot cull a
+// primitive the exact test would hit.
+__device__ inline bool box_hit(const BvhNode &b, const double o[3], const double inv[3], double t_max) {
+  double t0 = 0.0, t1 = t_max;
+  for (int k = 0; k < 3; ++k) {
+    double ta = (b.lo[k] - o[k]) * inv[k], tb = (b.hi[k] - o[k]) * inv[k];
+    double tl = fmin(ta, tb), th = fmax(ta, tb);
+    t0 = fmax(t0, tl);
+    t1 = fmin(t1, th);
+  }
+  return t0 <= t1;
+}
+
+// Sphere::hit (sphere.rs:53-88) / Quad::hit (quad.rs:75-127) for t in
+// (0, t_max): the root, the hit point and the outward normal.
+__device__ inline bool hit_object(const SceneObj &s, const double o[3], const double dir[3], double t_max, double &t,
+                                  double p[3], double nrm[3]) {
+#pragma clang fp contract(off)
+  if (s.shape == SHAPE_SPHERE) {
+    const double r = s.n[0];
+    const double oc[3] = {o[0] - s.a[0], o[1] - s.a[1], o[2] - s.a[2]};
+    const double a = sdot3(dir, dir);
+    const double half_b = sdot3(oc, dir);
+    const double c = sdot3(oc, oc) - r * r;
+    const double disc = half_b * half_b - a * c;
+    if (disc < 0.0) return false;
+    const double sq = sqrt(disc);
+    double root = (-half_b - sq) / a;
+    if (!(0.0 < root && root < t_max)) {
+      root = (-half_b + sq) / a;
+      if (!(0.0 < root && root < t_max)) return false;
+    }
+    t = root;
+    for (int k = 0; k < 3; ++k) p[k] = o[k] + dir[k] * root;
+    for (int k = 0; k < 3; ++k) nrm[k] = (p[k] - s.a[k]) / r;
+    return true;
+  }
+  const double denom = sdot3(s.n, dir);
+  if (fabs(denom) < 1e-8) return false;
+  const double tq = (s.d - sdot3(s.n, o)) / denom;
+  if (!(0.0 < tq && tq < t_max)) return false;
+  double q[3];
+  for (int k = 0; k < 3; ++k) q[k] = o[k] + dir[k] * tq;
+  const double pl[3] = {q[0] - s.a[0], q[1] - s.a[1], q[2] - s.a[2]};
+  const double al = sdot3(pl, s.ea), be = sdot3(pl, s.eb);
+  if (al < 0.0 || al > 1.0 || be < 0.0 || be > 1.0) return false;
+  t = tq;
+  for (int k = 0; k < 3; ++k) { p[k] = q[k]; nrm[k] = s.n[k]; }
+  return true;
+}
+
+struct SceneHit {
+  double t;
+  double p[3];
+  double nrm[3];     // facing against the chord (HitRecord::new, core/hit.rs:26-48)
+  uint32_t obj;
+};
+
+// Closest primitive hit by the chord origin + t dir, t in (0, 1).
+__device__ inline bool scene_hit(const SceneObj *objs, const BvhNode *nodes, const double o[3], const double dir[3],
+                                 SceneHit &h) {
+#pragma clang fp contract(off)
+  const double inv[3] = {1.0 / dir[0], 1.0 / dir[1], 1.0 / dir[2]};
+  double t_max = 1.0;
+  bool any = false;
+  int32_t stack[BVH_STACK];
+  int top = 0;
+  if (!box_hit(nodes[0], o, inv, t_max)) return false;
+  stack[top++] = 0;
+  while (top > 0) {
+    const BvhNode &nd = nodes[stack[--top]];
+    if (nd.right < 0) {
+      const int32_t end = nd.left - nd.right;
+      for (int32_t i = nd.left; i < end; ++i) {
+        double t, p[3], nr[3];
+        if (hit_object(objs[i], o, dir, t_max, t, p, nr)) {
+          t_max = t;
+          any = true;
+          h.t = t;
+          h.obj = (uint32_t)i;
+          for (int k = 0; k < 3; ++k) { h.p[k] = p[k]; h.nrm[k] = nr[k]; }
+        }
+      }
+    } else {
+      const bool hl = box_hit(nodes[nd.left], o, inv, t_max), hr = box_hit(nodes[nd.right], o, inv, t_max);
+      if (hr && top < BVH_STACK) stack[top++] = nd.right;
+      if (hl && top < BVH_STACK) stack[top++] = nd.left;
+    }
+  }
+  if (any) {
+    // front_face = dir . outward < 0; the normal faces against the chord
+    if (!(sdot3(dir, h.nrm) < 0.0))
+      for (int k = 0; k < 3; ++k) h.nrm[k] = -h.nrm[k];
+  }
+  return any;
+}
+
+// shade_hit (tracer.rs:369-384): albedo x (0.5 + 0.5 max(-d.n, 0)) + emission.
+__device__ inline void shade_hit(const SceneObj &s, const double dir[3], const double nrm[3], double lin[3]) {
+#pragma clang fp contract(off)
+  const double len = sqrt(sdot3(dir, dir));
+  const double ud[3] = {dir[0] / len, dir[1] / len, dir[2] / len};
+  double alb[3] = {s.albedo[0], s.albedo[1], s.albedo[2]};
+  if (s.metal) {
+    // reflect(unit d, n) = d - (2 d.n) n, scattered only above the surface
+    const double dn = sdot3(ud, nrm);
+    const double rf[3] = {ud[0] - 2.0 * dn * nrm[0], ud[1] - 2.0 * dn * nrm[1], ud[2] - 2.0 * dn * nrm[2]};
+    if (!(sdot3(rf, nrm) > 0.0)) alb[0] = alb[1] = alb[2] = 0.0;
+  }
+  const double mud[3] = {-ud[0], -ud[1], -ud[2]};
+  double nd = sdot3(mud, nrm);
+  nd = nd > 0.0 ? nd : 0.0;                                     // f64::max(nd, 0.0)
+  const double facing = 0.5 + 0.5 * nd;
+  for (int k = 0; k < 3; ++k) lin[k] = alb[k] * facing + s.emit[k];
+}
+
+// redshift_g4_static (tracer.rs:40-52) at cartesian_to_spherical(point)
+// (tracer.rs:23-28): (g_tt(emit) / g_tt(obs))^2, 0 where either is <= 0.
+template <int METRIC>
+__device__ inline double g4_static(const DevScene &S, const double pt[3]) {
+#pragma clang fp contract(off)
+  double r = sqrt(pt[0] * pt[0] + pt[1] * pt[1] + pt[2] * pt[2]);
+  r = r >= 1e-10 ? r : 1e-10;
+  double zr = pt[2] / r;
+  zr = zr < -1.0 ? -1.0 : (zr > 1.0 ? 1.0 : zr);
+  const double th = acos(zr);
+  const double gte = -metric64<METRIC>(S, r, th).tt;
+  const double gto = S.g_tt_obs;
+  if (gte <= 0.0 || gto <= 0.0) return 0.0;
+  const double g = sqrt(gte / gto);
+  const double g2 = g * g;
+  return g2 * g2;
+}
+
+}  // namespace
+
+}  // namespace gg

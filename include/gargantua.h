@@ -161,7 +161,8 @@ typedef enum gg_ray_class {
   GG_CLASS_HORIZON = 0,
   GG_CLASS_ESCAPED = 1,
   GG_CLASS_DISK = 2,
-  GG_CLASS_MAX_STEPS = 3
+  GG_CLASS_MAX_STEPS = 3,
+  GG_CLASS_SCENE = 4      /* RayOutcome::Scene: a chord met a gg_hittable      */
 } gg_ray_class;
 
 /* Per-ray outputs, structure of arrays.  N = (width*spa) * (height*spa),
@@ -178,7 +179,35 @@ typedef struct gg_outputs {
   uint32_t *accepted;   /* [N] accepted steps                                 */
   double *state;        /* [9*N] SoA planes: t r th ph vt vr vth vph lambda   */
   uint8_t *rgb;         /* [3*width*height] reference shading + sRGB          */
+  double *color;        /* [3*N] linear RGB of GG_CLASS_SCENE rays (0 for
+                           other classes); written only while hittables are
+                           set (gg_set_hittables)                             */
 } gg_outputs;
+
+/* ---- Hittable scene (crates/raytracer/src/{shape,material}) ---- */
+typedef enum gg_shape {
+  GG_SHAPE_SPHERE = 0,    /* Sphere::new(center = q, radius, material)        */
+  GG_SHAPE_QUAD = 1       /* Quad::new(q, u, v, material)                      */
+} gg_shape;
+
+typedef enum gg_material {
+  GG_MATERIAL_LAMBERTIAN = 0,    /* Lambertian::new(albedo = color)            */
+  GG_MATERIAL_METAL = 1,         /* Metal::new(albedo = color, fuzz); fuzz 0 only */
+  GG_MATERIAL_DIELECTRIC = 2,    /* Dielectric::new(ior): white attenuation     */
+  GG_MATERIAL_DIFFUSE_LIGHT = 3  /* DiffuseLight::new(emit = color)             */
+} gg_material;
+
+typedef struct gg_hittable {
+  int32_t shape;          /* gg_shape                                          */
+  int32_t material;       /* gg_material                                       */
+  double q[3];            /* sphere centre | quad corner                       */
+  double u[3];            /* quad edge u                                       */
+  double v[3];            /* quad edge v                                       */
+  double radius;          /* sphere radius (> 0)                               */
+  double color[3];        /* albedo, or the emission of a diffuse light        */
+  double fuzz;            /* metal (clamped to [0, 1] as Metal::new does)      */
+  double ior;             /* dielectric (does not enter the one-bounce shading) */
+} gg_hittable;
 
 /* Launch statistics (filled when the pointer is non-NULL). */
 typedef struct gg_stats {
@@ -356,6 +385,19 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target,
  * default); this call is the library's counterpart of RenderOptions.sky. */
 int gg_set_sky_hdri(gg_context *ctx, uint32_t width, uint32_t height,
                     const float *pixels, float intensity);
+
+/* Hittable geometry for every following gg_render / gg_trace_states on this
+ * context: render_with_scene (disk kind NONE) and render_with_disk_and_scene
+ * (disk kind VOLUMETRIC), crates/raytracer/src/curved/renderer.rs:103-254,
+ * tracer.rs:207-364.  Each accepted step's Cartesian chord is tested against
+ * the n primitives (the library builds its own BVH; the closest hit is what
+ * the reference's BvhNode/World returns); a hit ends the ray as
+ * GG_CLASS_SCENE with its linear colour in gg_outputs.color (one-bounce
+ * shade_hit x static g^4, plus the disk emission in front of it).  n = 0
+ * clears the scene.  GG_ERR_UNSUPPORTED for a thin disk or a batch while
+ * hittables are set, and for a metal with fuzz > 0 (its reference shading
+ * draws a random number). */
+int gg_set_hittables(gg_context *ctx, uint32_t n, const gg_hittable *objects);
 
 /* Device memory gg_grad's automatic mode (checkpoint_every = 0) may use for
  * step records; 0 (the default): a quarter of the free device memory.  The

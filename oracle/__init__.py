@@ -37,6 +37,9 @@ _SIGS = [
     ("oracle_encode_srgb", None, [_P, _P]),
     ("oracle_sky", None, [_P, _P]),
     ("oracle_set_sky_hdri", C.c_int, [C.c_uint32, C.c_uint32, _P, C.c_float]),
+    ("oracle_set_hittables", C.c_int, [C.c_uint32, _P]),
+    ("oracle_scene_hit", C.c_int, [_P, _P, _P]),
+    ("oracle_g4_static", C.c_double, [_P, _P, C.c_double]),
     ("oracle_hdri_sample", None, [C.c_uint32, C.c_uint32, _P, C.c_float, _P, _P]),
     ("oracle_splitmix64", C.c_uint64, [C.c_uint64, C.c_uint64]),
     ("oracle_render_rays", C.c_int, [C.POINTER(_abi.gg_scene), C.c_uint64, _P, C.POINTER(_abi.gg_outputs), C.c_int]),
@@ -69,6 +72,7 @@ class Oracle:
             f.restype = res
             f.argtypes = args
         self.threads = threads
+        self.n_hittables = 0
 
     # ---- metric level ----
     def metric_tensor(self, metric, pos):
@@ -201,8 +205,32 @@ class Oracle:
         return self.lib.oracle_r_isco(mass, spin)
 
     # ---- whole paths ----
+    def set_hittables(self, world=None):
+        """Scene geometry of render / trace_states (None: none); returns the status code."""
+        if world is None or len(world) == 0:
+            self.lib.oracle_set_hittables(0, None)
+            self.n_hittables = 0
+            return 0
+        arr = world.to_c()
+        rc = self.lib.oracle_set_hittables(len(world), C.cast(arr, C.c_void_p))
+        self.n_hittables = len(world) if rc == 0 else 0
+        return rc
+
+    def scene_hit(self, origin, direction):
+        """Closest hit of origin + t direction, t in (0, 1): None or (object, point, normal, shade_hit colour)."""
+        out = np.zeros(12)
+        o, d = _f64(origin, 3), _f64(direction, 3)
+        if not self.lib.oracle_scene_hit(o.ctypes.data, d.ctypes.data, out.ctypes.data):
+            return None
+        return int(out[1]), out[2:5].copy(), out[5:8].copy(), out[8:11].copy()
+
+    def g4_static(self, metric, point, g_tt_obs):
+        p = _f64(point, 3)
+        return self.lib.oracle_g4_static(C.byref(metric.to_c()), p.ctypes.data, g_tt_obs)
+
     def render(self, scene: Scene, rgb: bool = False, state: bool = True) -> dict:
-        o = alloc_outputs(scene.n_rays, state=state, rgb_pixels=scene.width * scene.height if rgb else 0)
+        o = alloc_outputs(scene.n_rays, state=state, rgb_pixels=scene.width * scene.height if rgb else 0,
+                          color=self.n_hittables > 0)
         sc = scene.to_c()
         rc = self.lib.oracle_render(C.byref(sc), C.byref(outputs_struct(o)), self.threads)
         if rc:
@@ -212,7 +240,7 @@ class Oracle:
 
     def render_rays(self, scene: Scene, idx, state: bool = True) -> dict:
         idx = np.ascontiguousarray(idx, dtype=np.uint64)
-        o = alloc_outputs(len(idx), state=state)
+        o = alloc_outputs(len(idx), state=state, color=self.n_hittables > 0)
         sc = scene.to_c()
         rc = self.lib.oracle_render_rays(C.byref(sc), len(idx), idx.ctypes.data, C.byref(outputs_struct(o)),
                                          self.threads)
@@ -223,7 +251,7 @@ class Oracle:
 
     def trace_states(self, scene: Scene, states) -> dict:
         states = _f64(states).reshape(-1, 9)
-        o = alloc_outputs(len(states))
+        o = alloc_outputs(len(states), color=self.n_hittables > 0)
         sc = scene.to_c()
         rc = self.lib.oracle_trace_states(C.byref(sc), len(states), states.ctypes.data, C.byref(outputs_struct(o)),
                                           self.threads)

@@ -164,6 +164,7 @@ double disk_emission(const DiskT<double> &d, const V4<double> &pos) {
 template <class T> struct TraceOut {
   uint8_t cls;
   T intensity;
+  double color[3] = {0.0, 0.0, 0.0};   // RayOutcome::Scene { color }
   uint32_t attempts, accepted;
   uint64_t rhs;
   State<T> st;
@@ -178,6 +179,119 @@ struct StepLog {
 };
 
 template <class T> T logistic(const T &x) { using std::exp; return 1.0 / (1.0 + exp(-x)); }
+
+void disk_color(double intensity, double rgb[3]);
+
+// ---- Hittable scene: World::hit over Sphere / Quad, shade_hit, static g^4 ----
+struct HitObj {
+  int32_t shape, material;
+  double q[3], u[3], v[3], radius, color[3], fuzz;
+  // Quad::with_arc_material (quad.rs:24-72)
+  double normal[3], d, e_alpha[3], e_beta[3];
+};
+std::vector<HitObj> g_objs;
+
+V3<double> v3(const double *p) { return V3<double>{{p[0], p[1], p[2]}}; }
+
+// Sphere::hit (sphere.rs:53-88) / Quad::hit (quad.rs:75-127) on the
+// interval (t_min, t_max): root, point and outward normal.
+bool hit_one(const HitObj &h, const V3<double> &o, const V3<double> &dir, double t_min, double t_max, double &t,
+             V3<double> &pt, V3<double> &outward) {
+  auto surrounds = [&](double x) { return t_min < x && x < t_max; };   // Interval::surrounds
+  if (h.shape == GG_SHAPE_SPHERE) {
+    V3<double> oc{{o.x[0] - h.q[0], o.x[1] - h.q[1], o.x[2] - h.q[2]}};
+    double a = dot3(dir, dir);
+    double half_b = dot3(oc, dir);
+    double c = dot3(oc, oc) - h.radius * h.radius;
+    double disc = half_b * half_b - a * c;
+    if (disc < 0.0) return false;
+    double sq = std::sqrt(disc);
+    double root = (-half_b - sq) / a;
+    if (!surrounds(root)) {
+      root = (-half_b + sq) / a;
+      if (!surrounds(root)) return false;
+    }
+    t = root;
+    for (int k = 0; k < 3; ++k) pt.x[k] = o.x[k] + dir.x[k] * root;                 // ray.at
+    for (int k = 0; k < 3; ++k) outward.x[k] = (pt.x[k] - h.q[k]) / h.radius;
+    return true;
+  }
+  V3<double> nrm = v3(h.normal);
+  double denom = dot3(nrm, dir);
+  if (std::fabs(denom) < 1e-8) return false;
+  double tq = (h.d - dot3(nrm, o)) / denom;
+  if (!surrounds(tq)) return false;
+  for (int k = 0; k < 3; ++k) pt.x[k] = o.x[k] + dir.x[k] * tq;
+  V3<double> planar{{pt.x[0] - h.q[0], pt.x[1] - h.q[1], pt.x[2] - h.q[2]}};
+  double alpha = dot3(planar, v3(h.e_alpha)), beta = dot3(planar, v3(h.e_beta));
+  if (alpha < 0.0 || alpha > 1.0 || beta < 0.0 || beta > 1.0) return false;
+  t = tq;
+  outward = nrm;
+  return true;
+}
+
+// World::hit (scene/world.rs:67-80) on (0, 1), then HitRecord::new
+// (core/hit.rs:26-48) orients the normal against the ray.
+bool world_hit(const V3<double> &o, const V3<double> &dir, int &obj, V3<double> &pt, V3<double> &normal) {
+  double closest = 1.0;
+  bool any = false;
+  for (size_t i = 0; i < g_objs.size(); ++i) {
+    double t;
+    V3<double> p, out;
+    if (hit_one(g_objs[i], o, dir, 0.0, closest, t, p, out)) {
+      closest = t; any = true; obj = (int)i; pt = p; normal = out;
+    }
+  }
+  if (any && !(dot3(dir, normal) < 0.0))
+    for (int k = 0; k < 3; ++k) normal.x[k] = -normal.x[k];
+  return any;
+}
+
+// shade_hit (tracer.rs:369-384) with each material's scatter attenuation and
+// emission: Lambertian albedo (lambertian.rs:41-56), Metal albedo when the
+// mirror direction leaves the surface (metal.rs:40-57, fuzz 0), Dielectric
+// white (dielectric.rs:65-90), DiffuseLight no scatter and its emission
+// (diffuse_light.rs:22-35).
+void shade_hit(const HitObj &h, const V3<double> &dir, const V3<double> &normal, double lin[3]) {
+  double albedo[3] = {0.0, 0.0, 0.0}, emission[3] = {0.0, 0.0, 0.0};
+  V3<double> unit = normalize3(dir);
+  switch (h.material) {
+    case GG_MATERIAL_LAMBERTIAN: for (int k = 0; k < 3; ++k) albedo[k] = h.color[k]; break;
+    case GG_MATERIAL_METAL: {
+      double dn = dot3(unit, normal);
+      V3<double> refl;
+      for (int k = 0; k < 3; ++k) refl.x[k] = unit.x[k] - 2.0 * dn * normal.x[k];  // v - 2 v.n n
+      if (dot3(refl, normal) > 0.0) for (int k = 0; k < 3; ++k) albedo[k] = h.color[k];
+      break;
+    }
+    case GG_MATERIAL_DIELECTRIC: albedo[0] = albedo[1] = albedo[2] = 1.0; break;
+    default: for (int k = 0; k < 3; ++k) emission[k] = h.color[k]; break;
+  }
+  V3<double> neg{{-unit.x[0], -unit.x[1], -unit.x[2]}};
+  double nd = std::fmax(dot3(neg, normal), 0.0);
+  double facing = 0.5 + 0.5 * nd;
+  for (int k = 0; k < 3; ++k) lin[k] = albedo[k] * facing + emission[k];
+}
+
+// spherical_to_cartesian (tracer.rs:12-18).
+V3<double> sph_to_cart(const V4<double> &p) {
+  double r = p[1], st = std::sin(p[2]);
+  return V3<double>{{r * st * std::cos(p[3]), r * st * std::sin(p[3]), r * std::cos(p[2])}};
+}
+
+// cartesian_to_spherical (tracer.rs:23-28) and redshift_g4_static (:40-52).
+double g4_static(const MetricP &mp, double a, const V3<double> &p, double g_tt_obs) {
+  double r = std::fmax(std::sqrt(p.x[0] * p.x[0] + p.x[1] * p.x[1] + p.x[2] * p.x[2]), 1e-10);
+  double z = p.x[2] / r;
+  z = z < -1.0 ? -1.0 : (z > 1.0 ? 1.0 : z);
+  V4<double> pe;
+  pe[0] = 0.0; pe[1] = r; pe[2] = std::acos(z); pe[3] = std::atan2(p.x[1], p.x[0]);
+  double g_tt_emit = -metric_tensor(mp, a, pe).tt;
+  if (g_tt_emit <= 0.0 || g_tt_obs <= 0.0) return 0.0;
+  double g = std::sqrt(g_tt_emit / g_tt_obs);
+  double g2 = g * g;
+  return g2 * g2;
+}
 
 // Per-ray loop.  GG_DISK_NONE restates trace_ray (tracer.rs:54-70),
 // GG_DISK_VOLUMETRIC restates trace_ray_with_disk (tracer.rs:80-152) and
@@ -203,6 +317,17 @@ TraceOut<T> trace(const MetricP &mp, const T &a, const IntegP &ip, const DiskT<T
     if (!(val(g_tt_obs) >= 0.0)) g_tt_obs = T(0.0);
     sqrt_neg_gtt_obs = sqrt(g_tt_obs);                                    // tracer.rs:94
   }
+  // trace_ray_with_scene (tracer.rs:207-248) without a disk and
+  // trace_ray_with_disk_and_scene (tracer.rs:258-364) with a volumetric one
+  const bool scene = !g_objs.empty() && (disk.kind == GG_DISK_NONE || disk.kind == GG_DISK_VOLUMETRIC);
+  double g_tt_obs_scene = 0.0;
+  double absorption = disk.absorption;
+  if constexpr (std::is_same<T, double>::value) {
+    if (scene) {
+      g_tt_obs_scene = -metric_tensor(mp, a, observer).tt;              // tracer.rs:216, 270
+      absorption = 0.1;                                                  // tracer.rs:268
+    }
+  }
   uint8_t cls = GG_CLASS_MAX_STEPS;
   uint32_t k = 0;
   for (; k < ip.max_steps; ++k) {
@@ -215,6 +340,30 @@ TraceOut<T> trace(const MetricP &mp, const T &a, const IntegP &ip, const DiskT<T
     if (log && log->recording) log->acc.push_back(info.accepted ? 1 : 0);
     o.rhs += info.rhs;
     if (info.accepted) ++o.accepted;
+
+    if constexpr (std::is_same<T, double>::value) {
+      if (scene) {
+        V3<double> cb = sph_to_cart(before.pos), ca = sph_to_cart(st.pos);
+        V3<double> seg{{ca.x[0] - cb.x[0], ca.x[1] - cb.x[1], ca.x[2] - cb.x[2]}};
+        if (std::sqrt(dot3(seg, seg)) > 0.0) {
+          int obj;
+          V3<double> pt, nrm;
+          if (world_hit(cb, seg, obj, pt, nrm)) {
+            double lin[3];
+            shade_hit(g_objs[obj], seg, nrm, lin);
+            double g4 = g4_static(mp, a, pt, g_tt_obs_scene);
+            if (disk.kind == GG_DISK_VOLUMETRIC) {
+              double dl[3];
+              disk_color(intensity, dl);
+              for (int c = 0; c < 3; ++c) o.color[c] = lin[c] * g4 * transmission + dl[c];
+            } else {
+              for (int c = 0; c < 3; ++c) o.color[c] = lin[c] * g4;
+            }
+            cls = GG_CLASS_SCENE; ++k; break;
+          }
+        }
+      }
+    }
 
     if (disk.kind == GG_DISK_VOLUMETRIC) {
       if constexpr (std::is_same<T, double>::value) {
@@ -231,7 +380,7 @@ TraceOut<T> trace(const MetricP &mp, const T &a, const IntegP &ip, const DiskT<T
             double g2 = g * g;
             double redshifted = j * (g2 * g2);                           // powi(4)
             intensity = intensity + redshifted * dlambda * transmission;
-            transmission *= std::exp(-rho * dlambda * disk.absorption);
+            transmission *= std::exp(-rho * dlambda * absorption);
             if (transmission < 1e-3) { cls = GG_CLASS_DISK; ++k; break; }
           }
         }
@@ -363,8 +512,9 @@ uint8_t to_u8(double c) {
 
 // shade_outcome_with (renderer.rs:44-53) with the HDRI sky when one is set,
 // else the procedural sky.
-void shade(uint8_t cls, double intensity, const double vel[4], double lin[3]) {
+void shade(uint8_t cls, double intensity, const double vel[4], const double color[3], double lin[3]) {
   switch (cls) {
+    case GG_CLASS_SCENE: for (int c = 0; c < 3; ++c) lin[c] = color[c]; break;
     case GG_CLASS_HORIZON: lin[0] = lin[1] = lin[2] = 0.0; break;
     case GG_CLASS_ESCAPED: {
       double d[3];
@@ -379,6 +529,8 @@ void shade(uint8_t cls, double intensity, const double vel[4], double lin[3]) {
 
 void store(const gg_outputs *out, uint64_t i, const TraceOut<double> &o) {
   if (out->cls) out->cls[i] = o.cls;
+  if (out->color && !g_objs.empty())
+    for (int c = 0; c < 3; ++c) out->color[3 * i + c] = o.color[c];
   if (out->r) out->r[i] = o.st.pos[1];
   if (out->theta) out->theta[i] = o.st.pos[2];
   if (out->phi) out->phi[i] = o.st.pos[3];
@@ -574,6 +726,54 @@ void oracle_encode_srgb(const double lin[3], uint8_t out[3]) {
   for (int i = 0; i < 3; ++i) out[i] = to_u8(lin[i]);
 }
 void oracle_sky(const double dir[3], double rgb[3]) { sky_sample(dir, rgb); }
+
+// The Hittable scene of oracle_render / oracle_trace_states (n = 0: none).
+int oracle_set_hittables(uint32_t n, const gg_hittable *objs) {
+  std::vector<HitObj> v(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const gg_hittable &g = objs[i];
+    HitObj &h = v[i];
+    h.shape = g.shape; h.material = g.material;
+    for (int k = 0; k < 3; ++k) { h.q[k] = g.q[k]; h.u[k] = g.u[k]; h.v[k] = g.v[k]; h.color[k] = g.color[k]; }
+    h.radius = g.radius;
+    h.fuzz = std::fmin(std::fmax(g.fuzz, 0.0), 1.0);                 // Metal::new clamps
+    if (g.material == GG_MATERIAL_METAL && h.fuzz != 0.0) return GG_ERR_UNSUPPORTED;
+    if (g.material < GG_MATERIAL_LAMBERTIAN || g.material > GG_MATERIAL_DIFFUSE_LIGHT) return GG_ERR_INVALID_ARG;
+    if (g.shape == GG_SHAPE_QUAD) {
+      V3<double> u = v3(g.u), w0 = v3(g.v);
+      V3<double> n = cross3(u, w0);
+      V3<double> nn = normalize3(n);
+      for (int k = 0; k < 3; ++k) h.normal[k] = nn.x[k];
+      h.d = dot3(nn, v3(g.q));
+      double n2 = dot3(n, n);
+      V3<double> w{{n.x[0] / n2, n.x[1] / n2, n.x[2] / n2}};
+      V3<double> ea = cross3(w0, w), eb = cross3(w, u);
+      for (int k = 0; k < 3; ++k) { h.e_alpha[k] = ea.x[k]; h.e_beta[k] = eb.x[k]; }
+    } else if (g.shape != GG_SHAPE_SPHERE) {
+      return GG_ERR_INVALID_ARG;
+    }
+  }
+  g_objs = std::move(v);
+  return 0;
+}
+
+// One chord against the scene: hit flag, object, point, normal and the
+// shade_hit colour (out[0..11]).
+int oracle_scene_hit(const double origin[3], const double dir[3], double out[12]) {
+  int obj = -1;
+  V3<double> pt, nrm;
+  if (!world_hit(v3(origin), v3(dir), obj, pt, nrm)) { out[0] = 0.0; return 0; }
+  double lin[3];
+  shade_hit(g_objs[obj], v3(dir), nrm, lin);
+  out[0] = 1.0; out[1] = obj;
+  for (int k = 0; k < 3; ++k) { out[2 + k] = pt.x[k]; out[5 + k] = nrm.x[k]; out[8 + k] = lin[k]; }
+  return 1;
+}
+
+double oracle_g4_static(const gg_metric *m, const double point[3], double g_tt_obs) {
+  MetricP mp = make_metric(*m);
+  return g4_static(mp, mp.spin, v3(point), g_tt_obs);
+}
 // The sky of oracle_render (width = 0: the procedural sky).
 int oracle_set_sky_hdri(uint32_t width, uint32_t height, const float *pixels, float intensity) {
   if (width == 0) { g_hdri = HdriSkyT{}; return 0; }
@@ -645,7 +845,7 @@ int oracle_render(const gg_scene *s, const gg_outputs *out, int nthreads) {
           rhs.fetch_add(o.rhs, std::memory_order_relaxed);
           double lin[3];
           double vel[4] = {o.st.vel[0], o.st.vel[1], o.st.vel[2], o.st.vel[3]};
-          shade(o.cls, o.intensity, vel, lin);
+          shade(o.cls, o.intensity, vel, o.color, lin);
           acc[0] += lin[0]; acc[1] += lin[1]; acc[2] += lin[2];
         }
       if (out->rgb) {

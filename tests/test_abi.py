@@ -39,7 +39,8 @@ def test_struct_sizes_match_c_layout():
     assert C.sizeof(_abi.gg_integrator) == 80
     assert C.sizeof(_abi.gg_image) == 24
     assert C.sizeof(_abi.gg_scene) == 24 + 96 + 72 + 80 + 24
-    assert C.sizeof(_abi.gg_outputs) == 72
+    assert C.sizeof(_abi.gg_outputs) == 80
+    assert C.sizeof(_abi.gg_hittable) == 128
 
 
 def test_python_mirror_matches_c_field_offsets(tmp_path):
