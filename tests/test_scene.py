@@ -228,8 +228,15 @@ def _compare(got, want, min_agree=0.999):
                                                                                 1e-300)
     assert np.mean(d <= 1e-9) >= 0.999, np.sort(d)[-5:]
     if "rgb" in want:
+        # Escaped rays meet the procedural sky, which hashes the raw bits of
+        # the escape direction (sky.rs:26-40): a last-bit difference can add
+        # or remove a star, so those pixels have a looser bound (one ray per
+        # pixel here).
         px = np.abs(got["rgb"].astype(int) - want["rgb"].astype(int)).max(axis=1)
-        assert np.mean(px <= 1) >= 0.995
+        esc = want["cls"] == gg.GG_CLASS_ESCAPED
+        assert np.mean(px[~esc] <= 1) >= 0.999
+        if esc.any():
+            assert np.mean(px[esc] <= 1) >= 0.9
 
 
 @pytest.mark.gpu
