@@ -219,6 +219,60 @@ def bench_kerr4096(args):
             "mixed_vs_fp64": cmp}
 
 
+def bench_scene(args):
+    """The reference CLI's lensed-spheres demo (main.rs:78-202): Schwarzschild M=1, camera r=30M looking at the
+    hole, four Lambertian spheres and a light behind it, 800x600, without and with the volumetric disk."""
+    import torch
+    import gargantua_amd as gg
+    from oracle import Oracle
+    w, h = (800, 600) if args.size == 1024 else (args.size, args.size * 3 // 4)
+    world = gg.configs.lensed_spheres_world()
+    cam = gg.Camera(position=(0.0, 30.0, math.pi / 2, 0.0), look_at=(-1.0, 0.0, 0.0), up=(0.0, 0.0, 1.0),
+                    fov_y_radians=math.radians(60.0), aspect=w / h)
+    rend = gg.Renderer(0, stream=torch.cuda.current_stream().cuda_stream)
+    rend.set_hittables(world)
+    out = {}
+    for name, disk in (("scene", None), ("disk_scene", gg.AccretionDisk())):
+        scene = gg.Scene(metric=gg.Schwarzschild(1.0), camera=cam, disk=disk,
+                         integrator=gg.RK45Integrator(max_radius=200.0, max_steps=5000), width=w, height=h)
+        n = scene.n_rays
+        res, host = {}, {}
+        for prec in ("mixed", "fp64"):
+            outs = _dev_outputs(torch, n)
+            outs["color"] = torch.empty(3 * n, dtype=torch.float64, device="cuda")
+            wall, kms, st = time_precision(torch, rend, scene, prec, max(1, args.steps), 1, outs)
+            kms = sorted(kms)
+            res[prec] = {"geodesics_per_s": n / (wall / max(1, args.steps)), "kernel_ms": kms[len(kms) // 2],
+                         "attempts_per_ray": st["attempts"] / n}
+            host[prec] = {k: v.cpu().numpy() for k, v in outs.items()}
+            _log(f"scene {name} {prec}: {res[prec]['kernel_ms']:.2f} ms")
+        cm, cf = host["mixed"]["cls"], host["fp64"]["cls"]
+        hit = (cf == gg.GG_CLASS_SCENE) & (cm == cf)
+        col = host["mixed"]["color"].reshape(-1, 3)[hit] - host["fp64"]["color"].reshape(-1, 3)[hit]
+        res["classes"] = np.bincount(cf, minlength=5).tolist()
+        res["mixed_vs_fp64"] = {"class_agree": float(np.mean(cm == cf)),
+                                "scene_color_max_abs_diff": float(np.abs(col).max()) if hit.any() else 0.0}
+        res["mixed_speedup"] = res["fp64"]["kernel_ms"] / res["mixed"]["kernel_ms"]
+        if not args.no_cpu:
+            ora = Oracle(threads=max(1, min(16, len(os.sched_getaffinity(0)))))
+            ora.set_hittables(world)
+            idx = np.array([py * w + px for py in range(0, h, 8) for px in range(0, w, 8)], dtype=np.uint64)
+            t0 = time.perf_counter()
+            ora.render_rays(scene, idx, state=False)
+            dt = time.perf_counter() - t0
+            ora.set_hittables(None)
+            res["cpu_baseline"] = {"value": len(idx) / dt, "unit": "geodesics/s", "cores": ora.threads,
+                                   "kind": "port", "sample": f"every 8th row and column ({len(idx)} rays)"}
+        out[name] = res
+    rend.set_hittables(None)
+    rend.close()
+    return {"metric": f"geodesics/s lensed-spheres {w}x{h}", "value": out["scene"]["mixed"]["geodesics_per_s"],
+            "unit": "geodesics/s", "n_gpus": 1, "dtype": "f32+f64",
+            "config": {"workload": f"lensed-spheres {w}x{h} Schwarzschild M=1, camera r=30M FOV 60deg, "
+                                   "5 spheres, max_radius 200M, 5000 attempts"},
+            **out}
+
+
 # Algorithmic FLOPs of one accepted step back-propagated by the adjoint
 # reverse kernel: six reverse-mode RHS vector-Jacobian products (628 each,
 # counted from the source of kerr_rhs_vjp: 130 for the forward symbol
@@ -340,7 +394,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, default=1024)
-    ap.add_argument("--workload", default="kerr1024", choices=["kerr1024", "batch64", "kerr4096", "grad512", "copy"],
+    ap.add_argument("--workload", default="kerr1024", choices=["kerr1024", "batch64", "kerr4096", "grad512", "scene", "copy"],
                     help="copy: one 512 MiB device-to-device copy, the reference kernel that calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters for HBM traffic")
     ap.add_argument("--gd-steps", type=int, default=200, help="grad512: gradient-descent iterations (0: none)")
@@ -357,6 +411,9 @@ def main():
         return
     if args.workload == "grad512":
         print(json.dumps(bench_grad512(args)))
+        return
+    if args.workload == "scene":
+        print(json.dumps(bench_scene(args)))
         return
     if args.workload == "copy":
         import torch

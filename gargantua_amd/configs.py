@@ -120,3 +120,16 @@ def grad_512_params(width: int = 512, height: int = 512, spin=0.5, inclination_d
     g.height = height
     g.precision = {"fp64": _abi.GG_PREC_FP64, "mixed": _abi.GG_PREC_MIXED}[precision]
     return g
+
+
+def lensed_spheres_world():
+    """build_lensed_spheres (crates/gr-renderer/src/main.rs:173-202): a light behind the hole and four
+    Lambertian spheres in a cross around the line of sight, at r = 40-60M."""
+    from . import DiffuseLight, Lambertian, Sphere, World
+    return (World()
+            .add(Sphere((-60.0, 0.0, 0.0), 4.0, DiffuseLight((2.0, 1.6, 0.8))))
+            .add(Sphere((-40.0, 14.0, 0.0), 3.5, Lambertian((0.85, 0.10, 0.10))))
+            .add(Sphere((-40.0, -14.0, 0.0), 3.5, Lambertian((0.10, 0.75, 0.20))))
+            .add(Sphere((-40.0, 0.0, 14.0), 3.5, Lambertian((0.15, 0.30, 0.90))))
+            .add(Sphere((-40.0, 0.0, -14.0), 3.5, Lambertian((0.95, 0.85, 0.20))))
+            .build_bvh())

@@ -18,14 +18,7 @@ import gargantua_amd as gg
 
 
 def lensed_spheres():
-    """build_lensed_spheres (crates/gr-renderer/src/main.rs:173-202)."""
-    return (gg.World()
-            .add(gg.Sphere((-60.0, 0.0, 0.0), 4.0, gg.DiffuseLight((2.0, 1.6, 0.8))))
-            .add(gg.Sphere((-40.0, 14.0, 0.0), 3.5, gg.Lambertian((0.85, 0.10, 0.10))))
-            .add(gg.Sphere((-40.0, -14.0, 0.0), 3.5, gg.Lambertian((0.10, 0.75, 0.20))))
-            .add(gg.Sphere((-40.0, 0.0, 14.0), 3.5, gg.Lambertian((0.15, 0.30, 0.90))))
-            .add(gg.Sphere((-40.0, 0.0, -14.0), 3.5, gg.Lambertian((0.95, 0.85, 0.20))))
-            .build_bvh())
+    return gg.configs.lensed_spheres_world()
 
 
 def scene_camera(r=30.0, fov_deg=60.0):
