@@ -253,6 +253,13 @@ def bench_scene(args):
         res["mixed_vs_fp64"] = {"class_agree": float(np.mean(cm == cf)),
                                 "scene_color_max_abs_diff": float(np.abs(col).max()) if hit.any() else 0.0}
         res["mixed_speedup"] = res["fp64"]["kernel_ms"] / res["mixed"]["kernel_ms"]
+        # the same rays without the scene: what the chord tests cost on top
+        rend.set_hittables(None)
+        for prec in ("mixed", "fp64"):
+            outs = _dev_outputs(torch, n)
+            _, kms, _ = time_precision(torch, rend, scene, prec, max(1, args.steps), 1, outs)
+            res[prec]["kernel_ms_without_scene"] = sorted(kms)[len(kms) // 2]
+        rend.set_hittables(world)
         if not args.no_cpu:
             ora = Oracle(threads=max(1, min(16, len(os.sched_getaffinity(0)))))
             ora.set_hittables(world)

@@ -47,13 +47,18 @@ __device__ inline double sdot3(const double a[3], const double b[3]) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];           // nalgebra dot, U3: (a + b) + c
 }
 
-// spherical_to_cartesian (tracer.rs:12-18).
+// spherical_to_cartesian (tracer.rs:12-18).  sincos shares one argument
+// reduction between the sine and the cosine and returns the values sin and
+// cos return.
 __device__ inline void sph2cart(const double p[4], double c[3]) {
 #pragma clang fp contract(off)
-  const double r = p[1], st = sin(p[2]);
-  c[0] = r * st * cos(p[3]);
-  c[1] = r * st * sin(p[3]);
-  c[2] = r * cos(p[2]);
+  double st, ct, sp, cp;
+  sincos(p[2], &st, &ct);
+  sincos(p[3], &sp, &cp);
+  const double r = p[1];
+  c[0] = r * st * cp;
+  c[1] = r * st * sp;
+  c[2] = r * ct;
 }
 
 // Slab test of [lo, hi] against origin + t dir, t in (0, t_max).  The host
