@@ -32,6 +32,7 @@ import numpy as np
 
 from . import _abi
 from . import configs  # noqa: F401  (scene builders for the benchmark configurations)
+from . import hdr, png  # noqa: F401  (Radiance .hdr input, PNG output)
 from ._abi import (GG_CLASS_DISK, GG_CLASS_ESCAPED, GG_CLASS_HORIZON, GG_CLASS_MAX_STEPS, GG_CLASS_SCENE,  # noqa: F401
                    GG_DISK_NONE, GG_DISK_THIN, GG_DISK_VOLUMETRIC, GG_ERR_REFERENCE, GG_ERR_RTOL_ATOL,
                    GG_METRIC_KERR, GG_METRIC_SCHWARZSCHILD)
