@@ -79,6 +79,26 @@ class Kerr:
 # ------------------------------------------------------- scene components --
 
 @dataclass
+class UserMetric:
+    """A metric known only by its tensor (a Metric implementing metric_tensor alone,
+    crates/gr-core/src/metric.rs:11-80).
+
+    `source` is HIP device code defining
+        __device__ void gg_user_metric(const double x[4], double mass, double spin, double g[16])
+    that fills g[4*mu + nu] = g_mu,nu at x = (t, r, theta, phi); mass and spin are passed through.  The
+    library compiles it at run time into its trace kernel; Christoffel symbols are central differences
+    (metric.rs:97-142).  event_horizon is Metric::event_horizon (None: no horizon test).
+    """
+    source: str
+    mass: float = 1.0
+    spin: float = 0.0
+    event_horizon: Optional[float] = None
+
+    def to_c(self) -> _abi.gg_metric:
+        return _abi.gg_metric(kind=_abi.GG_METRIC_USER, mass=self.mass, spin=self.spin)
+
+
+@dataclass
 class RK45Integrator:
     """RK45Integrator defaults (geodesic.rs:170-181) plus the rtol/atol mode."""
     initial_step: float = 0.1
@@ -421,6 +441,7 @@ class Renderer:
     def render(self, scene: Scene, precision: str = "fp64", outputs: Optional[dict] = None, rgb: bool = False,
                state: bool = True) -> dict:
         """Trace every ray of `scene`; returns the per-ray arrays and stats."""
+        self._ensure_metric(scene)
         sc = scene.to_c()
         o = outputs if outputs is not None else alloc_outputs(
             scene.n_rays, state=state, rgb_pixels=scene.width * scene.height if rgb else 0,
@@ -457,6 +478,7 @@ class Renderer:
 
     def trace_states(self, scene: Scene, states: np.ndarray, precision: str = "fp64") -> dict:
         """trace_ray / trace_ray_with_disk from given states [n, 9] (t r th ph vt vr vth vph lambda)."""
+        self._ensure_metric(scene)
         states = np.ascontiguousarray(states, dtype=np.float64)
         n = states.shape[0]
         o = alloc_outputs(n, color=self.n_hittables > 0)
@@ -528,6 +550,21 @@ class Renderer:
         arr = world.to_c()
         _check(self.lib, self.lib.gg_set_hittables(self.ctx, len(world), C.cast(arr, C.c_void_p)), "gg_set_hittables")
         self.n_hittables = len(world)
+
+    def set_user_metric(self, metric: Optional[UserMetric]) -> None:
+        """Compile `metric.source` for scenes whose metric is a UserMetric (None: clear)."""
+        if metric is None:
+            _check(self.lib, self.lib.gg_set_user_metric(self.ctx, None, 0.0), "gg_set_user_metric")
+            self._user_metric = None
+            return
+        rh = float("nan") if metric.event_horizon is None else float(metric.event_horizon)
+        _check(self.lib, self.lib.gg_set_user_metric(self.ctx, metric.source.encode(), rh), "gg_set_user_metric")
+        self._user_metric = (metric.source, metric.event_horizon)
+
+    def _ensure_metric(self, scene) -> None:
+        m = scene.metric
+        if isinstance(m, UserMetric) and getattr(self, "_user_metric", None) != (m.source, m.event_horizon):
+            self.set_user_metric(m)
 
     def set_grad_memory(self, nbytes: int = 0) -> None:
         """Device memory grad()'s automatic mode may use for step records (0: a quarter of the free memory)."""

@@ -7,6 +7,7 @@
 // memory, launches the persistent trace kernel and times it with hipEvents
 // on the context's stream.
 #include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
 
 #include <algorithm>
 #include <chrono>
@@ -21,6 +22,9 @@
 #include "gargantua.h"
 #include "gg_internal.hpp"
 #include "gg_scene.hpp"
+// kRtcHeaderNames / kRtcHeaderSources / kRtcHeaderCount: the text of the
+// device headers, written at build time, for modules compiled at run time
+#include "gg_rtc_headers.inc"
 
 using namespace gg;
 
@@ -115,6 +119,13 @@ struct gg_context {
   DevBuf hit_objs, hit_nodes;
   uint32_t n_hit = 0;
   bool hit_off = false;
+  // gg_set_user_metric: the tensor's source, its horizon, and the trace
+  // modules compiled from it, one per (disk mode 0-4, ray source 0-1)
+  std::string user_src;
+  bool user_has_rh = false;
+  double user_rh = 0.0;
+  hipModule_t user_mod[10] = {};
+  hipFunction_t user_trace[10] = {}, user_at[10] = {};
   // the recording render gg_grad asks render_impl for (pool null: none)
   struct {
     double *pool;
@@ -125,6 +136,8 @@ struct gg_context {
 };
 
 namespace {
+
+int user_kernels(gg_context *ctx, int mode, int src, hipFunction_t *trace, hipFunction_t *at);
 
 // ---- Hittable scene: primitive constants and the BVH ----
 
@@ -284,9 +297,14 @@ void cross3(const double a[3], const double b[3], double r[3]) {
 
 bool finite(double x) { return std::isfinite(x); }
 
-int validate_metric(const gg_metric &m) {
+int validate_metric(const gg_metric &m, bool allow_user = false) {
+  if (m.kind == GG_METRIC_USER && allow_user) {
+    if (!finite(m.mass) || !finite(m.spin)) return fail(GG_ERR_INVALID_ARG, "metric.mass and spin must be finite");
+    return GG_OK;
+  }
   if (m.kind != GG_METRIC_SCHWARZSCHILD && m.kind != GG_METRIC_KERR)
-    return fail(GG_ERR_INVALID_ARG, "metric.kind %d is not Schwarzschild (0) or Kerr (1)", m.kind);
+    return fail(GG_ERR_INVALID_ARG, "metric.kind %d is not Schwarzschild (0) or Kerr (1)%s", m.kind,
+                m.kind == GG_METRIC_USER ? " (a user metric renders through gg_render / gg_trace_states)" : "");
   if (!(m.mass > 0.0) || !finite(m.mass)) return fail(GG_ERR_INVALID_ARG, "metric.mass must be positive and finite");
   if (!finite(m.spin)) return fail(GG_ERR_INVALID_ARG, "metric.spin must be finite");
   return GG_OK;
@@ -328,7 +346,7 @@ int validate_disk(const gg_disk &d) {
 int validate_scene(const gg_scene *s, bool need_image) {
   if (!s) return fail(GG_ERR_INVALID_ARG, "scene is NULL");
   int rc;
-  if ((rc = validate_metric(s->metric))) return rc;
+  if ((rc = validate_metric(s->metric, true))) return rc;
   if ((rc = validate_integrator(s->integrator))) return rc;
   if ((rc = validate_disk(s->disk))) return rc;
   for (int i = 0; i < 4; ++i)
@@ -364,7 +382,15 @@ double clamp_spin(const gg_metric &m) {
   return s;
 }
 
-void build_dev_scene(const gg_scene &s, DevScene &D) {
+// The parts of a GG_METRIC_USER scene the host cannot compute itself: the
+// tensor at the camera (evaluated on the device) and the horizon.
+struct UserMetricAt {
+  double g[16];
+  bool has_rh;
+  double rh;
+};
+
+void build_dev_scene(const gg_scene &s, DevScene &D, const UserMetricAt *um = nullptr) {
   std::memset(&D, 0, sizeof(D));
   D.metric_kind = s.metric.kind;
   D.disk_kind = s.disk.kind;
@@ -373,13 +399,18 @@ void build_dev_scene(const gg_scene &s, DevScene &D) {
   D.mass = s.metric.mass;
   D.spin = clamp_spin(s.metric);
   D.rs = 2.0 * s.metric.mass;
-  if (s.metric.kind == GG_METRIC_KERR) {
+  if (s.metric.kind == GG_METRIC_USER && um) {
+    // Metric::event_horizon: None disables both horizon tests (geodesic.rs:198-206)
+    D.spin = s.metric.spin;
+    D.r_h = um->has_rh ? um->rh : -INFINITY;
+  } else if (s.metric.kind == GG_METRIC_KERR) {
     double disc = D.mass * D.mass - D.spin * D.spin;  // kerr.rs:427-436 (disc > 0 after the clamp)
     D.r_h = D.mass + std::sqrt(disc);
   } else {
     D.r_h = D.rs;
   }
   D.r_capture = D.r_h * (1.0 + s.integrator.horizon_rel) + s.integrator.horizon_buffer;
+  if (!(D.r_h > -INFINITY)) D.r_capture = -INFINITY;
   D.max_radius = s.integrator.max_radius;
   D.h0 = s.integrator.initial_step;
   D.min_step = s.integrator.min_step;
@@ -408,7 +439,11 @@ void build_dev_scene(const gg_scene &s, DevScene &D) {
   D.thhat[0] = ct * cp; D.thhat[1] = ct * sp; D.thhat[2] = -st;
   D.phhat[0] = -sp; D.phhat[1] = cp; D.phhat[2] = 0.0;
   double tph;
-  metric_host(s.metric.kind, D.mass, D.spin, D.rs, c.position, D.g_tt, D.g_rr, D.g_thth, D.g_phph, tph);
+  if (s.metric.kind == GG_METRIC_USER && um) {
+    D.g_tt = um->g[0]; D.g_rr = um->g[5]; D.g_thth = um->g[10]; D.g_phph = um->g[15];
+  } else {
+    metric_host(s.metric.kind, D.mass, D.spin, D.rs, c.position, D.g_tt, D.g_rr, D.g_thth, D.g_phph, tph);
+  }
   D.sqrt_g_rr = std::sqrt(D.g_rr);
   D.sqrt_g_thth = std::sqrt(D.g_thth);
   D.sqrt_g_phph = std::sqrt(D.g_phph);
@@ -543,6 +578,8 @@ int gg_context_destroy(gg_context *ctx) {
   for (auto &b : ctx->stage) b.release();
   for (auto &b : ctx->batch_stage) b.release();
   ctx->hit_objs.release(); ctx->hit_nodes.release(); ctx->sky.release();
+  for (auto &m : ctx->user_mod)
+    if (m) (void)hipModuleUnload(m);
   if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -625,6 +662,30 @@ int gg_set_hittables(gg_context *ctx, uint32_t n, const gg_hittable *objects) {
   return GG_OK;
 }
 
+int gg_set_user_metric(gg_context *ctx, const char *source, double event_horizon) {
+  if (!ctx) return fail(GG_ERR_INVALID_ARG, "NULL context");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < 10; ++k) {
+    if (ctx->user_mod[k]) (void)hipModuleUnload(ctx->user_mod[k]);
+    ctx->user_mod[k] = nullptr;
+    ctx->user_trace[k] = ctx->user_at[k] = nullptr;
+  }
+  ctx->user_src.clear();
+  if (!source) return GG_OK;
+  if (!*source) return fail(GG_ERR_INVALID_ARG, "user metric source is empty");
+  if (!(std::isnan(event_horizon) || std::isfinite(event_horizon)))
+    return fail(GG_ERR_INVALID_ARG, "event_horizon must be finite or NaN (None)");
+  ctx->user_src = source;
+  ctx->user_has_rh = !std::isnan(event_horizon);
+  ctx->user_rh = ctx->user_has_rh ? event_horizon : 0.0;
+  // compile the camera-ray, no-disk kernel now, so that a bad source fails here
+  hipFunction_t t = nullptr, a = nullptr;
+  int rc = user_kernels(ctx, 0, 0, &t, &a);
+  if (rc) ctx->user_src.clear();
+  return rc;
+}
+
 int gg_set_grad_memory(gg_context *ctx, uint64_t bytes) {
   if (!ctx) return fail(GG_ERR_INVALID_ARG, "NULL context");
   ctx->grad_budget = bytes;
@@ -650,6 +711,80 @@ int gg_scene_rays(const gg_scene *scene, uint64_t *n_rays) {
 namespace {
 
 // Common body of gg_render / gg_render_batch / gg_trace_states.
+// The trace kernel for METRIC 2 (gg_user_metric.hpp) with the context's
+// tensor source, disk mode `mode` and ray source `src`, compiled with hipRTC
+// on first use, and the helper kernel that evaluates the tensor at a point.
+int user_kernels(gg_context *ctx, int mode, int src, hipFunction_t *trace, hipFunction_t *at) {
+  const int key = 2 * mode + src;
+  if (ctx->user_trace[key]) {
+    *trace = ctx->user_trace[key];
+    *at = ctx->user_at[key];
+    return GG_OK;
+  }
+  const std::string prog = "using namespace __hip_internal;\n#include \"gg_trace_kernel.hpp\"\nnamespace gg {\n" +
+                           ctx->user_src +
+                           "\n__global__ void gg_user_metric_at(const double *x, double mass, double spin, double *g) {\n"
+                           "  gg_user_metric(x, mass, spin, g);\n}\n}  // namespace gg\n";
+  hiprtcProgram p;
+  if (hiprtcCreateProgram(&p, prog.c_str(), "gg_user_metric.hip", kRtcHeaderCount, kRtcHeaderSources,
+                          kRtcHeaderNames) != HIPRTC_SUCCESS)
+    return fail(GG_ERR_INTERNAL, "hiprtcCreateProgram failed");
+  char expr[96];
+  std::snprintf(expr, sizeof expr, "gg::trace_kernel<2, %d, 0, %d, 0>", mode, src);
+  const char *at_expr = "gg::gg_user_metric_at";
+  hiprtcAddNameExpression(p, expr);
+  hiprtcAddNameExpression(p, at_expr);
+  // the library's own flags: gfx950, no FMA contraction, no packed FP32
+  const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult r = hiprtcCompileProgram(p, (int)(sizeof opts / sizeof opts[0]), opts);
+  if (r != HIPRTC_SUCCESS) {
+    size_t n = 0;
+    hiprtcGetProgramLogSize(p, &n);
+    std::string log(n + 1, '\0');
+    hiprtcGetProgramLog(p, &log[0]);
+    hiprtcDestroyProgram(&p);
+    return fail(GG_ERR_INVALID_ARG, "user metric does not compile: %.1500s", log.c_str());
+  }
+  const char *lt = nullptr, *la = nullptr;
+  hiprtcGetLoweredName(p, expr, &lt);
+  hiprtcGetLoweredName(p, at_expr, &la);
+  const std::string name_t = lt ? lt : "", name_a = la ? la : "";
+  size_t code_n = 0;
+  hiprtcGetCodeSize(p, &code_n);
+  std::vector<char> code(code_n);
+  hiprtcGetCode(p, code.data());
+  hiprtcDestroyProgram(&p);
+  if (name_t.empty() || name_a.empty() || code_n == 0) return fail(GG_ERR_INTERNAL, "hipRTC returned no kernel");
+  hipModule_t mod = nullptr;
+  HIP_TRY(hipModuleLoadData(&mod, code.data()));
+  hipFunction_t ft = nullptr, fa = nullptr;
+  hipError_t e1 = hipModuleGetFunction(&ft, mod, name_t.c_str());
+  hipError_t e2 = hipModuleGetFunction(&fa, mod, name_a.c_str());
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    (void)hipModuleUnload(mod);
+    return fail(GG_ERR_DEVICE, "hipModuleGetFunction: %s / %s", hipGetErrorString(e1), hipGetErrorString(e2));
+  }
+  ctx->user_mod[key] = mod;
+  *trace = ctx->user_trace[key] = ft;
+  *at = ctx->user_at[key] = fa;
+  return GG_OK;
+}
+
+// The user tensor at x, on the device (one thread of the helper kernel).
+int user_metric_at(gg_context *ctx, hipFunction_t at, const gg_metric &m, const double x[4], double g[16]) {
+  int rc;
+  if ((rc = ctx->misc_io.ensure(256))) return rc;
+  double *dx = (double *)ctx->misc_io.p, *dg = dx + 4;
+  HIP_TRY(hipMemcpyAsync(dx, x, 32, hipMemcpyHostToDevice, ctx->stream));
+  double mass = m.mass, spin = m.spin;
+  void *args[] = {&dx, &mass, &spin, &dg};
+  HIP_TRY(hipModuleLaunchKernel(at, 1, 1, 1, 1, 1, 1, 0, ctx->stream, args, nullptr));
+  HIP_TRY(hipMemcpyAsync(g, dg, 128, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return GG_OK;
+}
+
 int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int precision, const gg_outputs *outs,
                 const double *states_host, uint64_t n_states, gg_stats *stats) {
   auto t_start = std::chrono::steady_clock::now();
@@ -675,6 +810,22 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
     mode = mode == GG_DISK_NONE ? 3 : 4;
   }
   HIP_TRY(hipSetDevice(ctx->device));
+  // A user metric: its trace kernel (compiled on first use) and its tensor
+  // at the camera.
+  const bool user = scenes[0].metric.kind == GG_METRIC_USER;
+  hipFunction_t user_trace = nullptr;
+  UserMetricAt um{};
+  if (user) {
+    if (ctx->user_src.empty()) return fail(GG_ERR_INVALID_ARG, "metric.kind is GG_METRIC_USER but no user metric is set");
+    if (n_scenes > 1) return fail(GG_ERR_UNSUPPORTED, "a user metric renders one scene per call");
+    if (precision != GG_PREC_FP64) return fail(GG_ERR_UNSUPPORTED, "a user metric renders in GG_PREC_FP64 only");
+    if (ctx->rec_req.pool) return fail(GG_ERR_UNSUPPORTED, "a user metric has no recording render");
+    hipFunction_t at = nullptr;
+    if ((rc = user_kernels(ctx, mode, from_states ? 1 : 0, &user_trace, &at))) return rc;
+    if ((rc = user_metric_at(ctx, at, scenes[0].metric, scenes[0].camera.position, um.g))) return rc;
+    um.has_rh = ctx->user_has_rh;
+    um.rh = ctx->user_rh;
+  }
 
   std::vector<DevScene> ds(n_scenes);
   std::vector<DevOut> dout(n_scenes);
@@ -682,7 +833,7 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
   uint64_t total = 0;
   if (ctx->batch_stage.size() < 10ull * n_scenes) ctx->batch_stage.resize(10ull * n_scenes);
   for (uint32_t i = 0; i < n_scenes; ++i) {
-    build_dev_scene(scenes[i], ds[i]);
+    build_dev_scene(scenes[i], ds[i], user ? &um : nullptr);
     uint64_t n = from_states ? n_states : (uint64_t)ds[i].sub_w * ds[i].sub_h;
     ds[i].ray_begin = total;
     ds[i].out_index = i;
@@ -802,7 +953,11 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
     HIP_TRY(launch_reset(T.counters, s));
     HIP_TRY(hipEventRecord(ctx->ev0, s));
     if (total > 0) {
-      if (mode >= 3) HIP_TRY(launch_trace_scene(scenes[0].metric.kind, mode, precision, T, grid, block, s));
+      if (user) {
+        void *args[] = {&T};
+        HIP_TRY(hipModuleLaunchKernel(user_trace, grid, 1, 1, block, 1, 1, (unsigned)((size_t)block * 5 * 4 * 8), s,
+                                      args, nullptr));
+      } else if (mode >= 3) HIP_TRY(launch_trace_scene(scenes[0].metric.kind, mode, precision, T, grid, block, s));
       else HIP_TRY(launch_trace(scenes[0].metric.kind, mode, precision, T, grid, block, s));
     }
     HIP_TRY(hipEventRecord(ctx->ev1, s));

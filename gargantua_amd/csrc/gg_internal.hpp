@@ -1,8 +1,10 @@
 // gg_internal.hpp -- declarations shared by the host ABI and the kernels.
 #pragma once
 
+#ifndef __HIPCC_RTC__  // hipRTC supplies the runtime and fixed-width types itself
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "gg_physics.hpp"
 
@@ -110,6 +112,17 @@ constexpr uint32_t GRAD_STEP_REC = 32;  // doubles per step record: pos, vel, v_
 constexpr uint32_t GRAD_CHUNK = 16;     // step records per chunk of the recording render
 constexpr uint32_t GRAD_CHUNK_HEAD = 16;   // the back-pointer's 128-byte line
 constexpr uint32_t GRAD_CHUNK_DOUBLES = GRAD_CHUNK_HEAD + GRAD_CHUNK * GRAD_STEP_REC;
+
+// Equirectangular environment map on the device (HdriSky, sky.rs:43-122),
+// or pixels null for the procedural starfield.
+struct SkyDesc {
+  const float *pixels;             // width x height x 3, row 0 = +z pole
+  uint32_t width, height;
+  float intensity;
+};
+
+// ---- host-side launchers (not part of a module compiled at run time) ----
+#ifndef __HIPCC_RTC__
 size_t adjoint_lds_bytes(uint32_t block);
 size_t adjoint_fwd_lds_bytes(uint32_t block);
 hipError_t launch_adjoint_fwd(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s);
@@ -142,16 +155,9 @@ hipError_t launch_camera_rays(const DevScene *scene, uint64_t n, double *states,
 // ids[i] = i: input of the cost-order sort.
 hipError_t launch_order_keys(const DevScene *scenes, const DevOut *outs, uint32_t n_scenes, uint64_t n,
                              uint32_t *keys, uint32_t *ids, hipStream_t stream);
-// Equirectangular environment map on the device (HdriSky, sky.rs:43-122),
-// or pixels null for the procedural starfield.
-struct SkyDesc {
-  const float *pixels;             // width x height x 3, row 0 = +z pole
-  uint32_t width, height;
-  float intensity;
-};
-
 // RGB image (renderer.rs:44-80 shading + sRGB) from per-ray outputs.
 hipError_t launch_shade(const DevScene *scene, const DevOut *out, uint32_t width, uint32_t height, uint32_t spa,
                         const SkyDesc &sky, uint8_t *rgb, hipStream_t stream);
+#endif
 
 }  // namespace gg

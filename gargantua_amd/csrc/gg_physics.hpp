@@ -13,8 +13,10 @@
 //            gg_rk45.hpp), where its rounding cannot change a decision.
 #pragma once
 
+#ifndef __HIPCC_RTC__  // hipRTC supplies the runtime and fixed-width types itself
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "gargantua.h"
 #include "gg_types.hpp"

@@ -19,9 +19,12 @@
 // steps therefore use the same h sequence as the FP64 kernel.
 #pragma once
 
+#ifndef __HIPCC_RTC__  // hipRTC supplies the runtime and fixed-width types itself
 #include <hip/hip_runtime.h>
+#endif
 
 #include "gg_internal.hpp"
+#include "gg_user_metric.hpp"
 
 namespace gg {
 
@@ -171,11 +174,13 @@ __device__ inline StageOut stages(const DevScene &S, const KStore &K, const doub
 #pragma unroll 1
   for (int s = 1; s <= 7; ++s) {
     double qr, qth;
+    double qt = 0.0, qph = 0.0;      // METRIC 2: the metric may depend on t and phi too
     d4 vs;
     bool eval = true;
     if (s == 1) {
       qr = p0[1];                                                   // k1 = a(pos, vel) (:245)
       qth = p0[2];
+      if constexpr (METRIC == 2) { qt = p0[0]; qph = p0[3]; }
       vs = vel;
       eval = need_k1;
     } else if (s <= 6) {
@@ -203,6 +208,7 @@ __device__ inline StageOut stages(const DevScene &S, const KStore &K, const doub
       }
       qr = q1;
       qth = q2;
+      if constexpr (METRIC == 2) { qt = q0; qph = q3; }
       if (REC && rec) {
         double *o = (double *)__builtin_assume_aligned(rec + REC_VS + 4 * (s - 2), 32);
         o[0] = vs.x; o[1] = vs.y; o[2] = vs.z; o[3] = vs.w;
@@ -210,11 +216,17 @@ __device__ inline StageOut stages(const DevScene &S, const KStore &K, const doub
     } else {
       qr = madd<ARITH>(h, vel.y, p0[1]);                            // k7 at (pos6, new_vel) (:298)
       qth = madd<ARITH>(h, vel.z, p0[2]);
+      if constexpr (METRIC == 2) { qt = madd<ARITH>(h, vel.x, p0[0]); qph = madd<ARITH>(h, vel.w, p0[3]); }
       vs = o.nv;
     }
     d4 k;
     if (eval) {
-      k = eval_rhs<METRIC, ARITH>(S, qr, qth, vs);
+      if constexpr (METRIC == 2) {
+        const double q[4] = {qt, qr, qth, qph};
+        k = user_rhs(S, q, vs);
+      } else {
+        k = eval_rhs<METRIC, ARITH>(S, qr, qth, vs);
+      }
       ++o.nr;
     } else {
       k = K.get(0);
@@ -373,6 +385,7 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
     if (!(p3 >= 0.0 && p3 < TWO_PI)) {
       double ph = fmod(p3, TWO_PI);
       L.pos[3] = ph < 0.0 ? ph + TWO_PI : ph;
+      if constexpr (METRIC == 2) refl = true;      // k7 was taken at the unwrapped phi
     }
     K.put(0, o.k7);                                 // FSAL: k7 is the next k1 ...
     L.k1_ok = !refl;                                // ... unless theta was reflected

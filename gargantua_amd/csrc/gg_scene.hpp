@@ -13,6 +13,7 @@
 #pragma once
 
 #include "gg_physics.hpp"
+#include "gg_user_metric.hpp"
 
 namespace gg {
 
@@ -188,7 +189,13 @@ __device__ inline double g4_static(const DevScene &S, const double pt[3]) {
   double zr = pt[2] / r;
   zr = zr < -1.0 ? -1.0 : (zr > 1.0 ? 1.0 : zr);
   const double th = acos(zr);
-  const double gte = -metric64<METRIC>(S, r, th).tt;
+  double gte;
+  if constexpr (METRIC == 2) {
+    const double x[4] = {0.0, r, th, atan2(pt[1], pt[0])};
+    gte = -user_gmn(S, x).tt;
+  } else {
+    gte = -metric64<METRIC>(S, r, th).tt;
+  }
   const double gto = S.g_tt_obs;
   if (gte <= 0.0 || gto <= 0.0) return 0.0;
   const double g = sqrt(gte / gto);

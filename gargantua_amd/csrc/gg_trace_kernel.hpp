@@ -10,7 +10,9 @@
 // instantiates the disk modes, gg_trace_scene.hip the scene modes.
 #pragma once
 
+#ifndef __HIPCC_RTC__  // hipRTC supplies the runtime and fixed-width types itself
 #include <hip/hip_runtime.h>
+#endif
 
 #include "gg_internal.hpp"
 #include "gg_rk45.hpp"
@@ -296,7 +298,13 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
           if (rho > 0.0) {
             double j = disk_emission(*Sp, rho, mr, mth);
             d4 pm{0.5 * (v0.x + L.vel.x), 0.5 * (v0.y + L.vel.y), 0.5 * (v0.z + L.vel.z), 0.5 * (v0.w + L.vel.w)};
-            double g = redshift64<METRIC>(*Sp, mr, mth, pm);
+            double g;
+            if constexpr (METRIC == 2) {
+              const double mid[4] = {0.5 * (p0[0] + L.pos[0]), mr, mth, 0.5 * (p0[3] + L.pos[3])};
+              g = user_redshift(*Sp, mid, pm);
+            } else {
+              g = redshift64<METRIC>(*Sp, mr, mth, pm);
+            }
             double g2 = g * g;
             double red = j * (g2 * g2);
             L.intensity = L.intensity + red * dl * L.transmission;
@@ -320,7 +328,13 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
             if (hit) {
               d4 vc{v0.x + s * (L.vel.x - v0.x), v0.y + s * (L.vel.y - v0.y), v0.z + s * (L.vel.z - v0.z),
                     v0.w + s * (L.vel.w - v0.w)};
-              double g = redshift64<METRIC>(*Sp, rc, HALF_PI, vc);
+              double g;
+              if constexpr (METRIC == 2) {
+                const double pc[4] = {p0[0] + s * (L.pos[0] - p0[0]), rc, HALF_PI, p0[3]};
+                g = user_redshift(*Sp, pc, vc);
+              } else {
+                g = redshift64<METRIC>(*Sp, rc, HALF_PI, vc);
+              }
               double g2 = g * g;
               double I = pow(rc, -Sp->q) * (g2 * g2);
               if (w > 0.0) {

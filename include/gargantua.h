@@ -44,8 +44,10 @@
 #ifndef GARGANTUA_H
 #define GARGANTUA_H
 
+#ifndef __HIPCC_RTC__   /* hipRTC supplies the fixed-width types itself */
 #include <stddef.h>
 #include <stdint.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -66,7 +68,9 @@ typedef enum gg_status {
 /* ---- metric (crates/gr-core/src/kerr.rs:17-34, schwarzschild.rs:7-18) ---- */
 typedef enum gg_metric_kind {
   GG_METRIC_SCHWARZSCHILD = 0,
-  GG_METRIC_KERR = 1
+  GG_METRIC_KERR = 1,
+  GG_METRIC_USER = 2        /* the context's gg_set_user_metric tensor; mass and
+                               spin are passed to it unchanged               */
 } gg_metric_kind;
 
 typedef struct gg_metric {
@@ -398,6 +402,24 @@ int gg_set_sky_hdri(gg_context *ctx, uint32_t width, uint32_t height,
  * hittables are set, and for a metal with fuzz > 0 (its reference shading
  * draws a random number). */
 int gg_set_hittables(gg_context *ctx, uint32_t n, const gg_hittable *objects);
+
+/* A metric known only by its tensor, for scenes of kind GG_METRIC_USER on
+ * this context: a Metric that implements metric_tensor alone
+ * (crates/gr-core/src/metric.rs:11-80).  `source` is HIP device code that
+ * defines
+ *     __device__ void gg_user_metric(const double x[4], double mass,
+ *                                    double spin, double g[16]);
+ * filling g[4*mu + nu] = g_mu,nu at x = (t, r, theta, phi) (any helper
+ * functions it needs may precede it).  The library compiles it at run time
+ * (hipRTC) into its trace kernel; the Christoffel symbols are the reference's
+ * central differences (numerical_christoffel, metric.rs:97-142, h = 1e-6) of
+ * this tensor and its 4x4 inverse.  event_horizon is Metric::event_horizon:
+ * a radius, or NaN for None (no horizon test).  The emitter of disk light is
+ * static (orbital_four_velocity = None).  Renders of such scenes take
+ * precision GG_PREC_FP64 and one scene per call.  A compile error returns
+ * GG_ERR_INVALID_ARG with the compiler log in gg_last_error().  NULL source
+ * clears the metric. */
+int gg_set_user_metric(gg_context *ctx, const char *source, double event_horizon);
 
 /* Device memory gg_grad's automatic mode (checkpoint_every = 0) may use for
  * step records; 0 (the default): a quarter of the free device memory.  The

@@ -38,6 +38,8 @@ _SIGS = [
     ("oracle_sky", None, [_P, _P]),
     ("oracle_set_sky_hdri", C.c_int, [C.c_uint32, C.c_uint32, _P, C.c_float]),
     ("oracle_set_hittables", C.c_int, [C.c_uint32, _P]),
+    ("oracle_set_user_metric", None, [_P, C.c_int, C.c_double]),
+    ("oracle_inverse4", C.c_int, [_P, _P]),
     ("oracle_scene_hit", C.c_int, [_P, _P, _P]),
     ("oracle_g4_static", C.c_double, [_P, _P, C.c_double]),
     ("oracle_hdri_sample", None, [C.c_uint32, C.c_uint32, _P, C.c_float, _P, _P]),
@@ -205,6 +207,41 @@ class Oracle:
         return self.lib.oracle_r_isco(mass, spin)
 
     # ---- whole paths ----
+    def set_user_metric(self, metric=None):
+        """The tensor of GG_METRIC_USER scenes: metric.source (the device code a Renderer compiles) built as
+        host C++ with g++ (-ffp-contract=off), or None to clear."""
+        if metric is None:
+            self.lib.oracle_set_user_metric(None, 0, 0.0)
+            self._user_lib = None
+            return
+        import hashlib
+        import subprocess
+        import tempfile
+        tag = hashlib.sha1(metric.source.encode()).hexdigest()[:16]
+        d = os.path.join(tempfile.gettempdir(), "gg_oracle_user_metrics")
+        os.makedirs(d, exist_ok=True)
+        so = os.path.join(d, f"um_{tag}.so")
+        if not os.path.exists(so):
+            src = os.path.join(d, f"um_{tag}.cpp")
+            with open(src, "w") as f:
+                f.write("#include <cmath>\nusing namespace std;\n#define __device__\n#define __host__\n"
+                        "namespace gg {\n" + metric.source + "\n}\n"
+                        "extern \"C\" void gg_user_metric_host(const double *x, double m, double a, double *g) "
+                        "{ gg::gg_user_metric(x, m, a, g); }\n")
+            subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", so + ".tmp", src],
+                           check=True)
+            os.replace(so + ".tmp", so)
+        self._user_lib = C.CDLL(so)
+        fn = C.cast(self._user_lib.gg_user_metric_host, C.c_void_p)
+        rh = metric.event_horizon
+        self.lib.oracle_set_user_metric(fn, 0 if rh is None else 1, 0.0 if rh is None else float(rh))
+
+    def inverse4(self, m):
+        a = np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(16))
+        out = np.zeros(16)
+        ok = self.lib.oracle_inverse4(a.ctypes.data, out.ctypes.data)
+        return out.reshape(4, 4) if ok else None
+
     def set_hittables(self, world=None):
         """Scene geometry of render / trace_states (None: none); returns the status code."""
         if world is None or len(world) == 0:

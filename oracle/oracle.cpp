@@ -583,11 +583,22 @@ int oracle_metric_tensor(const gg_metric *m, const double pos[4], double g[16]) 
   return 0;
 }
 
+// The tensor of GG_METRIC_USER scenes (fn null: none) and its horizon
+// (has_rh 0: Metric::event_horizon is None).
+void oracle_set_user_metric(void (*fn)(const double *, double, double, double *), int has_rh, double rh) {
+  user_metric().g = fn;
+  user_metric().has_rh = has_rh != 0;
+  user_metric().rh = rh;
+}
+
+int oracle_inverse4(const double m[16], double inv[16]) { return inverse4(m, inv) ? 1 : 0; }
+
 int oracle_christoffel(const gg_metric *m, const double pos[4], double gamma[64]) {
   MetricP mp = make_metric(*m);
   V4<double> p{{pos[0], pos[1], pos[2], pos[3]}};
   double G[4][4][4];
-  if (mp.kind == GG_METRIC_KERR) kerr_christoffel(mp, mp.spin, p, G);
+  if (mp.kind == GG_METRIC_USER) numerical_christoffel(mp, pos, G);
+  else if (mp.kind == GG_METRIC_KERR) kerr_christoffel(mp, mp.spin, p, G);
   else schw_christoffel(mp, p, G);
   std::memcpy(gamma, G, sizeof(G));
   return 0;

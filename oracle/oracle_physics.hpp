@@ -13,6 +13,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "../include/gargantua.h"
 
@@ -35,6 +36,15 @@ struct MetricP {
   double rs;    // 2M
 };
 
+// A metric known only by its tensor (GG_METRIC_USER): the function that
+// fills g[4 mu + nu] at x, and Metric::event_horizon (has_rh false: None).
+struct UserMetricO {
+  void (*g)(const double x[4], double mass, double spin, double out[16]) = nullptr;
+  bool has_rh = false;
+  double rh = 0.0;
+};
+inline UserMetricO &user_metric() { static UserMetricO u; return u; }
+
 inline MetricP make_metric(const gg_metric &g) {
   MetricP m;
   m.kind = g.kind;
@@ -43,7 +53,7 @@ inline MetricP make_metric(const gg_metric &g) {
   double s = g.spin;
   if (s < -max_spin) s = -max_spin;                        // f64::clamp
   if (s > max_spin) s = max_spin;
-  m.spin = (g.kind == GG_METRIC_KERR) ? s : 0.0;
+  m.spin = (g.kind == GG_METRIC_KERR) ? s : (g.kind == GG_METRIC_USER ? g.spin : 0.0);
   m.rs = 2.0 * g.mass;                                     // schwarzschild.rs:16
   return m;
 }
@@ -58,6 +68,14 @@ template <class T>
 Gmn<T> metric_tensor(const MetricP &mp, const T &a, const V4<T> &pos) {
   using std::cos; using std::sin; using std::sqrt;
   Gmn<T> g;
+  if (mp.kind == GG_METRIC_USER) {
+    if constexpr (std::is_same<T, double>::value) {
+      double x[4] = {pos[0], pos[1], pos[2], pos[3]}, m[16];
+      user_metric().g(x, mp.mass, mp.spin, m);
+      g.tt = m[0]; g.rr = m[5]; g.thth = m[10]; g.phph = m[15]; g.tph = m[3];
+    }
+    return g;
+  }
   if (mp.kind == GG_METRIC_KERR) {
     T r = pos[1];
     if (!(val(r) >= 1e-10)) r = T(1e-10);                  // r.max(1e-10) (NaN -> 1e-10)
@@ -187,10 +205,95 @@ void schw_christoffel(const MetricP &mp, const V4<T> &pos, T gamma[4][4][4]) {
 // Metric::geodesic_acceleration.  Kerr uses the trait default: dense 64-term
 // contraction in mu, alpha, beta order (metric.rs:37-52).  Schwarzschild uses
 // its inlined override (schwarzschild.rs:99-140).
+// Matrix4::try_inverse (metric.rs:14-19): cofactor expansion over the
+// column-major storage (the MESA gluInvertMatrix scheme that nalgebra's 4x4
+// inverse follows), scaled by 1/det; false for a zero determinant.
+inline bool inverse4(const double m[16], double inv[16]) {
+  inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] +
+           m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+  inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] -
+           m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+  inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] +
+           m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+  inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] -
+            m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+  double det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+  if (det == 0.0) return false;
+  inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] -
+           m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+  inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] +
+           m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+  inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] -
+           m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+  inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] +
+            m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+  inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] +
+           m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+  inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] -
+           m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+  inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] +
+            m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+  inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] -
+            m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+  inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] -
+           m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+  inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] +
+           m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+  inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] -
+            m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+  inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] +
+            m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+  double inv_det = 1.0 / det;
+  for (int i = 0; i < 16; ++i) inv[i] = inv[i] * inv_det;
+  return true;
+}
+
+// numerical_christoffel (metric.rs:97-142): Gamma[mu][alpha][beta] from
+// central differences (h = 1e-6) of the user tensor and its inverse.
+inline void numerical_christoffel(const MetricP &mp, const double pos[4], double gamma[4][4][4]) {
+  const double h = 1e-6;
+  double dg[4][16];
+  auto G = user_metric().g;
+  for (int s = 0; s < 4; ++s) {
+    double pp[4] = {pos[0], pos[1], pos[2], pos[3]}, pm[4] = {pos[0], pos[1], pos[2], pos[3]};
+    pp[s] += h;
+    pm[s] -= h;
+    double gp[16], gm[16];
+    G(pp, mp.mass, mp.spin, gp);
+    G(pm, mp.mass, mp.spin, gm);
+    for (int i = 0; i < 16; ++i) dg[s][i] = (gp[i] - gm[i]) / (2.0 * h);
+  }
+  double g[16], gi[16];
+  G(pos, mp.mass, mp.spin, g);
+  if (!inverse4(g, gi))
+    for (int i = 0; i < 16; ++i) gi[i] = std::nan("");   // try_inverse fails: the reference panics
+  for (int mu = 0; mu < 4; ++mu)
+    for (int al = 0; al < 4; ++al)
+      for (int be = 0; be < 4; ++be) {
+        double sum = 0.0;
+        for (int s = 0; s < 4; ++s)
+          sum += gi[4 * s + mu] * (dg[al][4 * s + be] + dg[be][4 * s + al] - dg[s][4 * al + be]);
+        gamma[mu][al][be] = 0.5 * sum;
+      }
+}
+
 template <class T>
 V4<T> geodesic_acceleration(const MetricP &mp, const T &a, const V4<T> &pos, const V4<T> &vel) {
   using std::cos; using std::sin;
   V4<T> acc;
+  if (mp.kind == GG_METRIC_USER) {
+    // the trait default (metric.rs:37-52) over numerical Christoffels
+    if constexpr (std::is_same<T, double>::value) {
+      double x[4] = {pos[0], pos[1], pos[2], pos[3]}, gamma[4][4][4];
+      numerical_christoffel(mp, x, gamma);
+      for (int mu = 0; mu < 4; ++mu) {
+        acc[mu] = 0.0;
+        for (int al = 0; al < 4; ++al)
+          for (int be = 0; be < 4; ++be) acc[mu] = acc[mu] - gamma[mu][al][be] * vel[al] * vel[be];
+      }
+    }
+    return acc;
+  }
   if (mp.kind == GG_METRIC_KERR) {
     T gamma[4][4][4];
     kerr_christoffel(mp, a, pos, gamma);
@@ -227,6 +330,10 @@ V4<T> geodesic_acceleration(const MetricP &mp, const T &a, const V4<T> &pos, con
 
 // Metric::event_horizon (kerr.rs:427-436, schwarzschild.rs:142-144).
 inline bool event_horizon(const MetricP &mp, double *rh) {
+  if (mp.kind == GG_METRIC_USER) {
+    *rh = user_metric().rh;
+    return user_metric().has_rh;
+  }
   if (mp.kind == GG_METRIC_KERR) {
     double m = mp.mass, a = mp.spin;
     double disc = m * m - a * a;
@@ -256,6 +363,7 @@ bool orbital_four_velocity(const MetricP &mp, const T &a, const V4<T> &pos, V4<T
   using std::pow; using std::sqrt;
   T r = pos[1];
   double m = mp.mass;
+  if (mp.kind == GG_METRIC_USER) return false;               // the trait default: None
   if (mp.kind == GG_METRIC_KERR) {
     if (val(r) <= 0.0) return false;
     double sqrt_m = std::sqrt(m);
