@@ -158,8 +158,12 @@ def _close(got, want, min_agree=0.995, intensity=True):
     assert np.mean(rel <= 1e-6) >= 0.99, np.sort(rel)[-5:]
     disk = agree & (want["cls"] == gg.GG_CLASS_DISK)
     if intensity and disk.any():
+        # Central differences divide the tensor's rounding by 2h = 2e-6: a
+        # last-bit difference between the device's and the host's sin/cos
+        # becomes ~1e-10 of a Christoffel symbol, which the volumetric disk
+        # integrates over hundreds of steps into ~1e-6 of intensity.
         ri = np.abs(got["intensity"][disk] - want["intensity"][disk]) / np.abs(want["intensity"][disk])
-        assert np.mean(ri <= 1e-6) >= 0.99
+        assert np.mean(ri <= 1e-5) >= 0.99
 
 
 @pytest.mark.gpu
