@@ -280,6 +280,42 @@ def bench_scene(args):
             **out}
 
 
+def bench_user(args):
+    """Metrics known only by their tensor, compiled at run time: Reissner-Nordstrom Q=0.6 and Kerr a=0.9 written as
+    tensors, 512x512 thin disk, camera r=30M i=75deg FOV 40deg, FP64; the Kerr tensor beside the library's
+    closed-form Kerr kernel on the same scene."""
+    import torch
+    import gargantua_amd as gg
+    from tests.test_user_metric import KERR, rn_metric
+    size = 512 if args.size == 1024 else args.size
+    cam = gg.configs.inclination_camera(30.0, math.radians(75.0), math.radians(40.0))
+    disk = gg.AccretionDisk(kind=gg.GG_DISK_THIN, r_inner=gg.configs.r_isco(1.0, 0.9), r_outer=20.0)
+    rend = gg.Renderer(0, stream=torch.cuda.current_stream().cuda_stream)
+    out = {}
+    metrics = {"reissner_nordstrom": rn_metric(0.6),
+               "kerr_tensor": gg.UserMetric(KERR, mass=1.0, spin=0.9, event_horizon=1.0 + math.sqrt(1.0 - 0.81)),
+               "kerr_closed_form": gg.Kerr(1.0, 0.9)}
+    for name, m in metrics.items():
+        scene = gg.Scene(metric=m, camera=cam, disk=disk, integrator=gg.RK45Integrator(max_radius=200.0),
+                         width=size, height=size)
+        t0 = time.perf_counter()
+        rend._ensure_metric(scene)
+        compile_s = time.perf_counter() - t0
+        outs = _dev_outputs(torch, scene.n_rays)
+        wall, kms, st = time_precision(torch, rend, scene, "fp64", max(1, args.steps), 1, outs)
+        kms = sorted(kms)
+        out[name] = {"kernel_ms": kms[len(kms) // 2], "geodesics_per_s": scene.n_rays / (wall / max(1, args.steps)),
+                     "attempts_per_ray": st["attempts"] / scene.n_rays, "rhs_evals": st["rhs_evals"],
+                     "first_set_s": compile_s}
+        _log(f"user {name}: {out[name]['kernel_ms']:.1f} ms")
+    rend.close()
+    out["tensor_vs_closed_form"] = out["kerr_tensor"]["kernel_ms"] / out["kerr_closed_form"]["kernel_ms"]
+    return {"metric": f"geodesics/s user metric {size}x{size}", "value": out["reissner_nordstrom"]["geodesics_per_s"],
+            "unit": "geodesics/s", "n_gpus": 1, "dtype": "f64",
+            "config": {"workload": f"{size}x{size} thin disk, camera r=30M i=75deg FOV 40deg; metrics given as tensors "
+                                   "(numerical Christoffels, h=1e-6)"}, **out}
+
+
 # Algorithmic FLOPs of one accepted step back-propagated by the adjoint
 # reverse kernel: six reverse-mode RHS vector-Jacobian products (628 each,
 # counted from the source of kerr_rhs_vjp: 130 for the forward symbol
@@ -401,7 +437,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, default=1024)
-    ap.add_argument("--workload", default="kerr1024", choices=["kerr1024", "batch64", "kerr4096", "grad512", "scene", "copy"],
+    ap.add_argument("--workload", default="kerr1024", choices=["kerr1024", "batch64", "kerr4096", "grad512", "scene", "user", "copy"],
                     help="copy: one 512 MiB device-to-device copy, the reference kernel that calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters for HBM traffic")
     ap.add_argument("--gd-steps", type=int, default=200, help="grad512: gradient-descent iterations (0: none)")
@@ -421,6 +457,9 @@ def main():
         return
     if args.workload == "scene":
         print(json.dumps(bench_scene(args)))
+        return
+    if args.workload == "user":
+        print(json.dumps(bench_user(args)))
         return
     if args.workload == "copy":
         import torch
