@@ -271,6 +271,30 @@ def bench_scene(args):
             res["cpu_baseline"] = {"value": len(idx) / dt, "unit": "geodesics/s", "cores": ora.threads,
                                    "kind": "port", "sample": f"every 8th row and column ({len(idx)} rays)"}
         out[name] = res
+    # BVH scaling: 4096 small spheres scattered in a shell 25-90M around the hole (seeded)
+    rng = np.random.default_rng(2026)
+    many = gg.World()
+    mats = [gg.Lambertian((0.8, 0.3, 0.2)), gg.Lambertian((0.2, 0.5, 0.9)), gg.DiffuseLight((1.5, 1.3, 0.9))]
+    for i in range(4096):
+        d = rng.normal(size=3)
+        c = d / np.linalg.norm(d) * rng.uniform(25.0, 90.0)
+        if c[0] > 20.0 and abs(c[1]) < 15.0 and abs(c[2]) < 15.0:
+            c[0] = -c[0]                                   # keep the camera's neighbourhood clear
+        many.add(gg.Sphere(tuple(c), float(rng.uniform(0.3, 1.0)), mats[i % 3]))
+    rend.set_hittables(many)
+    scene = gg.Scene(metric=gg.Schwarzschild(1.0), camera=cam, integrator=gg.RK45Integrator(max_radius=200.0,
+                                                                                       max_steps=5000),
+                     width=w, height=h)
+    n = scene.n_rays
+    res = {"primitives": len(many)}
+    for prec in ("mixed", "fp64"):
+        outs = _dev_outputs(torch, n)
+        outs["color"] = torch.empty(3 * n, dtype=torch.float64, device="cuda")
+        _, kms, st = time_precision(torch, rend, scene, prec, max(1, args.steps), 1, outs)
+        res[prec] = {"kernel_ms": sorted(kms)[len(kms) // 2], "attempts_per_ray": st["attempts"] / n}
+        res["classes"] = np.bincount(outs["cls"].cpu().numpy(), minlength=5).tolist()
+        _log(f"scene many_spheres {prec}: {res[prec]['kernel_ms']:.2f} ms")
+    out["many_spheres"] = res
     rend.set_hittables(None)
     rend.close()
     return {"metric": f"geodesics/s lensed-spheres {w}x{h}", "value": out["scene"]["mixed"]["geodesics_per_s"],

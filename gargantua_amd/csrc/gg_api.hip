@@ -217,10 +217,11 @@ int make_scene_obj(const gg_hittable &h, uint32_t i, SceneObj &o, HitBox &b) {
 }
 
 // Median split on the longest axis of the centroid bounds, leaves of at
-// most four primitives; order[] becomes the leaf order.  Node 0 is the root.
+// most four primitives, nodes in depth-first order (node 0 the root, an
+// internal node's first child right after it, skip = the end of its
+// subtree); order[] becomes the leaf order.
 int32_t build_bvh(const std::vector<HitBox> &box, std::vector<uint32_t> &order, uint32_t b, uint32_t e,
-                  std::vector<BvhNode> &nodes, int depth, int &max_depth) {
-  max_depth = std::max(max_depth, depth);
+                  std::vector<BvhNode> &nodes) {
   const int32_t id = (int32_t)nodes.size();
   nodes.push_back(BvhNode{});
   BvhNode nd{};
@@ -237,8 +238,8 @@ int32_t build_bvh(const std::vector<HitBox> &box, std::vector<uint32_t> &order, 
     }
   }
   if (e - b <= 4) {
-    nd.left = (int32_t)b;
-    nd.right = -(int32_t)(e - b);
+    nd.first = (int32_t)b;
+    nd.count = (int32_t)(e - b);
   } else {
     int axis = 0;
     for (int k = 1; k < 3; ++k)
@@ -247,9 +248,12 @@ int32_t build_bvh(const std::vector<HitBox> &box, std::vector<uint32_t> &order, 
     std::nth_element(order.begin() + b, order.begin() + mid, order.begin() + e, [&](uint32_t p, uint32_t q) {
       return box[p].lo[axis] + box[p].hi[axis] < box[q].lo[axis] + box[q].hi[axis];
     });
-    nd.left = build_bvh(box, order, b, mid, nodes, depth + 1, max_depth);
-    nd.right = build_bvh(box, order, mid, e, nodes, depth + 1, max_depth);
+    nd.first = 0;
+    nd.count = 0;
+    build_bvh(box, order, b, mid, nodes);
+    build_bvh(box, order, mid, e, nodes);
   }
+  nd.skip = (int32_t)nodes.size();
   nodes[id] = nd;
   return id;
 }
@@ -644,9 +648,7 @@ int gg_set_hittables(gg_context *ctx, uint32_t n, const gg_hittable *objects) {
   std::vector<uint32_t> order(n);
   for (uint32_t i = 0; i < n; ++i) order[i] = i;
   nodes.reserve(2 * (size_t)n);
-  int depth = 0;
-  build_bvh(boxes, order, 0, n, nodes, 1, depth);
-  if (depth + 2 > BVH_STACK) return fail(GG_ERR_INTERNAL, "hittables: BVH depth %d exceeds the traversal stack", depth);
+  build_bvh(boxes, order, 0, n, nodes);
   std::vector<SceneObj> leaf_order(n);
   for (uint32_t i = 0; i < n; ++i) leaf_order[i] = prim[order[i]];
   HIP_TRY(hipSetDevice(ctx->device));

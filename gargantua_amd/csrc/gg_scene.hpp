@@ -7,9 +7,10 @@
 // and Quad (shape/quad.rs:75-127); materials enter only through the albedo
 // of one `scatter` call and their emission (shade_hit, tracer.rs:369-384).
 // The host builds a bounding-volume hierarchy over the primitives (flat
-// array, leaves of up to four primitives); the closest hit it returns is
-// the minimum over primitives of each primitive's own first root in the
-// open interval, so it equals the reference's BVH answer whatever the tree.
+// array in depth-first order, leaves of up to four primitives), walked
+// without a stack; the closest hit it returns is the minimum over
+// primitives of each primitive's own first root in the open interval, so it
+// equals the reference's BVH answer whatever the tree.
 #pragma once
 
 #include "gg_physics.hpp"
@@ -31,15 +32,15 @@ struct SceneObj {
   uint32_t metal;     // mirror: the albedo only when the reflection leaves the surface (metal.rs:40-57)
 };
 
-// BVH node (64 bytes).  Internal: left/right child indices.  Leaf:
-// left = first primitive of its range, right = -(primitive count).
+// BVH node (64 bytes) in depth-first order: an internal node's first child
+// is the next node; `skip` is the first node after the node's subtree (the
+// root's skip is the node count).  Leaf: primitives [first, first + count).
 struct BvhNode {
   double lo[3], hi[3];
-  int32_t left, right;
-  uint32_t pad[2];
+  int32_t first, count;   // count 0: internal node
+  int32_t skip;
+  uint32_t pad;
 };
-
-constexpr int BVH_STACK = 32;       // the host rejects trees deeper than this (median splits of 2^24 primitives: 23)
 
 namespace {
 
@@ -121,36 +122,39 @@ struct SceneHit {
   uint32_t obj;
 };
 
-// Closest primitive hit by the chord origin + t dir, t in (0, 1).
+// Closest primitive hit by the chord origin + t dir, t in (0, 1): a
+// stackless depth-first walk (every step moves to a later node, so it ends
+// after at most the node count).
 __device__ inline bool scene_hit(const SceneObj *objs, const BvhNode *nodes, const double o[3], const double dir[3],
                                  SceneHit &h) {
 #pragma clang fp contract(off)
   const double inv[3] = {1.0 / dir[0], 1.0 / dir[1], 1.0 / dir[2]};
   double t_max = 1.0;
   bool any = false;
-  int32_t stack[BVH_STACK];
-  int top = 0;
-  if (!box_hit(nodes[0], o, inv, t_max)) return false;
-  stack[top++] = 0;
-  while (top > 0) {
-    const BvhNode &nd = nodes[stack[--top]];
-    if (nd.right < 0) {
-      const int32_t end = nd.left - nd.right;
-      for (int32_t i = nd.left; i < end; ++i) {
-        double t, p[3], nr[3];
-        if (hit_object(objs[i], o, dir, t_max, t, p, nr)) {
-          t_max = t;
-          any = true;
-          h.t = t;
-          h.obj = (uint32_t)i;
-          for (int k = 0; k < 3; ++k) { h.p[k] = p[k]; h.nrm[k] = nr[k]; }
-        }
-      }
-    } else {
-      const bool hl = box_hit(nodes[nd.left], o, inv, t_max), hr = box_hit(nodes[nd.right], o, inv, t_max);
-      if (hr && top < BVH_STACK) stack[top++] = nd.right;
-      if (hl && top < BVH_STACK) stack[top++] = nd.left;
+  const int32_t end = nodes[0].skip;
+  int32_t i = 0;
+  while (i < end) {
+    const BvhNode &nd = nodes[i];
+    if (!box_hit(nd, o, inv, t_max)) {
+      i = nd.skip;
+      continue;
     }
+    if (nd.count == 0) {
+      ++i;                                   // descend: the first child follows
+      continue;
+    }
+    const int32_t last = nd.first + nd.count;
+    for (int32_t k = nd.first; k < last; ++k) {
+      double t, p[3], nr[3];
+      if (hit_object(objs[k], o, dir, t_max, t, p, nr)) {
+        t_max = t;
+        any = true;
+        h.t = t;
+        h.obj = (uint32_t)k;
+        for (int c = 0; c < 3; ++c) { h.p[c] = p[c]; h.nrm[c] = nr[c]; }
+      }
+    }
+    i = nd.skip;
   }
   if (any) {
     // front_face = dir . outward < 0; the normal faces against the chord
