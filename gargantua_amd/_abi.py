@@ -120,6 +120,7 @@ EXPORTS = [
     ("gg_set_sky_hdri", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_float]),
     ("gg_set_hittables", C.c_int, [_P, C.c_uint32, _P]),
     ("gg_set_user_metric", C.c_int, [_P, C.c_char_p, C.c_double]),
+    ("gg_check_user_metric", C.c_int, [C.c_char_p, C.c_int32, C.c_char_p, C.c_size_t]),
     ("gg_get_launch", C.c_int, [_P, C.POINTER(gg_launch)]),
     ("gg_scene_rays", C.c_int, [C.POINTER(gg_scene), C.POINTER(C.c_uint64)]),
     ("gg_render", C.c_int, [_P, C.POINTER(gg_scene), C.c_int, C.POINTER(gg_outputs), C.POINTER(gg_stats)]),

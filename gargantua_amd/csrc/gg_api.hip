@@ -138,6 +138,8 @@ struct gg_context {
 namespace {
 
 int user_kernels(gg_context *ctx, int mode, int src, hipFunction_t *trace, hipFunction_t *at);
+int compile_user_module(const std::string &user, int mode, int src, std::vector<char> &code, std::string &name_t,
+                        std::string &name_a, std::string &log);
 
 // ---- Hittable scene: primitive constants and the BVH ----
 
@@ -225,21 +227,26 @@ int32_t build_bvh(const std::vector<HitBox> &box, std::vector<uint32_t> &order, 
   const int32_t id = (int32_t)nodes.size();
   nodes.push_back(BvhNode{});
   BvhNode nd{};
-  double clo[3], chi[3];
-  for (int k = 0; k < 3; ++k) { nd.lo[k] = clo[k] = INFINITY; nd.hi[k] = chi[k] = -INFINITY; }
+  double lo[3], hi[3], clo[3], chi[3];
+  for (int k = 0; k < 3; ++k) { lo[k] = clo[k] = INFINITY; hi[k] = chi[k] = -INFINITY; }
   for (uint32_t i = b; i < e; ++i) {
     const HitBox &x = box[order[i]];
     for (int k = 0; k < 3; ++k) {
-      nd.lo[k] = std::min(nd.lo[k], x.lo[k]);
-      nd.hi[k] = std::max(nd.hi[k], x.hi[k]);
+      lo[k] = std::min(lo[k], x.lo[k]);
+      hi[k] = std::max(hi[k], x.hi[k]);
       const double c = 0.5 * (x.lo[k] + x.hi[k]);
       clo[k] = std::min(clo[k], c);
       chi[k] = std::max(chi[k], c);
     }
   }
+  // the FP32 box, rounded outward and padded far beyond FP32 rounding
+  for (int k = 0; k < 3; ++k) {
+    const double pad = 1e-5 * (1.0 + std::max(std::fabs(lo[k]), std::fabs(hi[k])));
+    nd.lo[k] = std::nextafter((float)(lo[k] - pad), -INFINITY);
+    nd.hi[k] = std::nextafter((float)(hi[k] + pad), INFINITY);
+  }
   if (e - b <= 4) {
-    nd.first = (int32_t)b;
-    nd.count = (int32_t)(e - b);
+    nd.leaf = (int32_t)(8 * b + (e - b));
   } else {
     int axis = 0;
     for (int k = 1; k < 3; ++k)
@@ -248,8 +255,7 @@ int32_t build_bvh(const std::vector<HitBox> &box, std::vector<uint32_t> &order, 
     std::nth_element(order.begin() + b, order.begin() + mid, order.begin() + e, [&](uint32_t p, uint32_t q) {
       return box[p].lo[axis] + box[p].hi[axis] < box[q].lo[axis] + box[q].hi[axis];
     });
-    nd.first = 0;
-    nd.count = 0;
+    nd.leaf = 0;
     build_bvh(box, order, b, mid, nodes);
     build_bvh(box, order, mid, e, nodes);
   }
@@ -664,6 +670,18 @@ int gg_set_hittables(gg_context *ctx, uint32_t n, const gg_hittable *objects) {
   return GG_OK;
 }
 
+int gg_check_user_metric(const char *source, int32_t disk_mode, char *log, size_t log_size) {
+  if (!source || !*source) return fail(GG_ERR_INVALID_ARG, "user metric source is empty");
+  if (disk_mode < 0 || disk_mode > 4) return fail(GG_ERR_INVALID_ARG, "disk_mode must be 0..4");
+  std::vector<char> code;
+  std::string name_t, name_a, text;
+  const int rc = compile_user_module(source, disk_mode, 0, code, name_t, name_a, text);
+  if (log && log_size) {
+    std::snprintf(log, log_size, "%s", text.c_str());
+  }
+  return rc;
+}
+
 int gg_set_user_metric(gg_context *ctx, const char *source, double event_horizon) {
   if (!ctx) return fail(GG_ERR_INVALID_ARG, "NULL context");
   HIP_TRY(hipSetDevice(ctx->device));
@@ -716,15 +734,14 @@ namespace {
 // The trace kernel for METRIC 2 (gg_user_metric.hpp) with the context's
 // tensor source, disk mode `mode` and ray source `src`, compiled with hipRTC
 // on first use, and the helper kernel that evaluates the tensor at a point.
-int user_kernels(gg_context *ctx, int mode, int src, hipFunction_t *trace, hipFunction_t *at) {
-  const int key = 2 * mode + src;
-  if (ctx->user_trace[key]) {
-    *trace = ctx->user_trace[key];
-    *at = ctx->user_at[key];
-    return GG_OK;
-  }
+// Compile the trace kernel for METRIC 2 with tensor source `user`, disk mode
+// `mode` and ray source `src`, and the helper that evaluates the tensor at a
+// point, with hipRTC (no device needed): the code object and both kernels'
+// lowered names, or the compiler log.
+int compile_user_module(const std::string &user, int mode, int src, std::vector<char> &code, std::string &name_t,
+                        std::string &name_a, std::string &log) {
   const std::string prog = "using namespace __hip_internal;\n#include \"gg_trace_kernel.hpp\"\nnamespace gg {\n" +
-                           ctx->user_src +
+                           user +
                            "\n__global__ void gg_user_metric_at(const double *x, double mass, double spin, double *g) {\n"
                            "  gg_user_metric(x, mass, spin, g);\n}\n}  // namespace gg\n";
   hiprtcProgram p;
@@ -740,24 +757,41 @@ int user_kernels(gg_context *ctx, int mode, int src, hipFunction_t *trace, hipFu
   const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
                         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
   const hiprtcResult r = hiprtcCompileProgram(p, (int)(sizeof opts / sizeof opts[0]), opts);
+  size_t n = 0;
+  hiprtcGetProgramLogSize(p, &n);
+  log.assign(n + 1, '\0');
+  hiprtcGetProgramLog(p, &log[0]);
   if (r != HIPRTC_SUCCESS) {
-    size_t n = 0;
-    hiprtcGetProgramLogSize(p, &n);
-    std::string log(n + 1, '\0');
-    hiprtcGetProgramLog(p, &log[0]);
     hiprtcDestroyProgram(&p);
     return fail(GG_ERR_INVALID_ARG, "user metric does not compile: %.1500s", log.c_str());
   }
   const char *lt = nullptr, *la = nullptr;
   hiprtcGetLoweredName(p, expr, &lt);
   hiprtcGetLoweredName(p, at_expr, &la);
-  const std::string name_t = lt ? lt : "", name_a = la ? la : "";
+  name_t = lt ? lt : "";
+  name_a = la ? la : "";
   size_t code_n = 0;
   hiprtcGetCodeSize(p, &code_n);
-  std::vector<char> code(code_n);
-  hiprtcGetCode(p, code.data());
+  code.assign(code_n, 0);
+  if (code_n) hiprtcGetCode(p, code.data());
   hiprtcDestroyProgram(&p);
   if (name_t.empty() || name_a.empty() || code_n == 0) return fail(GG_ERR_INTERNAL, "hipRTC returned no kernel");
+  return GG_OK;
+}
+
+// The context's METRIC 2 kernels for (mode, src), compiled and loaded on
+// first use.
+int user_kernels(gg_context *ctx, int mode, int src, hipFunction_t *trace, hipFunction_t *at) {
+  const int key = 2 * mode + src;
+  if (ctx->user_trace[key]) {
+    *trace = ctx->user_trace[key];
+    *at = ctx->user_at[key];
+    return GG_OK;
+  }
+  std::vector<char> code;
+  std::string name_t, name_a, log;
+  int rc = compile_user_module(ctx->user_src, mode, src, code, name_t, name_a, log);
+  if (rc) return rc;
   hipModule_t mod = nullptr;
   HIP_TRY(hipModuleLoadData(&mod, code.data()));
   hipFunction_t ft = nullptr, fa = nullptr;

@@ -32,14 +32,17 @@ struct SceneObj {
   uint32_t metal;     // mirror: the albedo only when the reflection leaves the surface (metal.rs:40-57)
 };
 
-// BVH node (64 bytes) in depth-first order: an internal node's first child
+// BVH node (32 bytes) in depth-first order: an internal node's first child
 // is the next node; `skip` is the first node after the node's subtree (the
-// root's skip is the node count).  Leaf: primitives [first, first + count).
+// root's skip is the node count).  Leaf: primitives [first, first + count)
+// with leaf = 8 first + count, count 1..4; an internal node has leaf = 0.
+// The box is FP32, padded outward by 1e-5 (1 + |coordinate|): far more than
+// the FP32 rounding of the slab test, so the test is conservative (it never
+// culls a primitive the FP64 hit test would find) at half the FP64 cost.
 struct BvhNode {
-  double lo[3], hi[3];
-  int32_t first, count;   // count 0: internal node
+  float lo[3], hi[3];
+  int32_t leaf;
   int32_t skip;
-  uint32_t pad;
 };
 
 namespace {
@@ -63,16 +66,17 @@ __device__ inline void sph2cart(const double p[4], double c[3]) {
   c[2] = r * ct;
 }
 
-// Slab test of [lo, hi] against origin + t dir, t in (0, t_max).  The host
-// pads every box, so an origin on a slab plane (0 * inf) cannot cull a
-// primitive the exact test would hit.
-__device__ inline bool box_hit(const BvhNode &b, const double o[3], const double inv[3], double t_max) {
-  double t0 = 0.0, t1 = t_max;
+// Slab test of [lo, hi] against origin + t dir, t in [0, t_max], in FP32.
+// A chord parallel to a slab whose origin lies on its plane gives 0 * inf:
+// that axis then does not constrain t.
+__device__ inline bool box_hit(const BvhNode &b, const float o[3], const float inv[3], float t_max) {
+  float t0 = 0.0f, t1 = t_max;
   for (int k = 0; k < 3; ++k) {
-    double ta = (b.lo[k] - o[k]) * inv[k], tb = (b.hi[k] - o[k]) * inv[k];
-    double tl = fmin(ta, tb), th = fmax(ta, tb);
-    t0 = fmax(t0, tl);
-    t1 = fmin(t1, th);
+    const float ta = (b.lo[k] - o[k]) * inv[k], tb = (b.hi[k] - o[k]) * inv[k];
+    float tl = fminf(ta, tb), th = fmaxf(ta, tb);
+    if (ta != ta || tb != tb) { tl = -__builtin_inff(); th = __builtin_inff(); }
+    t0 = fmaxf(t0, tl);
+    t1 = fminf(t1, th);
   }
   return t0 <= t1;
 }
@@ -128,26 +132,29 @@ struct SceneHit {
 __device__ inline bool scene_hit(const SceneObj *objs, const BvhNode *nodes, const double o[3], const double dir[3],
                                  SceneHit &h) {
 #pragma clang fp contract(off)
-  const double inv[3] = {1.0 / dir[0], 1.0 / dir[1], 1.0 / dir[2]};
+  const float of[3] = {(float)o[0], (float)o[1], (float)o[2]};
+  const float inv[3] = {1.0f / (float)dir[0], 1.0f / (float)dir[1], 1.0f / (float)dir[2]};
   double t_max = 1.0;
+  float t_max_f = 1.0001f;                 // above every t_max the boxes can be asked about
   bool any = false;
   const int32_t end = nodes[0].skip;
   int32_t i = 0;
   while (i < end) {
     const BvhNode &nd = nodes[i];
-    if (!box_hit(nd, o, inv, t_max)) {
+    if (!box_hit(nd, of, inv, t_max_f)) {
       i = nd.skip;
       continue;
     }
-    if (nd.count == 0) {
+    if (nd.leaf == 0) {
       ++i;                                   // descend: the first child follows
       continue;
     }
-    const int32_t last = nd.first + nd.count;
-    for (int32_t k = nd.first; k < last; ++k) {
+    const int32_t first = nd.leaf >> 3, last = first + (nd.leaf & 7);
+    for (int32_t k = first; k < last; ++k) {
       double t, p[3], nr[3];
       if (hit_object(objs[k], o, dir, t_max, t, p, nr)) {
         t_max = t;
+        t_max_f = (float)t * 1.0001f + 1e-6f;  // rounded up: the FP32 box test stays conservative
         any = true;
         h.t = t;
         h.obj = (uint32_t)k;

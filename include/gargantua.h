@@ -421,6 +421,13 @@ int gg_set_hittables(gg_context *ctx, uint32_t n, const gg_hittable *objects);
  * clears the metric. */
 int gg_set_user_metric(gg_context *ctx, const char *source, double event_horizon);
 
+/* Compile a gg_set_user_metric source for disk mode 0-4 (none, volumetric,
+ * thin, hittables, volumetric + hittables) without loading it: no device is
+ * needed.  The compiler log (warnings too) is copied into log (log_size
+ * bytes, NUL-terminated) when log is non-NULL.  No counterpart in the
+ * reference; for build-time checks of a tensor source. */
+int gg_check_user_metric(const char *source, int32_t disk_mode, char *log, size_t log_size);
+
 /* Device memory gg_grad's automatic mode (checkpoint_every = 0) may use for
  * step records; 0 (the default): a quarter of the free device memory.  The
  * choice of mode follows it; the gradient does not depend on it. */
