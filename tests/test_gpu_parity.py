@@ -1,6 +1,6 @@
 """GPU parity: libgargantua (HIP, gfx950) against the FP64 oracle.
 
-Tolerances: final (r, theta, phi) within 1e-6 (relative
+Tolerances (the parity bar): final (r, theta, phi) within 1e-6 (relative
 r, absolute angle) on at least 99% of the rays whose class agrees, class
 agreement >= 99.9% overall on the benchmark scenes and exact outside a
 +-2 px band around class boundaries where the image is large enough to
