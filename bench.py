@@ -310,14 +310,18 @@ def bench_user(args):
     closed-form Kerr kernel on the same scene."""
     import torch
     import gargantua_amd as gg
-    from tests.test_user_metric import KERR, rn_metric
+    from tests.test_user_metric import KERR, SYMMETRIC, rn_metric
     size = 512 if args.size == 1024 else args.size
     cam = gg.configs.inclination_camera(30.0, math.radians(75.0), math.radians(40.0))
     disk = gg.AccretionDisk(kind=gg.GG_DISK_THIN, r_inner=gg.configs.r_isco(1.0, 0.9), r_outer=20.0)
     rend = gg.Renderer(0, stream=torch.cuda.current_stream().cuda_stream)
     out = {}
-    metrics = {"reissner_nordstrom": rn_metric(0.6),
-               "kerr_tensor": gg.UserMetric(KERR, mass=1.0, spin=0.9, event_horizon=1.0 + math.sqrt(1.0 - 0.81)),
+    rh_kerr = 1.0 + math.sqrt(1.0 - 0.81)
+    rn = rn_metric(0.6)
+    metrics = {"reissner_nordstrom": rn,
+               "reissner_nordstrom_symmetric": gg.UserMetric(SYMMETRIC + rn.source, rn.mass, rn.spin, rn.event_horizon),
+               "kerr_tensor": gg.UserMetric(KERR, mass=1.0, spin=0.9, event_horizon=rh_kerr),
+               "kerr_tensor_symmetric": gg.UserMetric(SYMMETRIC + KERR, mass=1.0, spin=0.9, event_horizon=rh_kerr),
                "kerr_closed_form": gg.Kerr(1.0, 0.9)}
     for name, m in metrics.items():
         scene = gg.Scene(metric=m, camera=cam, disk=disk, integrator=gg.RK45Integrator(max_radius=200.0),
@@ -334,6 +338,8 @@ def bench_user(args):
         _log(f"user {name}: {out[name]['kernel_ms']:.1f} ms")
     rend.close()
     out["tensor_vs_closed_form"] = out["kerr_tensor"]["kernel_ms"] / out["kerr_closed_form"]["kernel_ms"]
+    out["symmetric_tensor_vs_closed_form"] = (out["kerr_tensor_symmetric"]["kernel_ms"] /
+                                              out["kerr_closed_form"]["kernel_ms"])
     return {"metric": f"geodesics/s user metric {size}x{size}", "value": out["reissner_nordstrom"]["geodesics_per_s"],
             "unit": "geodesics/s", "n_gpus": 1, "dtype": "f64",
             "config": {"workload": f"{size}x{size} thin disk, camera r=30M i=75deg FOV 40deg; metrics given as tensors "

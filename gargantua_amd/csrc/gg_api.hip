@@ -740,9 +740,11 @@ namespace {
 // lowered names, or the compiler log.
 int compile_user_module(const std::string &user, int mode, int src, std::vector<char> &code, std::string &name_t,
                         std::string &name_a, std::string &log) {
-  const std::string prog = "using namespace __hip_internal;\n#include \"gg_trace_kernel.hpp\"\nnamespace gg {\n" +
-                           user +
-                           "\n__global__ void gg_user_metric_at(const double *x, double mass, double spin, double *g) {\n"
+  // The source comes first, so that the symmetry macros it may define are
+  // seen by the headers' numerical Christoffels.
+  const std::string prog = "using namespace __hip_internal;\nnamespace gg {\n" + user +
+                           "\n}  // namespace gg\n#include \"gg_trace_kernel.hpp\"\nnamespace gg {\n"
+                           "__global__ void gg_user_metric_at(const double *x, double mass, double spin, double *g) {\n"
                            "  gg_user_metric(x, mass, spin, g);\n}\n}  // namespace gg\n";
   hiprtcProgram p;
   if (hiprtcCreateProgram(&p, prog.c_str(), "gg_user_metric.hip", kRtcHeaderCount, kRtcHeaderSources,

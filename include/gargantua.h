@@ -410,7 +410,11 @@ int gg_set_hittables(gg_context *ctx, uint32_t n, const gg_hittable *objects);
  *     __device__ void gg_user_metric(const double x[4], double mass,
  *                                    double spin, double g[16]);
  * filling g[4*mu + nu] = g_mu,nu at x = (t, r, theta, phi) (any helper
- * functions it needs may precede it).  The library compiles it at run time
+ * functions it needs may precede it).  A source may also
+ * `#define GG_METRIC_STATIONARY` (g does not depend on t) and/or
+ * `#define GG_METRIC_AXISYMMETRIC` (g does not depend on phi): the two
+ * tensor evaluations of that coordinate's central difference, which are
+ * then exactly zero, are skipped, with bit-identical results.  The library compiles it at run time
  * (hipRTC) into its trace kernel; the Christoffel symbols are the reference's
  * central differences (numerical_christoffel, metric.rs:97-142, h = 1e-6) of
  * this tensor and its 4x4 inverse.  event_horizon is Metric::event_horizon:

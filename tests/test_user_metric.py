@@ -193,6 +193,23 @@ def test_gpu_user_metric_matches_oracle(renderer, oracle, with_user, disk_kind):
     _close(got, want)
 
 
+SYMMETRIC = "#define GG_METRIC_STATIONARY 1\n#define GG_METRIC_AXISYMMETRIC 1\n"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("disk_kind", [gg.GG_DISK_VOLUMETRIC, gg.GG_DISK_THIN], ids=["vol", "thin"])
+def test_gpu_symmetry_declarations_change_no_bit(renderer, disk_kind):
+    # Declaring no t and no phi dependence skips central differences that are
+    # exactly zero for such a tensor: every output is bit-identical.
+    disk = gg.AccretionDisk(kind=disk_kind, r_inner=5.0, r_outer=20.0)
+    for src, spin, rh in ((REISSNER_NORDSTROM, 0.6, 1.8), (KERR, 0.9, 1.0 + math.sqrt(0.19))):
+        plain = renderer.render(_scene(gg.UserMetric(src, 1.0, spin, rh), disk), "fp64")
+        sym = renderer.render(_scene(gg.UserMetric(SYMMETRIC + src, 1.0, spin, rh), disk), "fp64")
+        for k in ("cls", "r", "theta", "phi", "intensity", "attempts", "accepted", "state"):
+            np.testing.assert_array_equal(sym[k], plain[k], err_msg=k)
+        assert sym["stats"]["rhs_evals"] == plain["stats"]["rhs_evals"]
+
+
 @pytest.mark.gpu
 def test_gpu_user_kerr_tracks_builtin_kerr(renderer):
     # the generic path (numerical Christoffels of the Kerr tensor) against the
