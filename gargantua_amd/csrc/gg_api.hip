@@ -119,6 +119,7 @@ struct gg_context {
   DevBuf hit_objs, hit_nodes;
   uint32_t n_hit = 0;
   bool hit_off = false;
+  double hit_rmin = 0.0;      // distance from the origin to the nearest primitive box
   // gg_set_user_metric: the tensor's source, its horizon, and the trace
   // modules compiled from it, one per (disk mode 0-4, ray source 0-1)
   std::string user_src;
@@ -655,6 +656,16 @@ int gg_set_hittables(gg_context *ctx, uint32_t n, const gg_hittable *objects) {
   for (uint32_t i = 0; i < n; ++i) order[i] = i;
   nodes.reserve(2 * (size_t)n);
   build_bvh(boxes, order, 0, n, nodes);
+  // the nearest point of any primitive's (padded) box to the origin
+  double rmin = INFINITY;
+  for (const HitBox &b : boxes) {
+    double d2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double q = b.lo[k] > 0.0 ? b.lo[k] : (b.hi[k] < 0.0 ? -b.hi[k] : 0.0);
+      d2 += q * q;
+    }
+    rmin = std::min(rmin, std::sqrt(d2) * (1.0 - 1e-9));
+  }
   std::vector<SceneObj> leaf_order(n);
   for (uint32_t i = 0; i < n; ++i) leaf_order[i] = prim[order[i]];
   HIP_TRY(hipSetDevice(ctx->device));
@@ -667,6 +678,7 @@ int gg_set_hittables(gg_context *ctx, uint32_t n, const gg_hittable *objects) {
                          ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   ctx->n_hit = n;
+  ctx->hit_rmin = rmin;
   return GG_OK;
 }
 
@@ -882,6 +894,7 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
     if (with_hits) {
       ds[i].objs = ctx->hit_objs.p;
       ds[i].nodes = ctx->hit_nodes.p;
+      ds[i].scene_rmin = ctx->hit_rmin;
     }
     bool need_rgb = (o.rgb != nullptr) && !from_states;
     // rgb shading reads cls, intensity and the velocity planes: stage them

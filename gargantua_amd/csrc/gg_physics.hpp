@@ -60,6 +60,7 @@ struct DevScene {
   const void *objs;
   const void *nodes;
   double g_tt_obs;
+  double scene_rmin;      // no primitive reaches inside this radius (chords within it cannot hit)
   // batch bookkeeping
   uint64_t ray_begin;     // first global ray index of this scene
   uint64_t out_index;     // which output set (batch)

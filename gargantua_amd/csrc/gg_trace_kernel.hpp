@@ -260,7 +260,9 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
         L.accepted++;
         if (record && L.accepted % GRAD_CHUNK == 0) r_need = true;
       }
-      if (DISK >= 3 && acc) {
+      // A chord whose two end points lie inside the ball r < scene_rmin stays
+      // in it (the ball is convex) and no primitive reaches into it: no test.
+      if (DISK >= 3 && acc && fmax(p0[1], L.pos[1]) >= Sp->scene_rmin) {
         // The accepted step's Cartesian chord against the scene, before the
         // step's own result (tracer.rs:220-238, 282-305).
         double cb[3], ca[3];
