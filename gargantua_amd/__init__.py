@@ -340,7 +340,8 @@ class Scene:
         """Plain-JSON form of the scene (see from_dict)."""
         from dataclasses import asdict
         m = self.metric
-        return {"metric": {"kind": "kerr" if isinstance(m, Kerr) else "schwarzschild", **asdict(m)},
+        kind = "kerr" if isinstance(m, Kerr) else ("user" if isinstance(m, UserMetric) else "schwarzschild")
+        return {"metric": {"kind": kind, **asdict(m)},
                 "camera": {k: list(v) if isinstance(v, (tuple, list)) else v for k, v in asdict(self.camera).items()},
                 "disk": asdict(self.disk) if self.disk is not None else None,
                 "integrator": asdict(self.integrator), "width": self.width, "height": self.height,
@@ -350,7 +351,7 @@ class Scene:
     def from_dict(d: dict) -> "Scene":
         md = dict(d["metric"])
         kind = md.pop("kind")
-        metric = Kerr(**md) if kind == "kerr" else Schwarzschild(**md)
+        metric = Kerr(**md) if kind == "kerr" else (UserMetric(**md) if kind == "user" else Schwarzschild(**md))
         cam = d["camera"]
         return Scene(metric=metric,
                      camera=Camera(position=tuple(cam["position"]), look_at=tuple(cam["look_at"]), up=tuple(cam["up"]),
