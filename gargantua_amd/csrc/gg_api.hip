@@ -80,7 +80,7 @@ struct gg_context {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   uint32_t num_cus = 256;
-  gg_launch launch{256, 2, 16, 1, 0};
+  gg_launch launch{64, 8, 64, 1, 0};
   // cached batch graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -603,9 +603,9 @@ int gg_context_destroy(gg_context *ctx) {
 int gg_set_launch(gg_context *ctx, const gg_launch *l) {
   if (!ctx || !l) return fail(GG_ERR_INVALID_ARG, "NULL argument");
   gg_launch v = *l;
-  if (v.block_threads == 0) v.block_threads = 256;
-  if (v.blocks_per_cu == 0) v.blocks_per_cu = 2;
-  if (v.refill_interval == 0) v.refill_interval = 16;
+  if (v.block_threads == 0) v.block_threads = 64;
+  if (v.blocks_per_cu == 0) v.blocks_per_cu = 8;
+  if (v.refill_interval == 0) v.refill_interval = 64;
   if (v.graph == 0) v.graph = 1;
   if (v.graph > 2) return fail(GG_ERR_INVALID_ARG, "graph must be 0, 1 or 2");
   if (v.order > 1) return fail(GG_ERR_INVALID_ARG, "order must be 0 or 1");

@@ -229,9 +229,9 @@ typedef struct gg_stats {
 
 /* Launch shape of the persistent trace kernel. 0 = library default. */
 typedef struct gg_launch {
-  uint32_t block_threads;     /* 64..256, multiple of 64 (default 256)        */
-  uint32_t blocks_per_cu;     /* blocks launched per CU (default 2)           */
-  uint32_t refill_interval;   /* attempts between retired-lane refills (16)   */
+  uint32_t block_threads;     /* 64..256, multiple of 64 (default 64)         */
+  uint32_t blocks_per_cu;     /* blocks launched per CU (default 8)           */
+  uint32_t refill_interval;   /* attempts between retired-lane refills (64)   */
   uint32_t graph;             /* gg_render_batch: 1 = capture the launch in a
                                  hipGraph and replay it on later calls with the
                                  same shape (default), 2 = launch directly     */
