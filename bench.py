@@ -337,6 +337,20 @@ def bench_user(args):
                      "first_set_s": compile_s}
         _log(f"user {name}: {out[name]['kernel_ms']:.1f} ms")
     rend.close()
+    if not args.no_cpu:
+        # the FP64 oracle (the same tensor compiled for the host) on every 8th row and column
+        from oracle import Oracle
+        ora = Oracle(threads=max(1, min(16, len(os.sched_getaffinity(0)))))
+        ora.set_user_metric(rn)
+        scene = gg.Scene(metric=rn, camera=cam, disk=disk, integrator=gg.RK45Integrator(max_radius=200.0),
+                         width=size, height=size)
+        idx = np.array([py * size + px for py in range(0, size, 8) for px in range(0, size, 8)], dtype=np.uint64)
+        t0 = time.perf_counter()
+        ora.render_rays(scene, idx, state=False)
+        dt = time.perf_counter() - t0
+        ora.set_user_metric(None)
+        out["cpu_baseline"] = {"value": len(idx) / dt, "unit": "geodesics/s", "cores": ora.threads, "kind": "port",
+                               "sample": f"Reissner-Nordstrom tensor, every 8th row and column ({len(idx)} rays)"}
     out["tensor_vs_closed_form"] = out["kerr_tensor"]["kernel_ms"] / out["kerr_closed_form"]["kernel_ms"]
     out["symmetric_tensor_vs_closed_form"] = (out["kerr_tensor_symmetric"]["kernel_ms"] /
                                               out["kerr_closed_form"]["kernel_ms"])
