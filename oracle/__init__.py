@@ -222,15 +222,16 @@ class Oracle:
         os.makedirs(d, exist_ok=True)
         so = os.path.join(d, f"um_{tag}.so")
         if not os.path.exists(so):
-            src = os.path.join(d, f"um_{tag}.cpp")
+            src = os.path.join(d, f"um_{tag}_{os.getpid()}.cpp")
             with open(src, "w") as f:
                 f.write("#include <cmath>\nusing namespace std;\n#define __device__\n#define __host__\n"
                         "namespace gg {\n" + metric.source + "\n}\n"
                         "extern \"C\" void gg_user_metric_host(const double *x, double m, double a, double *g) "
                         "{ gg::gg_user_metric(x, m, a, g); }\n")
-            subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", so + ".tmp", src],
+            tmp = f"{so}.{os.getpid()}.tmp"
+            subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", tmp, src],
                            check=True)
-            os.replace(so + ".tmp", so)
+            os.replace(tmp, so)              # atomic: concurrent test processes see a whole library
         self._user_lib = C.CDLL(so)
         fn = C.cast(self._user_lib.gg_user_metric_host, C.c_void_p)
         rh = metric.event_horizon
