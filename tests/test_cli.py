@@ -55,3 +55,30 @@ def test_gpu_cli_image_matches_oracle(oracle, tmp_path, argv):
     assert close[~esc].all()
     if esc.any():
         assert close[esc].mean() >= 0.9
+
+
+@pytest.mark.gpu
+def test_gpu_cli_hdri_file(oracle, tmp_path):
+    # --hdri / --hdri-intensity (main.rs:66-72, 91-99): a Radiance file through HdriSky.from_file
+    from gargantua_amd import hdr
+    th = (np.arange(16) + 0.5) / 16 * math.pi
+    ph = (np.arange(32) + 0.5) / 32 * 2 * math.pi
+    sky = np.stack([0.6 + 0.4 * np.cos(th)[:, None] + 0 * ph[None, :],
+                    0.5 + 0.3 * np.sin(ph)[None, :] * np.sin(th)[:, None],
+                    0.3 + 0.2 * np.cos(2 * ph)[None, :] + 0 * th[:, None]], axis=-1)
+    path = tmp_path / "sky.hdr"
+    hdr.write(path, sky)
+    out = tmp_path / "o.png"
+    args = cli.parse_args(["--metric", "schwarzschild", "--width", "32", "--height", "24", "--hdri", str(path),
+                           "--hdri-intensity", "1.5", "--quiet", "--output", str(out)])
+    img = cli.run(args)
+    cam = gg.Camera(position=(0.0, 30.0, math.pi / 2, 0.0), look_at=(-1.0, 0.0, 0.0), up=(0.0, 0.0, 1.0),
+                    fov_y_radians=math.radians(60.0), aspect=32 / 24)
+    scene = gg.Scene(metric=gg.Schwarzschild(1.0), camera=cam,
+                     integrator=gg.RK45Integrator(max_radius=200.0, max_steps=5000), width=32, height=24)
+    oracle.set_sky_hdri(hdr.read(path), 1.5)
+    try:
+        want = oracle.render(scene, rgb=True)["rgb"].reshape(24, 32, 3)
+    finally:
+        oracle.set_sky_hdri(None)
+    assert np.abs(img.astype(int) - want.astype(int)).max() <= 1
