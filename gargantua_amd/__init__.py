@@ -34,8 +34,8 @@ from . import _abi
 from . import configs  # noqa: F401  (scene builders for the benchmark configurations)
 from . import hdr, png  # noqa: F401  (Radiance .hdr input, PNG output)
 from ._abi import (GG_CLASS_DISK, GG_CLASS_ESCAPED, GG_CLASS_HORIZON, GG_CLASS_MAX_STEPS, GG_CLASS_SCENE,  # noqa: F401
-                   GG_DISK_NONE, GG_DISK_THIN, GG_DISK_VOLUMETRIC, GG_ERR_REFERENCE, GG_ERR_RTOL_ATOL,
-                   GG_METRIC_KERR, GG_METRIC_SCHWARZSCHILD)
+                   GG_DISK_NONE, GG_DISK_THIN, GG_DISK_VOLUMETRIC, GG_ERR_INVALID_ARG, GG_ERR_REFERENCE,
+                   GG_ERR_RTOL_ATOL, GG_ERR_UNSUPPORTED, GG_METRIC_KERR, GG_METRIC_SCHWARZSCHILD, GG_OK)
 
 PRECISIONS = {"fp64": _abi.GG_PREC_FP64, "mixed": _abi.GG_PREC_MIXED, "fp64fast": _abi.GG_PREC_FP64_FAST}
 
@@ -549,7 +549,10 @@ class Renderer:
             self.n_hittables = 0
             return
         arr = world.to_c()
-        _check(self.lib, self.lib.gg_set_hittables(self.ctx, len(world), C.cast(arr, C.c_void_p)), "gg_set_hittables")
+        rc = self.lib.gg_set_hittables(self.ctx, len(world), C.cast(arr, C.c_void_p))
+        if rc not in (GG_OK, GG_ERR_INVALID_ARG, GG_ERR_UNSUPPORTED):
+            self.n_hittables = 0            # a device error leaves the context without geometry
+        _check(self.lib, rc, "gg_set_hittables")
         self.n_hittables = len(world)
 
     def set_user_metric(self, metric: Optional[UserMetric]) -> None:
@@ -559,6 +562,7 @@ class Renderer:
             self._user_metric = None
             return
         rh = float("nan") if metric.event_horizon is None else float(metric.event_horizon)
+        self._user_metric = None            # the library drops the previous source before compiling
         _check(self.lib, self.lib.gg_set_user_metric(self.ctx, metric.source.encode(), rh), "gg_set_user_metric")
         self._user_metric = (metric.source, metric.event_horizon)
 

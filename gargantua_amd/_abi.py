@@ -11,6 +11,8 @@ import os
 ABI_VERSION = 1
 
 GG_OK = 0
+GG_ERR_INVALID_ARG = -1
+GG_ERR_UNSUPPORTED = -5
 GG_METRIC_SCHWARZSCHILD = 0
 GG_METRIC_KERR = 1
 GG_METRIC_USER = 2

@@ -626,6 +626,9 @@ int gg_set_sky_hdri(gg_context *ctx, uint32_t width, uint32_t height, const floa
   if ((uint64_t)width * height > (1ull << 28)) return fail(GG_ERR_INVALID_ARG, "HDRI sky larger than 2^28 texels");
   if (!std::isfinite(intensity)) return fail(GG_ERR_INVALID_ARG, "HDRI sky intensity must be finite");
   HIP_TRY(hipSetDevice(ctx->device));
+  // the buffer may be reallocated below: until the copy is done the context
+  // has no sky, so a failure leaves the procedural one
+  ctx->sky_w = ctx->sky_h = 0;
   const size_t bytes = 12ull * width * height;
   int rc = ctx->sky.ensure(bytes);
   if (rc) return rc;
@@ -669,6 +672,9 @@ int gg_set_hittables(gg_context *ctx, uint32_t n, const gg_hittable *objects) {
   std::vector<SceneObj> leaf_order(n);
   for (uint32_t i = 0; i < n; ++i) leaf_order[i] = prim[order[i]];
   HIP_TRY(hipSetDevice(ctx->device));
+  // the buffers may be reallocated below: until both copies are done the
+  // context has no hittables, so a failure leaves none
+  ctx->n_hit = 0;
   int rc;
   if ((rc = ctx->hit_objs.ensure(sizeof(SceneObj) * n))) return rc;
   if ((rc = ctx->hit_nodes.ensure(sizeof(BvhNode) * nodes.size()))) return rc;
@@ -742,10 +748,6 @@ int gg_scene_rays(const gg_scene *scene, uint64_t *n_rays) {
 
 namespace {
 
-// Common body of gg_render / gg_render_batch / gg_trace_states.
-// The trace kernel for METRIC 2 (gg_user_metric.hpp) with the context's
-// tensor source, disk mode `mode` and ray source `src`, compiled with hipRTC
-// on first use, and the helper kernel that evaluates the tensor at a point.
 // Compile the trace kernel for METRIC 2 with tensor source `user`, disk mode
 // `mode` and ray source `src`, and the helper that evaluates the tensor at a
 // point, with hipRTC (no device needed): the code object and both kernels'
@@ -835,6 +837,7 @@ int user_metric_at(gg_context *ctx, hipFunction_t at, const gg_metric &m, const 
   return GG_OK;
 }
 
+// Common body of gg_render / gg_render_batch / gg_trace_states.
 int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int precision, const gg_outputs *outs,
                 const double *states_host, uint64_t n_states, gg_stats *stats) {
   auto t_start = std::chrono::steady_clock::now();

@@ -386,7 +386,9 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target,
  * +z pole, columns wrapping in azimuth; bilinear with wrap-x / clamp-y,
  * scaled by intensity.  pixels (host or device) are copied.  width = 0
  * restores the procedural starfield (ProceduralSky, the reference's
- * default); this call is the library's counterpart of RenderOptions.sky. */
+ * default); this call is the library's counterpart of RenderOptions.sky.
+ * An invalid argument leaves the previous sky; a device error (allocation,
+ * copy) leaves the procedural one. */
 int gg_set_sky_hdri(gg_context *ctx, uint32_t width, uint32_t height,
                     const float *pixels, float intensity);
 
@@ -400,7 +402,8 @@ int gg_set_sky_hdri(gg_context *ctx, uint32_t width, uint32_t height,
  * shade_hit x static g^4, plus the disk emission in front of it).  n = 0
  * clears the scene.  GG_ERR_UNSUPPORTED for a thin disk or a batch while
  * hittables are set, and for a metal with fuzz > 0 (its reference shading
- * draws a random number). */
+ * draws a random number).  An invalid primitive leaves the previous
+ * geometry; a device error (allocation, copy) leaves none. */
 int gg_set_hittables(gg_context *ctx, uint32_t n, const gg_hittable *objects);
 
 /* A metric known only by its tensor, for scenes of kind GG_METRIC_USER on

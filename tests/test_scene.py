@@ -327,6 +327,12 @@ def test_gpu_scene_unsupported_and_reset(renderer):
             renderer.render_batch([sc, sc], "fp64")
         lit = renderer.render(sc, "fp64", rgb=True)
         assert np.any(lit["cls"] == gg.GG_CLASS_SCENE)
+        # an invalid primitive leaves the geometry already set (gargantua.h, gg_set_hittables)
+        with pytest.raises(gg.GargantuaError):
+            renderer.set_hittables(gg.World([gg.Sphere((0, 0, 0), 0.0, gg.Lambertian((1, 1, 1)))]))
+        kept = renderer.render(sc, "fp64", rgb=True)
+        np.testing.assert_array_equal(kept["cls"], lit["cls"])
+        np.testing.assert_array_equal(kept["rgb"], lit["rgb"])
     finally:
         renderer.set_hittables(None)
     again = renderer.render(sc, "fp64", rgb=True)

@@ -260,6 +260,13 @@ def test_gpu_user_metric_errors(renderer):
         renderer.render(_scene(m, w=8, h=8), "mixed")
     with pytest.raises(gg.GargantuaError):
         renderer.render_batch([_scene(m, w=8, h=8)] * 2, "fp64")
+    # a source that fails to compile drops the previous one; the next render of m compiles it again
+    first = renderer.render(_scene(m, w=8, h=8), "fp64")
+    with pytest.raises(gg.GargantuaError):
+        renderer.set_user_metric(gg.UserMetric("__device__ void gg_user_metric() { oops }"))
+    again = renderer.render(_scene(m, w=8, h=8), "fp64")
+    np.testing.assert_array_equal(again["cls"], first["cls"])
+    np.testing.assert_array_equal(again["r"], first["r"])
     renderer.set_user_metric(None)
     sc = _scene(m, w=8, h=8)
     o = gg.alloc_outputs(sc.n_rays)
