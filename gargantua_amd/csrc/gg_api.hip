@@ -33,7 +33,7 @@ namespace {
 thread_local std::string g_last_error;
 
 int fail(int code, const char *fmt, ...) {
-  char buf[512];
+  char buf[2048];   // room for a hipRTC log excerpt of 1500 characters
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(buf, sizeof(buf), fmt, ap);

@@ -155,6 +155,13 @@ def test_user_metric_sources_compile_for_every_mode():
     bad = b"__device__ void gg_user_metric(const double x[4], double m, double a, double g[16]) { g[0] = y; }"
     assert lib.gg_check_user_metric(bad, 0, log, len(log)) != 0
     assert b"error" in log.value
+    # gg_last_error carries up to 1500 characters of the log: every one of these errors is named
+    many = b"__device__ void gg_user_metric(const double x[4], double m, double a, double g[16]) {" + \
+        b"".join(b" g[%d] = undefined_name_%d;" % (i, i) for i in range(12)) + b" }"
+    assert lib.gg_check_user_metric(many, 0, log, len(log)) != 0
+    assert len(log.value) > 1500
+    msg = lib.gg_last_error()
+    assert len(msg) > 1200 and b"undefined_name_0" in msg and b"undefined_name_3" in msg
 
 
 # --- GPU: the same tensor compiled at run time ---------------------------------------------
