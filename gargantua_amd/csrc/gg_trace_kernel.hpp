@@ -54,10 +54,12 @@ __device__ inline uint32_t find_scene(const DevScene *scenes, uint32_t n, uint64
 // DISK 0 none, 1 volumetric, 2 thin; 3 the Hittable scene alone
 // (trace_ray_with_scene), 4 volumetric disk and scene
 // (trace_ray_with_disk_and_scene).
-template <int METRIC, int DISK, int PREC, int SRC, int BATCH>
+// GG_MIN_WAVES: waves per SIMD the register budget must allow (2: 256
+// VGPRs; the run-time compiled user metrics take 1, for 512 VGPRs + AGPRs).
 #ifndef GG_MIN_WAVES
 #define GG_MIN_WAVES 2
 #endif
+template <int METRIC, int DISK, int PREC, int SRC, int BATCH>
 __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T) {
 #pragma clang fp contract(off)
   const uint32_t lane = threadIdx.x & 63;

@@ -79,11 +79,20 @@ __device__ inline void user_g(const DevScene &S, const double x[4], double g[16]
 // GG_METRIC_AXISYMMETRIC (no phi dependence) skips those two tensor
 // evaluations: for such a tensor g(x + h e_s) and g(x - h e_s) are equal, so
 // the difference the reference computes is exactly +0 and the result is
-// the same bit for bit.
+// the same bit for bit.  The derivatives are read from the upper triangle
+// (d_a g_{s b} as d_a g_{min, max}): the tensor is symmetric, so for a source
+// that fills g_{mu nu} and g_{nu mu} alike this is the reference's value, and
+// the 24 lower-triangle differences, and the registers they would hold, drop
+// out of the kernel.
+// Index of g_{i j} in the upper triangle (i <= j) of the row-major tensor.
+__device__ constexpr int sym(int i, int j) { return i <= j ? 4 * i + j : 4 * j + i; }
+
 __device__ inline d4 user_rhs(const DevScene &S, const double x[4], d4 v) {
 #pragma clang fp contract(off)
   const double h = 1e-6;
   double dg[4][16];
+  // unrolled, so that dg lives in registers rather than in private memory
+#pragma unroll
   for (int s = 0; s < 4; ++s) {
 #ifdef GG_METRIC_STATIONARY
     if (s == 0) {
@@ -118,8 +127,8 @@ __device__ inline d4 user_rhs(const DevScene &S, const double x[4], d4 v) {
     for (int a = 0; a < 4; ++a)
       for (int b = 0; b < 4; ++b) {
         double sum = 0.0;
-        for (int s = 0; s < 4; ++s)      // g^{mu s} (d_a g_{s b} + d_b g_{s a} - d_s g_{a b}); row-major g[4 s + b]
-          sum += gi[4 * s + mu] * (dg[a][4 * s + b] + dg[b][4 * s + a] - dg[s][4 * a + b]);
+        for (int s = 0; s < 4; ++s)      // g^{mu s} (d_a g_{s b} + d_b g_{s a} - d_s g_{a b})
+          sum += gi[4 * s + mu] * (dg[a][sym(s, b)] + dg[b][sym(s, a)] - dg[s][sym(a, b)]);
         const double gamma = 0.5 * sum;
         acc[mu] -= gamma * vv[a] * vv[b];
       }

@@ -413,7 +413,9 @@ int gg_set_hittables(gg_context *ctx, uint32_t n, const gg_hittable *objects);
  *     __device__ void gg_user_metric(const double x[4], double mass,
  *                                    double spin, double g[16]);
  * filling g[4*mu + nu] = g_mu,nu at x = (t, r, theta, phi) (any helper
- * functions it needs may precede it).  A source may also
+ * functions it needs may precede it).  The tensor is symmetric: the
+ * Christoffel symbols read its derivatives from the upper triangle
+ * (mu <= nu), so g_mu,nu and g_nu,mu should hold the same value.  A source may also
  * `#define GG_METRIC_STATIONARY` (g does not depend on t) and/or
  * `#define GG_METRIC_AXISYMMETRIC` (g does not depend on phi): the two
  * tensor evaluations of that coordinate's central difference, which are
