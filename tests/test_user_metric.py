@@ -59,6 +59,21 @@ def with_user(oracle):
     oracle.set_user_metric(None)
 
 
+@pytest.mark.parametrize("name", ["SCHWARZSCHILD", "KERR", "REISSNER_NORDSTROM"])
+def test_tensor_sources_are_bitwise_symmetric(oracle, with_user, name):
+    # The device reads d_a g_{s b} from the upper triangle; it equals the reference's full
+    # read (metric.rs:97-142) exactly when g_{mu nu} and g_{nu mu} carry the same bits.
+    m = gg.UserMetric(globals()[name], mass=1.0, spin=0.6 if name == "REISSNER_NORDSTROM" else 0.9)
+    with_user(m)
+    rng = np.random.default_rng(7)
+    for _ in range(50):
+        pos = (rng.uniform(-5, 5), rng.uniform(2.5, 40.0), rng.uniform(0.05, math.pi - 0.05), rng.uniform(0, 2 * math.pi))
+        g = oracle.metric_tensor(m, pos)
+        assert np.isfinite(g).all() and g[0, 0] < 0.0 < g[1, 1]   # the source's tensor, outside any horizon
+        np.testing.assert_array_equal(g, g.T)
+    assert name != "KERR" or g[0, 3] != 0.0                 # the off-diagonal pair is really exercised
+
+
 def test_inverse4_identity(oracle):
     rng = np.random.default_rng(3)
     for _ in range(20):
