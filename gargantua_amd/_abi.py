@@ -98,7 +98,7 @@ class gg_grad_scene(C.Structure):
                 ("emissivity_q", C.c_double), ("r_inner", C.c_double), ("r_outer", C.c_double),
                 ("edge_width", C.c_double), ("r_camera", C.c_double), ("fov_y_radians", C.c_double),
                 ("integrator", gg_integrator), ("width", C.c_uint32), ("height", C.c_uint32),
-                ("precision", C.c_int32), ("_pad", C.c_uint32)]
+                ("precision", C.c_int32), ("frozen_steps", C.c_uint32)]
 
 
 class gg_grad_stats(C.Structure):

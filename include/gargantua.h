@@ -323,7 +323,13 @@ typedef struct gg_grad_scene {
                               and room for the recorded steps (the mixed
                               kernel's steps depend on the lane grouping, so
                               only the render's own records are its path) */
-  uint32_t _pad;
+  uint32_t frozen_steps;   /* 0 (default): the step sizes carry derivatives
+                              through the error estimate and the step-size
+                              controller (geodesic.rs:308-325): the gradient
+                              of the loss the render computes.  1: each
+                              ray's step sizes are held at the render's
+                              values (the discrete map with its step
+                              sequence fixed)                                */
 } gg_grad_scene;
 
 /* The gg_scene gg_grad renders: Kerr(mass, spin), camera at

@@ -54,6 +54,7 @@ _SIGS = [
     ("oracle_loss_frozen", C.c_int, [C.POINTER(_abi.gg_grad_scene), C.POINTER(_abi.gg_grad_scene), _P,
                                      C.POINTER(_D), _P, C.c_int]),
     ("oracle_grad_scene_to_scene", C.c_int, [C.POINTER(_abi.gg_grad_scene), C.POINTER(_abi.gg_scene)]),
+    ("oracle_grad_decisions", C.c_int, [C.POINTER(_abi.gg_grad_scene), _P, C.c_int]),
 ]
 
 
@@ -311,6 +312,26 @@ class Oracle:
         out = {"loss": loss.value, "grad": grad, "intensity": inten}
         if want_dI:
             out["dI"] = dI.reshape(n, 4)
+        return out
+
+    def grad_render(self, gscene):
+        """The oracle render of the scene oracle_grad differentiates (cls, intensity, attempts, accepted, state)."""
+        sc = _abi.gg_scene()
+        rc = self.lib.oracle_grad_scene_to_scene(C.byref(gscene), C.byref(sc))
+        if rc:
+            raise RuntimeError(f"oracle_grad_scene_to_scene failed: {rc}")
+        o = alloc_outputs(gscene.width * gscene.height, state=True)
+        rc = self.lib.oracle_render(C.byref(sc), C.byref(outputs_struct(o)), self.threads)
+        if rc:
+            raise RuntimeError(f"oracle_render failed: {rc}")
+        return o
+
+    def grad_decisions(self, gscene):
+        """[N] uint64: hash of each ray's sequence of step decisions in the render oracle_grad differentiates."""
+        out = np.zeros(gscene.width * gscene.height, dtype=np.uint64)
+        rc = self.lib.oracle_grad_decisions(C.byref(gscene), out.ctypes.data, self.threads)
+        if rc:
+            raise RuntimeError(f"oracle_grad_decisions failed: {rc}")
         return out
 
     def loss_frozen(self, ref, gscene, target):

@@ -163,6 +163,7 @@ double disk_emission(const DiskT<double> &d, const V4<double> &pos) {
 
 template <class T> struct TraceOut {
   uint8_t cls;
+  uint64_t decisions = 14695981039346656037ull;   // FNV-1a of every attempt's StepInfo::decisions
   T intensity;
   double color[3] = {0.0, 0.0, 0.0};   // RayOutcome::Scene { color }
   uint32_t attempts, accepted;
@@ -300,7 +301,9 @@ double g4_static(const MetricP &mp, double a, const V3<double> &p, double g_tt_o
 // theta = pi/2 sits at s = (pi/2 - th0)/(th1 - th0) of it; r_c and the photon
 // momentum are interpolated linearly at s, and a hit gives
 // I = r_c^-q g^4 (x logistic edge weights when edge_width > 0).
-template <class T>
+// H: the type of the step size (double, or T to differentiate the
+// step-size controller as well; see rk45_step).
+template <class T, class H = double>
 TraceOut<T> trace(const MetricP &mp, const T &a, const IntegP &ip, const DiskT<T> &disk, const V4<T> &observer,
                   State<T> st, StepLog *log = nullptr) {
   using std::sqrt; using std::exp; using std::pow;
@@ -308,7 +311,7 @@ TraceOut<T> trace(const MetricP &mp, const T &a, const IntegP &ip, const DiskT<T
   o.intensity = T(0.0);
   o.accepted = 0;
   o.rhs = 0;
-  double h = ip.initial_step;
+  H h = H(ip.initial_step);
   T intensity = T(0.0);
   double transmission = 1.0;
   T sqrt_neg_gtt_obs = T(0.0);
@@ -334,9 +337,12 @@ TraceOut<T> trace(const MetricP &mp, const T &a, const IntegP &ip, const DiskT<T
     State<T> before = st;
     StepInfo info;
     int force = -1;
-    if (log && log->replaying && k < log->h.size()) { h = log->h[k]; force = log->acc[k]; }
-    if (log && log->recording) log->h.push_back(h);
+    if constexpr (std::is_same<H, double>::value) {
+      if (log && log->replaying && k < log->h.size()) { h = log->h[k]; force = log->acc[k]; }
+      if (log && log->recording) log->h.push_back(h);
+    }
     StepResult res = rk45_step(mp, a, ip, st, h, &info, force);
+    o.decisions = (o.decisions ^ info.decisions) * 1099511628211ull;
     if (log && log->recording) log->acc.push_back(info.accepted ? 1 : 0);
     o.rhs += info.rhs;
     if (info.accepted) ++o.accepted;
@@ -906,11 +912,13 @@ double oracle_r_isco(double mass, double spin) {
 
 // ---- differentiable thin-disk render: FP64 forward-mode dual numbers ------
 //
-// Tangent directions 0..3 = (spin, inclination, q, r_inner).  The step-size
-// controller reads values only, so this is the gradient of the discrete
-// algorithm with its step sequence held fixed.  mode 0: L and dL/dtheta by
-// dual numbers; mode 1: L only with the step sequence replayed from the
-// unperturbed run of `ref` (for frozen-step finite differences).
+// Tangent directions 0..3 = (spin, inclination, q, r_inner).  oracle_grad:
+// L and dL/dtheta by dual numbers, the step sizes carrying tangents through
+// the error estimate and the controller (the gradient of the loss the render
+// computes), or, with frozen_steps set, held at their values (the gradient of
+// the discrete algorithm with its step sequence fixed).  oracle_loss_frozen:
+// L only, with the step sequence replayed from the unperturbed run of `ref`
+// (frozen-step finite differences).
 namespace {
 struct GradCommon {
   MetricP mp;
@@ -973,7 +981,8 @@ int oracle_grad(const gg_grad_scene *g, const double *target, double *loss, doub
   parallel_for(n, nthreads, 16, [&](uint64_t i) {
     uint32_t px = (uint32_t)(i % g->width), py = (uint32_t)(i / g->width);
     State<D> st = from_camera(mp, a, cam, img, px, py, g->width, g->height);
-    TraceOut<D> o = trace(mp, a, ip, disk, cam.pos, st);
+    TraceOut<D> o = g->frozen_steps ? trace<D, double>(mp, a, ip, disk, cam.pos, st)
+                                    : trace<D, D>(mp, a, ip, disk, cam.pos, st);
     D I = o.intensity;
     if (intensity) intensity[i] = I.v;
     if (dI) for (int k = 0; k < 4; ++k) dI[4 * i + k] = I.d[k];
@@ -988,6 +997,31 @@ int oracle_grad(const gg_grad_scene *g, const double *target, double *loss, doub
   }
   if (loss) *loss = L / (double)n;
   for (int k = 0; k < 4; ++k) grad[k] = G[k] / (double)n;
+  return 0;
+}
+
+// Per pixel, a hash of the sequence of discrete decisions (accept/reject,
+// err == 0, clamps of h, invalid stages, pole reflections) of the FP64 render
+// oracle_grad differentiates.  Where two parameter sets give equal hashes, the
+// loss is smooth between them.
+int oracle_grad_decisions(const gg_grad_scene *g, uint64_t *out, int nthreads) {
+  if (!g || !out || g->width == 0 || g->height == 0 || g->mass <= 0.0) return GG_ERR_INVALID_ARG;
+  gg_metric gm{GG_METRIC_KERR, 0, g->mass, g->spin};
+  MetricP mp = make_metric(gm);
+  IntegP ip = make_integ(g->integrator);
+  CameraT<double> cam = incl_camera<double>(g->inclination, g->r_camera, g->fov_y_radians,
+                                            (double)g->width / (double)g->height);
+  DiskT<double> disk;
+  disk.kind = GG_DISK_THIN; disk.r_inner = g->r_inner; disk.q = g->emissivity_q; disk.r_outer = g->r_outer;
+  disk.edge_width = g->edge_width; disk.scale_height = 1.0; disk.density_0 = 1.0; disk.temperature_0 = 1.0;
+  disk.absorption = 0.1;
+  gg_image img{g->width, g->height, 1, 0, 0};
+  uint64_t n = (uint64_t)g->width * g->height;
+  parallel_for(n, nthreads, 16, [&](uint64_t i) {
+    uint32_t px = (uint32_t)(i % g->width), py = (uint32_t)(i / g->width);
+    State<double> st = from_camera(mp, mp.spin, cam, img, px, py, g->width, g->height);
+    out[i] = trace(mp, mp.spin, ip, disk, cam.pos, st).decisions;
+  });
   return 0;
 }
 

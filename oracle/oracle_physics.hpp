@@ -451,7 +451,12 @@ struct StepInfo {
   bool reflected = false;
   int rhs = 0;          // RHS evaluations made by this attempt
   double h_used = 0.0;  // h of the attempt
+  // The attempt's discrete decisions (STEP_DEC_* bits): accept, err == 0,
+  // a clamp of the next h, an invalid stage.  Two runs with equal decision
+  // sequences differ smoothly in their parameters.
+  uint32_t decisions = 0;
 };
+enum { STEP_DEC_ACCEPT = 1, STEP_DEC_ERR_ZERO = 2, STEP_DEC_CLAMP = 4, STEP_DEC_INVALID = 8, STEP_DEC_REFLECT = 16 };
 
 // nalgebra Vector4::norm(): sqrt((x0^2 + x2^2) + (x1^2 + x3^2)).
 inline double norm4(const double d[4]) {
@@ -461,19 +466,32 @@ inline double norm4(const double d[4]) {
   return std::sqrt(0.0 + (a + b));
 }
 
+// x as the scalar type E: the value alone when E is double, else x itself.
+template <class E, class T> inline E as_scalar(const T &x) {
+  if constexpr (std::is_same<E, double>::value) return val(x); else return E(x);
+}
+// f64::max / fmax on values, carrying the tangent of the operand it returns.
+inline double fmax_t(double x, double y) { return std::fmax(x, y); }
+template <class T> inline T fmax_t(const T &x, const T &y) { return std::fmax(val(x), val(y)) == val(x) ? x : y; }
+
 // RK45Integrator::step (geodesic.rs:185-328): Dormand-Prince velocity update,
 // forward-Euler position, absolute error on ||new_vel - vel_err||_2.
-// The step-size controller acts on values only (h is a plain double), which
-// is what makes the dual-number instantiation a discretise-then-differentiate
-// gradient with the step sequence frozen.
+// H is the type of the step size h.  H = double: the controller acts on
+// values only, so a dual-number T gives the discretise-then-differentiate
+// gradient with the step sizes frozen.  H = T: the error estimate and the
+// controller (geodesic.rs:308-325) carry tangents too, so the gradient is
+// that of the loss the algorithm computes, step sizes included; decisions
+// (accept/reject, the clamps, err == 0) read values only, so the value part
+// is the plain double run bit for bit.
 // `force` >= 0 replays a recorded decision (1 accept, 0 reject) instead of
 // testing the error estimate; the finite-difference gradient check uses it to
 // hold the step sequence fixed.
-template <class T>
-StepResult rk45_step(const MetricP &mp, const T &a, const IntegP &ip, State<T> &st, double &h,
+template <class T, class H = double>
+StepResult rk45_step(const MetricP &mp, const T &a, const IntegP &ip, State<T> &st, H &h,
                      StepInfo *info = nullptr, int force = -1) {
+  using E = H;     // type of the error estimate
   V4<T> pos = st.pos, vel = st.vel;
-  if (info) { info->accepted = false; info->reflected = false; info->rhs = 0; info->h_used = h; }
+  if (info) { info->accepted = false; info->reflected = false; info->rhs = 0; info->h_used = val(h); }
   if (val(pos[1]) > ip.max_radius) return STEP_ESCAPED;                   // :194-196
   double rh;
   if (event_horizon(mp, &rh)) {
@@ -496,28 +514,28 @@ StepResult rk45_step(const MetricP &mp, const T &a, const IntegP &ip, State<T> &
   V4<T> k1 = geodesic_acceleration(mp, a, pos, vel); ++nrhs;               // :245
   V4<T> p2, v2, p3, v3, p4, v4, p5, v5, p6, v6;
   for (int i = 0; i < 4; ++i) { p2[i] = pos[i] + h * a2 * vel[i]; v2[i] = vel[i] + h * b21 * k1[i]; }  // :248-249
-  if (!is_valid_position(p2)) { h *= 0.5; if (info) info->rhs = nrhs; return STEP_CONTINUE; }
+  if (!is_valid_position(p2)) { h = h * 0.5; if (info) { info->rhs = nrhs; info->decisions = STEP_DEC_INVALID; } return STEP_CONTINUE; }
   V4<T> k2 = geodesic_acceleration(mp, a, p2, v2); ++nrhs;
   for (int i = 0; i < 4; ++i) { p3[i] = pos[i] + h * a3 * vel[i]; v3[i] = vel[i] + h * (b31 * k1[i] + b32 * k2[i]); }
-  if (!is_valid_position(p3)) { h *= 0.5; if (info) info->rhs = nrhs; return STEP_CONTINUE; }
+  if (!is_valid_position(p3)) { h = h * 0.5; if (info) { info->rhs = nrhs; info->decisions = STEP_DEC_INVALID; } return STEP_CONTINUE; }
   V4<T> k3 = geodesic_acceleration(mp, a, p3, v3); ++nrhs;
   for (int i = 0; i < 4; ++i) {
     p4[i] = pos[i] + h * a4 * vel[i];
     v4[i] = vel[i] + h * (b41 * k1[i] + b42 * k2[i] + b43 * k3[i]);
   }
-  if (!is_valid_position(p4)) { h *= 0.5; if (info) info->rhs = nrhs; return STEP_CONTINUE; }
+  if (!is_valid_position(p4)) { h = h * 0.5; if (info) { info->rhs = nrhs; info->decisions = STEP_DEC_INVALID; } return STEP_CONTINUE; }
   V4<T> k4 = geodesic_acceleration(mp, a, p4, v4); ++nrhs;
   for (int i = 0; i < 4; ++i) {
     p5[i] = pos[i] + h * a5 * vel[i];
     v5[i] = vel[i] + h * (b51 * k1[i] + b52 * k2[i] + b53 * k3[i] + b54 * k4[i]);
   }
-  if (!is_valid_position(p5)) { h *= 0.5; if (info) info->rhs = nrhs; return STEP_CONTINUE; }
+  if (!is_valid_position(p5)) { h = h * 0.5; if (info) { info->rhs = nrhs; info->decisions = STEP_DEC_INVALID; } return STEP_CONTINUE; }
   V4<T> k5 = geodesic_acceleration(mp, a, p5, v5); ++nrhs;
   for (int i = 0; i < 4; ++i) {
     p6[i] = pos[i] + h * vel[i];
     v6[i] = vel[i] + h * (b61 * k1[i] + b62 * k2[i] + b63 * k3[i] + b64 * k4[i] + b65 * k5[i]);
   }
-  if (!is_valid_position(p6)) { h *= 0.5; if (info) info->rhs = nrhs; return STEP_CONTINUE; }
+  if (!is_valid_position(p6)) { h = h * 0.5; if (info) { info->rhs = nrhs; info->decisions = STEP_DEC_INVALID; } return STEP_CONTINUE; }
   V4<T> k6 = geodesic_acceleration(mp, a, p6, v6); ++nrhs;
   V4<T> new_vel;
   for (int i = 0; i < 4; ++i)                                              // :294-295
@@ -528,41 +546,50 @@ StepResult rk45_step(const MetricP &mp, const T &a, const IntegP &ip, State<T> &
     vel_err[i] = vel[i] + h * (d1 * k1[i] + d3 * k3[i] + d4 * k4[i] + d5 * k5[i] + d6 * k6[i] + d7 * k7[i]);
   if (info) info->rhs = nrhs;
 
-  double diff[4];
-  for (int i = 0; i < 4; ++i) diff[i] = val(new_vel[i]) - val(vel_err[i]);
-  double err;       // the quantity compared against the acceptance threshold
+  using std::sqrt; using std::fabs; using std::pow;
+  E diff[4];
+  for (int i = 0; i < 4; ++i) diff[i] = as_scalar<E>(new_vel[i]) - as_scalar<E>(vel_err[i]);
+  E err;            // the quantity compared against the acceptance threshold
   double thresh;    // tolerance (reference) or 1 (scaled)
   if (ip.err_mode == GG_ERR_RTOL_ATOL) {
-    double s = 0.0;
+    E s = E(0.0);
     for (int i = 0; i < 4; ++i) {
-      double sc = ip.atol + ip.rtol * std::fmax(std::fabs(val(vel[i])), std::fabs(val(new_vel[i])));
-      double q = diff[i] / sc;
+      E sc = ip.atol + ip.rtol * fmax_t(E(fabs(as_scalar<E>(vel[i]))), E(fabs(as_scalar<E>(new_vel[i]))));
+      E q = diff[i] / sc;
       s = s + q * q;
     }
-    err = std::sqrt(0.25 * s);
+    err = sqrt(0.25 * s);
     thresh = 1.0;
   } else {
-    err = norm4(diff);                                                     // :308
+    // nalgebra Vector4::norm: sqrt((x0^2 + x2^2) + (x1^2 + x3^2))  (:308)
+    E x0 = diff[0] * diff[0], x1 = diff[1] * diff[1], x2 = diff[2] * diff[2], x3 = diff[3] * diff[3];
+    x0 = x0 + x2;
+    x1 = x1 + x3;
+    err = sqrt(0.0 + (x0 + x1));
     thresh = ip.tolerance;
   }
 
-  bool accept = (err < thresh || h <= ip.min_step);                        // :310
+  bool accept = (val(err) < thresh || val(h) <= ip.min_step);              // :310
   if (force >= 0) accept = (force == 1);
   if (accept) {
     for (int i = 0; i < 4; ++i) st.pos[i] = pos[i] + h * vel[i];           // :312
     st.vel = new_vel;
     st.lambda = st.lambda + h;
     bool refl = normalize_spherical_coords(st.pos);
-    if (info) { info->accepted = true; info->reflected = refl; }
-    if (err > 0.0) {                                                       // :318-321
-      double hn = h * 0.9 * std::pow(thresh / err, 0.2);
-      if (hn < ip.min_step) hn = ip.min_step;
-      if (hn > ip.max_step) hn = ip.max_step;
+    uint32_t dec = STEP_DEC_ACCEPT | (refl ? STEP_DEC_REFLECT : 0u);
+    if (val(err) > 0.0) {                                                  // :318-321
+      H hn = h * 0.9 * pow(thresh / err, 0.2);
+      if (val(hn) < ip.min_step) { hn = H(ip.min_step); dec |= STEP_DEC_CLAMP; }
+      if (val(hn) > ip.max_step) { hn = H(ip.max_step); dec |= STEP_DEC_CLAMP; }
       h = hn;
+    } else {
+      dec |= STEP_DEC_ERR_ZERO;
     }
+    if (info) { info->accepted = true; info->reflected = refl; info->decisions = dec; }
   } else {
-    double hn = h * 0.9 * std::pow(thresh / err, 0.25);                   // :324
-    h = std::fmax(hn, ip.min_step);
+    H hn = h * 0.9 * pow(thresh / err, 0.25);                              // :324
+    if (info && val(hn) < ip.min_step) info->decisions = STEP_DEC_CLAMP;
+    h = fmax_t(hn, H(ip.min_step));
   }
   return STEP_CONTINUE;
 }
