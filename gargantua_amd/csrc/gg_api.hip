@@ -104,9 +104,10 @@ struct gg_context {
   // gg_grad: cls, intensity, accepted, target, resid, dI, list, counters,
   // results, scene, checkpoint pool, segment buffers, keys, sorted list,
   // sorted keys, ray records, last-segment records, sort scratch,
-  DevBuf grad[21];
+  DevBuf grad[22];
   // record pool, last chunk per pixel and final states of the recording
-  // render (grad[18..20]); chunks the last recording needed
+  // render (grad[18..20]), attempts per pixel (grad[21]); chunks the last
+  // recording needed
   uint64_t grad_chunks = 0;
   uint64_t grad_budget = 0;   // gg_set_grad_memory (0: a quarter of the free memory)
   // gg_set_sky_hdri: equirectangular environment map for rgb outputs
@@ -1282,6 +1283,7 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
   if (checkpoint_every > 65536) return fail(GG_ERR_INVALID_ARG, "checkpoint_every must be in [0, 65536]");
   if (scene->precision != GG_PREC_FP64 && scene->precision != GG_PREC_MIXED)
     return fail(GG_ERR_INVALID_ARG, "gg_grad_scene.precision must be GG_PREC_FP64 or GG_PREC_MIXED");
+  if (scene->frozen_steps > 1) return fail(GG_ERR_INVALID_ARG, "gg_grad_scene.frozen_steps must be 0 or 1");
   const bool mixed = scene->precision == GG_PREC_MIXED;
   if (mixed && checkpoint_every != 0)
     return fail(GG_ERR_UNSUPPORTED, "the mixed forward render needs checkpoint_every = 0 (recorded steps): its FP32 "
@@ -1296,22 +1298,24 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
       (rc = B[3].ensure(8 * n)) || (rc = B[4].ensure(8 * n)) || (rc = B[5].ensure(32 * n)) ||
       (rc = B[6].ensure(4 * n)) || (rc = B[7].ensure(sizeof(GradCounters))) || (rc = B[8].ensure(64 + grad_reduce_scratch_bytes())) ||
       (rc = B[9].ensure(sizeof(DevScene))) || (rc = B[12].ensure(4 * n)) || (rc = B[13].ensure(4 * n)) ||
-      (rc = B[14].ensure(4 * n)))
+      (rc = B[14].ensure(4 * n)) || (rc = B[21].ensure(4 * n)))
     return rc;
   uint8_t *d_cls = (uint8_t *)B[0].p;
   double *d_int = (double *)B[1].p, *d_target = (double *)B[3].p, *d_resid = (double *)B[4].p;
   double *d_dI = (double *)B[5].p, *d_out5 = (double *)B[8].p;
   uint32_t *d_acc = (uint32_t *)B[2].p, *d_list_in = (uint32_t *)B[6].p, *d_keys_in = (uint32_t *)B[12].p;
+  uint32_t *d_att = (uint32_t *)B[21].p;
   uint32_t *d_list = (uint32_t *)B[13].p, *d_keys = (uint32_t *)B[14].p;
   GradCounters *d_cnt = (GradCounters *)B[7].p;
   hipStream_t s = ctx->stream;
 
   // Modes.  2 (checkpoint_every = 0, when the records fit): the render
-  // records every accepted step and the reverse sweep reads those records.
+  // records every attempt and the reverse sweep reads those records.
   // 1: a re-trace of the disk rays stores every step record (checkpoint_every
   // at least the longest ray's accepted count, or automatic when the render's
   // records did not fit but the disk rays' do).  0: the re-trace stores a
-  // checkpoint every K steps and the reverse sweep recomputes segments.
+  // checkpoint every K accepted steps and the reverse sweep recomputes
+  // segments.
   // Budget of the automatic choice: gg_set_grad_memory, else a quarter of the
   // free device memory, counting the record pool this context already holds.
   const bool automatic = (checkpoint_every == 0);
@@ -1341,13 +1345,14 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
       }
     }
     HIP_TRY(launch_grad_reset(d_cnt, s));
-    // the FP64 render: classes, intensities, accepted counts (and, recording,
-    // the step records and final states)
+    // the FP64 render: classes, intensities, attempt and accepted counts
+    // (and, recording, the step records and final states)
     gg_outputs o;
     std::memset(&o, 0, sizeof(o));
     o.cls = d_cls;
     o.intensity = d_int;
     o.accepted = d_acc;
+    o.attempts = d_att;
     if (record) {
       o.state = (double *)B[20].p;
       ctx->rec_req = {(double *)B[18].p, pool_chunks, &d_cnt->rec_cursor, (uint32_t *)B[19].p};
@@ -1362,7 +1367,7 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
     build_dev_scene(sc, D);
     HIP_TRY(hipMemcpyAsync(B[9].p, &D, sizeof(D), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_target, target, 8 * n, hipMemcpyDefault, s));
-    GradSelect sel{n, d_cls, d_int, d_target, d_acc, K, d_resid, d_dI, d_list_in, d_keys_in, d_cnt};
+    GradSelect sel{n, d_cls, d_int, d_target, d_acc, d_att, K, d_resid, d_dI, d_list_in, d_keys_in, d_cnt};
     HIP_TRY(launch_grad_select(sel, s));
     HIP_TRY(hipMemcpyAsync(&cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -1390,12 +1395,16 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
     tail_total = 0;
   } else if (K >= cnt.max_acc) {
     mode = 1;
-  } else if (automatic && 8 * GRAD_STEP_REC * cnt.acc_total + 72 * n_list <= budget) {
+  } else if (automatic && 8 * GRAD_STEP_REC * cnt.att_total + 72 * n_list <= budget) {
     K = (uint32_t)cnt.max_acc;
     chk_total = n_list;
-    tail_total = cnt.acc_total;
+    tail_total = cnt.att_total;
     mode = 1;
   }
+  // a recomputed segment holds K accepted steps and at most every attempt
+  // its ray did not accept
+  const uint64_t seg_cap = (uint64_t)K + cnt.max_rej;
+  if (mode == 0 && seg_cap > 0xFFFFFFFFull) return fail(GG_ERR_INVALID_ARG, "segment buffer too large");
 
   // launch shapes: forward 256-thread blocks, 2 per CU; reverse 64-thread
   // blocks, 4 per CU when it recomputes segments (its LDS and registers allow
@@ -1410,7 +1419,7 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
   const uint64_t n_threads = grid_r * 64;
   const uint64_t chk_bytes = 72 * (chk_total ? chk_total : 1);
   const uint64_t tail_bytes = 8 * GRAD_STEP_REC * (tail_total ? tail_total : 1);
-  const uint64_t seg_bytes = mode == 0 ? 8 * GRAD_STEP_REC * (uint64_t)K * n_threads : 0;
+  const uint64_t seg_bytes = mode == 0 ? 8 * GRAD_STEP_REC * seg_cap * n_threads : 0;
   if ((rc = B[10].ensure(chk_bytes)) || (rc = B[11].ensure(seg_bytes ? seg_bytes : 1)) ||
       (rc = B[15].ensure(8 * GRAD_REC * (n_list + 1))) || (rc = B[16].ensure(tail_bytes)))
     return rc;
@@ -1426,6 +1435,7 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
     G.scene = (const DevScene *)B[9].p;
     G.list = d_list;
     G.keys = d_keys;
+    G.accepted = d_acc;
     G.n_list = n_list;
     G.rec = (double *)B[15].p;
     G.chk = (double *)B[10].p;
@@ -1433,6 +1443,8 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
     G.tail = (double *)B[16].p;
     G.tail_cap = tail_total;
     G.seg = (double *)B[11].p;
+    G.seg_cap = mode == 0 ? (uint32_t)seg_cap : 0u;
+    G.live = scene->frozen_steps ? 0u : 1u;
     G.K = K;
     G.width = scene->width;
     G.height = scene->height;
@@ -1463,15 +1475,15 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
   if (d_intensity) HIP_TRY(hipMemcpyAsync(d_intensity, d_dI, 32 * n, hipMemcpyDefault, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (n_list > 0) HIP_TRY(hipEventElapsedTime(&adj_ms, ctx->ev0, ctx->ev1));
-  // every disk ray finished, and every accepted step of each was
-  // back-propagated exactly once
-  if (cnt.bad != 0 || cnt.done != n_list || cnt.steps != cnt.acc_total || cnt.chk_cursor != chk_total ||
+  // every disk ray finished, and every attempt of each was back-propagated
+  // exactly once
+  if (cnt.bad != 0 || cnt.done != n_list || cnt.steps != cnt.att_total || cnt.chk_cursor != chk_total ||
       cnt.tail_cursor != tail_total)
     return fail(GG_ERR_INTERNAL,
                 "adjoint: %llu of %llu rays done, %llu left the rendered path, %llu of %llu steps, %llu/%llu "
                 "checkpoints, %llu/%llu step records",
                 (unsigned long long)cnt.done, (unsigned long long)n_list, (unsigned long long)cnt.bad,
-                (unsigned long long)cnt.steps, (unsigned long long)cnt.acc_total,
+                (unsigned long long)cnt.steps, (unsigned long long)cnt.att_total,
                 (unsigned long long)cnt.chk_cursor, (unsigned long long)chk_total,
                 (unsigned long long)cnt.tail_cursor, (unsigned long long)tail_total);
   if (loss) *loss = out5[0];

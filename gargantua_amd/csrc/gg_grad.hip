@@ -1,30 +1,36 @@
 // gg_grad.hip -- hand-written reverse-mode adjoint of the thin-disk render.
 //
-// d(loss)/d(spin, inclination, q, r_in) for L = mean_p (I_p - target_p)^2,
-// differentiating the discrete FP64 algorithm with every ray's step-size
-// sequence, accept/reject pattern and disk-crossing step held at the values
-// of the forward render (discretise-then-differentiate).
+// d(loss)/d(spin, inclination, q, r_in) for L = mean_p (I_p - target_p)^2.
+// By default the step sizes carry derivatives: each attempt's h is the
+// controller's function of the previous h and error estimate
+// (geodesic.rs:308-325), so the gradient is that of the loss the render
+// computes; the discrete decisions (accept/reject, clamps, err == 0, the
+// disk-crossing step) are those of the forward render.  With frozen_steps
+// the step sizes are held at the render's values instead.
 //
-// The disk-hitting rays, sorted by accepted-step count (longest first, so
-// the long photon-ring rays start early and neighbouring lanes stay in the
-// same phase), go through three kernels, one lane per ray:
+// The disk-hitting rays, sorted by attempt count (longest first, so the
+// long photon-ring rays start early and neighbouring lanes stay in the same
+// phase), go through these kernels, one lane per ray:
 //   adjoint_fwd_kernel  re-traces the ray from the camera with the trace
 //       kernel's own attempt() (bit-identical steps), storing a checkpoint
 //       (pos, vel, h) every K accepted steps in a pool sized exactly from
-//       the render's accepted counts, and the start state and h of each step
-//       of the ray's last segment; at the render's accepted count the step is
-//       the disk crossing, whose intensity is differentiated in forward mode
-//       (12 inputs): the seeds of the reverse sweep.
-//   adjoint_rev_kernel  back-propagates one accepted step per iteration
-//       through its Dormand-Prince stages (the six k_i are recomputed, their
-//       cotangents live in LDS), segment by segment from the last; before
-//       each earlier segment it re-runs that segment from its checkpoint into
-//       a per-lane segment buffer (rejections included, so h repeats exactly,
-//       and the end state must equal the next checkpoint bit for bit).
-//       When K is at least every ray's accepted count there is one segment
-//       per ray, the re-trace stores every step record, and
-//       adjoint_rev_stored_kernel (the same sweep without recomputation,
-//       two wavefronts per SIMD) reads them.
+//       the render's accepted counts, and the step record of every attempt
+//       of the ray's last segment; at the render's accepted count the step
+//       is the disk crossing, whose intensity is differentiated in forward
+//       mode (12 inputs): the seeds of the reverse sweep.
+//   adjoint_rev_kernel  back-propagates one attempt per iteration through
+//       its controller, error estimate and Dormand-Prince stages (the RHS
+//       vector-Jacobian products at the recorded stage points; stage
+//       cotangents in LDS), segment by segment from the last; before each
+//       earlier segment it re-runs that segment from its checkpoint into a
+//       per-lane segment buffer (so h repeats exactly, and the end state
+//       must equal the next checkpoint bit for bit).  When K is at least
+//       every ray's accepted count there is one segment per ray, the
+//       re-trace stores every step record, and adjoint_rev_stored_kernel
+//       (the same sweep without recomputation, two wavefronts per SIMD)
+//       reads them.  adjoint_rev_chunked_kernel reads the records the
+//       recording render wrote instead (adjoint_seed_kernel computes the
+//       crossing seeds from them).
 //   adjoint_cam_kernel  adds the camera ray's dependence on (a, i) and
 //       writes dI/d(a, i, q, r_in) per pixel.
 #include <hip/hip_runtime.h>
@@ -42,35 +48,144 @@ enum { PH_IDLE = 0, PH_FWD = 1, PH_REV = 2, PH_RECOMP = 3 };
 
 static_assert(STEP_REC == GRAD_STEP_REC, "step record layout");
 
-// Adjoint of one accepted Kerr step (p, v) -> (p + h v normalised,
-// v + h sum_i c_i k_i), k_i = a(p + h a_i v, v_i), from its step record
-// (p, v, v_2..v_6, h).  On entry (pb_r, pb_th, vb) is the cotangent of the
-// step's end state (t and phi carry none: nothing depends on them); on exit
-// that of its start.  ga accumulates the spin cotangent.  LDS slots 0..5
-// hold the stage cotangents kbar_1..kbar_6.
-__device__ inline void step_adjoint(const KStore &K, const double *rec_in, double m, double a, double &pb_r,
-                                    double &pb_th, d4 &vb, double &ga) {
+// Reverse-sweep state of one lane between two step records.
+struct Adj {
+  double pr, pth;   // cotangent of the current r, theta (t and phi carry none: nothing depends on them)
+  d4 vb;            // ... of the current velocity
+  double hb;        // ... of the h the next attempt takes (the controller's output)
+  double ga;        // spin cotangent
+  d4 gv;            // start velocity of the accepted attempt after the current record
+  double hn;        // h of the next attempt
+};
+
+// Adjoint of one recorded attempt (geodesic.rs:185-328), whose record holds
+// its start (r, theta), h, kind, a velocity, the stage velocities v_2..v_6
+// and the error difference new_vel - vel_err.  On entry A holds the
+// cotangents of the attempt's outputs (state and next h); on exit those of
+// its inputs.  LDS slots 0..5 hold the stage cotangents kbar_1..kbar_6.
+//
+// Accepted: pos' = p + h v (normalised), v' = v + h sum_i c_i k_i,
+//   k_i = a(p + h a_i v, v_i).  Rejected: p, v pass through.  Invalid: h/2.
+// live: the next h is the controller's function of h and the error
+//   estimate, h' = clamp(0.9 h err^-0.2) after an accept (h itself when err
+//   is 0, geodesic.rs:318) and max(0.9 h err^-0.25, min_step) after a
+//   reject; where a clamp binds h' carries no derivative.  The error's
+//   cotangent goes through new_vel - vel_err, the rtol/atol scale or the
+//   Euclidean norm, and k7 = a(p + h v, new_vel).  h enters the stage
+//   positions (a_i v each) and the stage combinations, all of the form
+//   v + h sum_j w_j k_j, whose h-derivative adds kbar_j . k_j / h; since a is
+//   quadratic in its velocity, kbar . a(x, u) = (J_u^T kbar) . u / 2, the
+//   VJP's own velocity output dotted with u.
+// !live: the step sizes are held fixed; rejected and invalid attempts then
+//   move nothing.
+__device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const double *rec_in, bool live, Adj &A) {
   GG_CONTRACT_FAST
   const double *rec = (const double *)__builtin_assume_aligned(rec_in, 32);
-  const double r0 = rec[1], th0 = rec[2];
-  const d4 v{rec[4], rec[5], rec[6], rec[7]};
   const double h = rec[REC_H];
-  // normalize_spherical_coords (geodesic.rs:337-352): theta -> -theta or
-  // 2 pi - theta flips the sign of its cotangent
-  double th1 = h * v.z + th0;
-  double sg = 1.0;
-  if (th1 < 0.0) { th1 = -th1; sg = -sg; }
-  if (th1 > PI) sg = -sg;
-  pb_th *= sg;
-  // new_vel = v + h sum_i c_i k_i: kbar_i = h c_i vb; pos' = p + h v
-  const d4 vbp = vb;
+  const int kind = (int)rec[REC_KIND];
+  const double hn = A.hn;
+  A.hn = h;
+  if (kind == REC_INVALID) { A.hb *= 0.5; return; }   // geodesic.rs:250-291: h halved
+  const bool acc = (kind == REC_ACCEPT);
+  if (!acc && !live) return;
+  const double m = S.mass, a = S.spin;
+  const double r0 = rec[REC_R], th0 = rec[REC_TH];
+  const d4 rv{rec[REC_V], rec[REC_V + 1], rec[REC_V + 2], rec[REC_V + 3]};
+  const d4 v = acc ? rv : A.gv;
+  const d4 nv = acc ? A.gv : rv;
+
+  // ---- controller (geodesic.rs:318-325): cotangents of h and of the error ----
+  double hbar = 0.0, eb = 0.0;
+  const d4 dv{rec[REC_DV], rec[REC_DV + 1], rec[REC_DV + 2], rec[REC_DV + 3]};
+  double err = 0.0;
+  if (live && A.hb != 0.0) {
+    if (S.err_mode == 1) {
+      const double q0 = dv.x / (S.atol + S.rtol * fmax(fabs(v.x), fabs(nv.x)));
+      const double q1 = dv.y / (S.atol + S.rtol * fmax(fabs(v.y), fabs(nv.y)));
+      const double q2 = dv.z / (S.atol + S.rtol * fmax(fabs(v.z), fabs(nv.z)));
+      const double q3 = dv.w / (S.atol + S.rtol * fmax(fabs(v.w), fabs(nv.w)));
+      err = sqrt(0.25 * (q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3));
+    } else {
+      err = sqrt(dv.x * dv.x + dv.y * dv.y + dv.z * dv.z + dv.w * dv.w);
+    }
+    if (acc) {
+      if (err == 0.0) {
+        hbar = A.hb;                                   // h unchanged
+      } else if (hn != S.max_step && hn != S.min_step) {
+        hbar = A.hb * hn / h;                          // h' = 0.9 h err^-0.2
+        eb = -0.2 * A.hb * hn / err;
+      }
+    } else if (hn != S.min_step) {
+      hbar = A.hb * hn / h;                            // h' = 0.9 h err^-0.25
+      eb = -0.25 * A.hb * hn / err;
+    }
+  }
+  if (!acc && eb == 0.0) { A.hb = hbar; return; }
+
+  double pr = A.pr, pth = A.pth;
+  d4 vbar = acc ? d4{0.0, 0.0, 0.0, 0.0} : A.vb;       // a rejection passes p, v through
+  d4 nvb{0.0, 0.0, 0.0, 0.0}, veb{0.0, 0.0, 0.0, 0.0}; // cotangents of new_vel, vel_err
+  if (acc) {
+    // normalize_spherical_coords (geodesic.rs:337-352): theta -> -theta or
+    // 2 pi - theta flips the sign of its cotangent
+    double th1 = h * v.z + th0;
+    double sg = 1.0;
+    if (th1 < 0.0) { th1 = -th1; sg = -sg; }
+    if (th1 > PI) sg = -sg;
+    pth *= sg;
+    // pos' = p + h v (geodesic.rs:312); v' = new_vel
+    vbar = d4{0.0, h * pr, h * pth, 0.0};
+    hbar += pr * v.y + pth * v.z;
+    nvb = A.vb;
+  }
+  if (eb != 0.0) {
+    // err = sqrt(mean (d_c / s_c)^2), s_c = atol + rtol max(|v_c|, |new_vel_c|);
+    // or the Euclidean norm of d = new_vel - vel_err (geodesic.rs:308)
+    d4 db;
+    if (S.err_mode == 1) {
+      const double dd[4] = {dv.x, dv.y, dv.z, dv.w};
+      const double vv[4] = {v.x, v.y, v.z, v.w}, nn[4] = {nv.x, nv.y, nv.z, nv.w};
+      double o[4], vs_[4] = {0.0, 0.0, 0.0, 0.0}, ns_[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const bool use_v = fabs(vv[c]) >= fabs(nn[c]);
+        const double sc = S.atol + S.rtol * (use_v ? fabs(vv[c]) : fabs(nn[c]));
+        const double q = dd[c] / sc;
+        const double qb = 0.25 * eb * q / err;
+        o[c] = qb / sc;
+        const double scb = -qb * q / sc * S.rtol;     // through max(|v_c|, |new_vel_c|)
+        if (use_v) vs_[c] = vv[c] > 0.0 ? scb : (vv[c] < 0.0 ? -scb : 0.0);
+        else ns_[c] = nn[c] > 0.0 ? scb : (nn[c] < 0.0 ? -scb : 0.0);
+      }
+      db = d4{o[0], o[1], o[2], o[3]};
+      vbar.x += vs_[0]; vbar.y += vs_[1]; vbar.z += vs_[2]; vbar.w += vs_[3];
+      nvb.x += ns_[0]; nvb.y += ns_[1]; nvb.z += ns_[2]; nvb.w += ns_[3];
+    } else {
+      const double k = eb / err;
+      db = d4{k * dv.x, k * dv.y, k * dv.z, k * dv.w};
+    }
+    nvb.x += db.x; nvb.y += db.y; nvb.z += db.z; nvb.w += db.w;
+    veb = d4{-db.x, -db.y, -db.z, -db.w};
+    // k7 = a(p + h v, new_vel) (geodesic.rs:298), weight h d7 in vel_err
+    const double hd7 = h * D7;
+    const RhsBar jb = kerr_rhs_vjp(m, a, h * v.y + r0, h * v.z + th0, nv,
+                                   d4{hd7 * veb.x, hd7 * veb.y, hd7 * veb.z, hd7 * veb.w});
+    A.ga += jb.a;
+    pr += jb.r;
+    pth += jb.th;
+    vbar.y += h * jb.r;
+    vbar.z += h * jb.th;
+    hbar += jb.r * v.y + jb.th * v.z + 0.5 * (jb.v.x * nv.x + jb.v.y * nv.y + jb.v.z * nv.z + jb.v.w * nv.w) / h;
+    nvb.x += jb.v.x; nvb.y += jb.v.y; nvb.z += jb.v.z; nvb.w += jb.v.w;
+  }
+  // new_vel = v + h sum_i c_i k_i, vel_err = v + h sum_i d_i k_i
+  vbar.x += nvb.x + veb.x; vbar.y += nvb.y + veb.y; vbar.z += nvb.z + veb.z; vbar.w += nvb.w + veb.w;
 #pragma unroll 1
   for (int i = 1; i <= 6; ++i) {
-    double hc = h * kC[i];
-    K.put(i - 1, d4{hc * vbp.x, hc * vbp.y, hc * vbp.z, hc * vbp.w});
+    const double hc = h * kC[i], hd = h * kD[i];
+    K.put(i - 1, d4{hc * nvb.x + hd * veb.x, hc * nvb.y + hd * veb.y, hc * nvb.z + hd * veb.z,
+                    hc * nvb.w + hd * veb.w});
   }
-  double pr = pb_r, pth = pb_th;
-  d4 vbar{vbp.x, vbp.y + h * pb_r, vbp.z + h * pb_th, vbp.w};
 #pragma unroll 1
   for (int i = 6; i >= 1; --i) {
     const d4 w = K.get(i - 1);
@@ -86,13 +201,15 @@ __device__ inline void step_adjoint(const KStore &K, const double *rec_in, doubl
       vs = d4{o[0], o[1], o[2], o[3]};
     }
     const RhsBar jb = kerr_rhs_vjp(m, a, qr, qth, vs, w);
-    ga += jb.a;
+    A.ga += jb.a;
     pr += jb.r;
     pth += jb.th;
     vbar.x += jb.v.x; vbar.y += jb.v.y; vbar.z += jb.v.z; vbar.w += jb.v.w;
+    if (live) hbar += 0.5 * (jb.v.x * vs.x + jb.v.y * vs.y + jb.v.z * vs.z + jb.v.w * vs.w) / h;
     if (i >= 2) {
       vbar.y += ha * jb.r;
       vbar.z += ha * jb.th;
+      if (live) hbar += kA[i] * (jb.r * v.y + jb.th * v.z);
 #pragma unroll 1
       for (int j = 1; j < i; ++j) {
         double hb = h * kB[i][j];
@@ -102,9 +219,11 @@ __device__ inline void step_adjoint(const KStore &K, const double *rec_in, doubl
       }
     }
   }
-  pb_r = pr;
-  pb_th = pth;
-  vb = vbar;
+  A.pr = pr;
+  A.pth = pth;
+  A.vb = vbar;
+  A.hb = live ? hbar : 0.0;
+  if (acc) A.gv = v;
 }
 
 __device__ inline void put_state(double *dst, const double p[4], d4 v, double h) {
@@ -124,7 +243,9 @@ __device__ inline void get_state(const double *src, double p[4], d4 &v, double &
 enum {
   R_PBR = 0, R_PBTH, R_VB0, R_VB1, R_VB2, R_VB3, R_GA, R_GI, R_GQ, R_GRIN,   // cotangents
   R_XR0, R_XTH0, R_XV0T, R_XV0P,                                            // crossing's direct terms
-  R_CHK, R_TAIL,                                                            // pool offsets (bits)
+  R_CHK, R_TAIL, R_NTAIL,                                                   // pool offsets, tail records (bits)
+  R_GV0, R_GV1, R_GV2, R_GV3,                                               // velocity after the crossing step
+  R_PAD0, R_PAD1, R_PAD2,
   REC
 };
 static_assert(REC == GRAD_REC, "record layout");
@@ -195,6 +316,10 @@ __device__ inline bool crossing_seeds(const GradLaunch &G, const DevScene &S, do
   return true;
 }
 
+__device__ inline void put_gv(double *rec, d4 v) {
+  rec[R_GV0] = v.x; rec[R_GV1] = v.y; rec[R_GV2] = v.z; rec[R_GV3] = v.w;
+}
+
 __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
 #pragma clang fp contract(off)
   extern __shared__ double lds_k[];
@@ -206,18 +331,22 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
   L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0;
   bool active = false, exhausted = false;
   uint64_t q = 0, chk_base = 0, tail_base = 0;
-  uint32_t pix = 0, acc_target = 0, last0 = 0, att = 0;
+  uint32_t pix = 0, acc_target = 0, att_target = 0, last0 = 0, att = 0, tail_cap = 0, seg_att = 0;
   unsigned long long n_att = 0, n_bad = 0;
   for (;;) {
     const uint64_t t = take(&G.cnt->queue, G.n_list, !active, exhausted);
     if (t != ~0ull) {
       q = t;
       pix = G.list[q];
-      acc_target = G.keys[q];
+      att_target = G.keys[q];
+      acc_target = G.accepted[pix];
       const uint32_t nchk = (acc_target + KK - 1) / KK;
-      last0 = (nchk - 1) * KK;
+      last0 = nchk ? (nchk - 1) * KK : 0u;
+      // the last segment's attempts: its accepted steps plus at most the
+      // ray's rejected and invalid attempts
+      tail_cap = att_target - last0;
       chk_base = atomicAdd(&G.cnt->chk_cursor, (unsigned long long)nchk);
-      tail_base = atomicAdd(&G.cnt->tail_cursor, (unsigned long long)(acc_target - last0));
+      tail_base = atomicAdd(&G.cnt->tail_cursor, (unsigned long long)tail_cap);
       camera_ray(S, pix % G.width, pix / G.width, L.pos, L.vel);
       L.lambda = 0.0;
       L.h = S.h0;
@@ -226,8 +355,10 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
       L.err_prev = 1.0;
       L.b2 = 0.0;
       att = 0;
+      seg_att = 0;
       active = true;
-      if (acc_target == 0 || chk_base + nchk > G.chk_cap || tail_base + (acc_target - last0) > G.tail_cap) {
+      if (acc_target == 0 || att_target < acc_target || chk_base + nchk > G.chk_cap ||
+          tail_base + tail_cap > G.tail_cap) {
         mark_bad(G, q, pix, n_bad);
         active = false;
       } else {
@@ -240,9 +371,16 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
     bool acc;
     double p0[4];
     d4 v0;
-    double *recp = L.accepted >= last0 ? G.tail + STEP_REC * (tail_base + (L.accepted - last0)) : nullptr;
+    const bool in_tail = L.accepted >= last0;
+    if (in_tail && seg_att >= tail_cap) {   // more attempts than the render took
+      mark_bad(G, q, pix, n_bad);
+      active = false;
+      continue;
+    }
+    double *recp = in_tail ? G.tail + STEP_REC * (tail_base + seg_att) : nullptr;
     const int res = attempt<1, 0, true>(S, L, K, acc, p0, v0, false, false, recp);
     ++n_att;
+    ++seg_att;
     if (res != RES_CONTINUE || ++att > S.max_steps) {   // the render reached the crossing; a re-trace must too
       mark_bad(G, q, pix, n_bad);
       active = false;
@@ -251,18 +389,24 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
     if (!acc) continue;
     L.accepted++;
     if (L.accepted < acc_target) {
-      if (L.accepted % KK == 0) put_state(G.chk + 9 * (chk_base + L.accepted / KK), L.pos, L.vel, L.h);
+      if (L.accepted % KK == 0) {
+        put_state(G.chk + 9 * (chk_base + L.accepted / KK), L.pos, L.vel, L.h);
+        seg_att = 0;
+      }
       continue;
     }
     // the disk crossing of the last accepted step (same test as the trace kernel)
     active = false;
     double *rec = G.rec + REC * q;
-    if (!crossing_seeds(G, S, rec, p0[1], p0[2], v0.x, v0.w, L.pos[1], L.pos[2], L.vel.x, L.vel.w)) {
+    if (att != att_target ||
+        !crossing_seeds(G, S, rec, p0[1], p0[2], v0.x, v0.w, L.pos[1], L.pos[2], L.vel.x, L.vel.w)) {
       mark_bad(G, q, pix, n_bad);
       continue;
     }
+    put_gv(rec, L.vel);
     rec[R_CHK] = as_f64(chk_base);
     rec[R_TAIL] = as_f64(tail_base);
+    rec[R_NTAIL] = as_f64(seg_att);
   }
   unsigned long long a = wave_sum(n_att), bd = wave_sum(n_bad), r = wave_sum(L.n_rhs);
   if (lane == 0) {
@@ -272,12 +416,13 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
   }
 }
 
-// The reverse sweep.  MODE 0: some rays have more than one segment, so the
-// earlier segments are recomputed from their checkpoints.  MODE 1: every
-// step record of every ray is in the last-segment pool (one segment per
-// ray).  MODE 2: the records are the recording render's chunks, walked back
-// through their back-pointers.  Modes 1 and 2 only read records (fewer live
-// registers: two wavefronts per SIMD).
+// The reverse sweep, one step record (attempt) per iteration, last first.
+// MODE 0: some rays have more than one segment, so the earlier segments are
+// recomputed from their checkpoints.  MODE 1: every step record of every ray
+// is in the last-segment pool (one segment per ray).  MODE 2: the records are
+// the recording render's chunks, walked back through their back-pointers.
+// Modes 1 and 2 only read records (fewer live registers: two wavefronts per
+// SIMD).
 template <int MODE>
 __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
 #pragma clang fp contract(off)
@@ -288,7 +433,8 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const DevScene &S = *G.scene;
   const uint32_t KK = G.K;
-  double *const seg = RECOMPUTE ? G.seg + gtid * (uint64_t)KK * STEP_REC : nullptr;
+  const bool live = G.live != 0;
+  double *const seg = RECOMPUTE ? G.seg + gtid * (uint64_t)G.seg_cap * STEP_REC : nullptr;
 
   Lane L;
   L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0;
@@ -296,10 +442,10 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
   int phase = IDLE;
   bool exhausted = false, first = false;
   uint64_t q = 0, chk_base = 0;
-  uint32_t pix = 0, c = 0, j = 0, att = 0, chunk = 0;
+  uint32_t pix = 0, c = 0, j = 0, att = 0, chunk = 0, seg_j = 0;
   const double *src = nullptr;
-  double pb_r = 0, pb_th = 0, ga = 0;
-  d4 vb{0, 0, 0, 0};
+  Adj A;
+  A.pr = 0; A.pth = 0; A.vb = d4{0, 0, 0, 0}; A.hb = 0; A.ga = 0; A.gv = d4{0, 0, 0, 0}; A.hn = 0;
   unsigned long long n_att = 0, n_adj = 0, n_done = 0, n_bad = 0;
 
   for (;;) {
@@ -307,24 +453,28 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
     if (t != ~0ull) {
       q = t;
       pix = G.list[q];
-      const uint32_t acc_target = G.keys[q];
       const double *rec = G.rec + REC * q;
       if (!(rec[R_PBR] == rec[R_PBR])) {   // flagged by the forward pass
         phase = IDLE;
       } else {
         // without recomputation the whole ray is one segment
+        const uint32_t acc_target = G.accepted[pix];
         const uint32_t nchk = RECOMPUTE ? (acc_target + KK - 1) / KK : 1;
         c = nchk - 1;
-        j = acc_target - c * KK - 1;
         if (MODE == 2) {
           chunk = (uint32_t)as_u64(rec[R_CHK]);
+          j = G.keys[q] - 1;
         } else {
           chk_base = as_u64(rec[R_CHK]);
           src = G.tail + STEP_REC * as_u64(rec[R_TAIL]);
+          j = (uint32_t)as_u64(rec[R_NTAIL]) - 1;
         }
-        pb_r = rec[R_PBR]; pb_th = rec[R_PBTH];
-        vb = d4{rec[R_VB0], rec[R_VB1], rec[R_VB2], rec[R_VB3]};
-        ga = rec[R_GA];
+        A.pr = rec[R_PBR]; A.pth = rec[R_PBTH];
+        A.vb = d4{rec[R_VB0], rec[R_VB1], rec[R_VB2], rec[R_VB3]};
+        A.ga = rec[R_GA];
+        A.gv = d4{rec[R_GV0], rec[R_GV1], rec[R_GV2], rec[R_GV3]};
+        A.hb = 0.0;       // the crossing ends the ray: its next h enters nothing
+        A.hn = 0.0;
         first = true;
         phase = REV;
       }
@@ -333,44 +483,50 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
 
     // ---- RECOMP: one attempt of the segment before ----
     if (RECOMPUTE && phase == RECOMP) {
-      bool acc;
-      double p0[4];
-      d4 v0;
-      double *recp = seg + STEP_REC * (uint64_t)(L.accepted - c * KK);
-      const int res = attempt<1, 0, true>(S, L, K, acc, p0, v0, false, false, recp);
-      ++n_att;
-      if (res != RES_CONTINUE || ++att > S.max_steps) {
+      if (seg_j >= G.seg_cap) {
         mark_bad(G, q, pix, n_bad);
         phase = IDLE;
-      } else if (acc) {
-        L.accepted++;
-        if (L.accepted == (c + 1) * KK) {
-          // the recomputed segment must land bit for bit on the next checkpoint
-          double qp[4], hq;
-          d4 vq;
-          get_state(G.chk + 9 * (chk_base + c + 1), qp, vq, hq);
-          if (qp[0] != L.pos[0] || qp[1] != L.pos[1] || qp[2] != L.pos[2] || qp[3] != L.pos[3] ||
-              vq.x != L.vel.x || vq.y != L.vel.y || vq.z != L.vel.z || vq.w != L.vel.w || hq != L.h) {
-            mark_bad(G, q, pix, n_bad);
-            phase = IDLE;
-          } else {
-            j = KK - 1;
-            src = seg;
-            phase = REV;
+      } else {
+        bool acc;
+        double p0[4];
+        d4 v0;
+        double *recp = seg + STEP_REC * (uint64_t)seg_j;
+        const int res = attempt<1, 0, true>(S, L, K, acc, p0, v0, false, false, recp);
+        ++n_att;
+        ++seg_j;
+        if (res != RES_CONTINUE || ++att > S.max_steps) {
+          mark_bad(G, q, pix, n_bad);
+          phase = IDLE;
+        } else if (acc) {
+          L.accepted++;
+          if (L.accepted == (c + 1) * KK) {
+            // the recomputed segment must land bit for bit on the next checkpoint
+            double qp[4], hq;
+            d4 vq;
+            get_state(G.chk + 9 * (chk_base + c + 1), qp, vq, hq);
+            if (qp[0] != L.pos[0] || qp[1] != L.pos[1] || qp[2] != L.pos[2] || qp[3] != L.pos[3] ||
+                vq.x != L.vel.x || vq.y != L.vel.y || vq.z != L.vel.z || vq.w != L.vel.w || hq != L.h) {
+              mark_bad(G, q, pix, n_bad);
+              phase = IDLE;
+            } else {
+              j = seg_j - 1;
+              src = seg;
+              phase = REV;
+            }
           }
         }
       }
     }
 
-    // ---- REV: one accepted step backwards ----
+    // ---- REV: one step record backwards ----
     if (phase == REV) {
       const double *sp = MODE == 2 ? G.pool + (uint64_t)chunk * GRAD_CHUNK_DOUBLES + GRAD_CHUNK_HEAD + STEP_REC * (j % GRAD_CHUNK)
                                    : src + STEP_REC * (uint64_t)j;
-      step_adjoint(K, sp, S.mass, S.spin, pb_r, pb_th, vb, ga);
+      attempt_adjoint(K, S, sp, live, A);
       ++n_adj;
       if (first) {
         const double *rec = G.rec + REC * q;
-        pb_r += rec[R_XR0]; pb_th += rec[R_XTH0]; vb.x += rec[R_XV0T]; vb.w += rec[R_XV0P];
+        A.pr += rec[R_XR0]; A.pth += rec[R_XTH0]; A.vb.x += rec[R_XV0T]; A.vb.w += rec[R_XV0P];
         first = false;
       }
       if (j > 0) {
@@ -382,12 +538,13 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
         L.k1_ok = false; L.k1_f32 = false; L.k1_fast = false;
         L.accepted = c * KK;
         att = 0;
+        seg_j = 0;
         phase = RECOMP;
       } else {
         double *rec = G.rec + REC * q;
-        rec[R_PBR] = pb_r; rec[R_PBTH] = pb_th;
-        rec[R_VB0] = vb.x; rec[R_VB1] = vb.y; rec[R_VB2] = vb.z; rec[R_VB3] = vb.w;
-        rec[R_GA] = ga;
+        rec[R_PBR] = A.pr; rec[R_PBTH] = A.pth;
+        rec[R_VB0] = A.vb.x; rec[R_VB1] = A.vb.y; rec[R_VB2] = A.vb.z; rec[R_VB3] = A.vb.w;
+        rec[R_GA] = A.ga;
         ++n_done;
         phase = IDLE;
       }
@@ -416,16 +573,18 @@ __global__ __launch_bounds__(256) void adjoint_seed_kernel(GradLaunch G) {
   unsigned long long n_bad = 0;
   if (q < G.n_list) {
     const uint32_t pix = G.list[q];
-    const uint32_t acc = G.keys[q];
+    const uint32_t att = G.keys[q];
     const uint32_t chunk = G.last_chunk[pix];
     double *rec = G.rec + REC * q;
-    bool ok = acc > 0 && chunk != 0xFFFFFFFFu && chunk < G.pool_chunks;
+    bool ok = att > 0 && chunk != 0xFFFFFFFFu && chunk < G.pool_chunks;
     if (ok) {
-      const double *r = G.pool + (uint64_t)chunk * GRAD_CHUNK_DOUBLES + GRAD_CHUNK_HEAD + STEP_REC * ((acc - 1) % GRAD_CHUNK);
+      const double *r = G.pool + (uint64_t)chunk * GRAD_CHUNK_DOUBLES + GRAD_CHUNK_HEAD + STEP_REC * ((att - 1) % GRAD_CHUNK);
       const uint64_t n = G.n_pix;
       const double *fs = G.final_state;
-      ok = crossing_seeds(G, *G.scene, rec, r[1], r[2], r[4], r[7], fs[n + pix], fs[2 * n + pix], fs[4 * n + pix],
-                          fs[7 * n + pix]);
+      ok = r[REC_KIND] == (double)REC_ACCEPT &&
+           crossing_seeds(G, *G.scene, rec, r[REC_R], r[REC_TH], r[REC_V], r[REC_V + 3], fs[n + pix],
+                          fs[2 * n + pix], fs[4 * n + pix], fs[7 * n + pix]);
+      put_gv(rec, d4{fs[4 * n + pix], fs[5 * n + pix], fs[6 * n + pix], fs[7 * n + pix]});
       rec[R_CHK] = as_f64(chunk);
     }
     if (!ok) mark_bad(G, q, pix, n_bad);
@@ -457,16 +616,17 @@ __global__ __launch_bounds__(256) void adjoint_cam_kernel(GradLaunch G) {
   o[3] = rec[R_GRIN];
 }
 
-// Residuals, zeroed gradients and the list of disk rays with their
-// accepted counts (sort keys), checkpoint and last-segment totals, the total
-// and the largest accepted count.  One atomic per block for the list and
-// each total (the list order does not matter: it is sorted next).
+// Residuals, zeroed gradients and the list of disk rays with their attempt
+// counts (sort keys: step records), checkpoint and last-segment totals, the
+// accepted and attempt totals, the largest accepted count and the most
+// attempts a ray did not accept.  One atomic per block for the list and each
+// total (the list order does not matter: it is sorted next).
 constexpr uint32_t SELECT_BLOCK = 1024;
 __global__ __launch_bounds__(SELECT_BLOCK) void select_kernel(GradSelect P) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   bool disk = false;
-  uint32_t nchk = 0, last = 0, acc = 0;
+  uint32_t nchk = 0, tail = 0, acc = 0, att = 0;
   if (i < P.n) {
     P.resid[i] = P.intensity[i] - P.target[i];
     double *o = P.dI + 4 * i;
@@ -474,25 +634,36 @@ __global__ __launch_bounds__(SELECT_BLOCK) void select_kernel(GradSelect P) {
     disk = P.cls[i] == GG_CLASS_DISK;
     if (disk) {
       acc = P.accepted[i];
+      att = P.attempts[i];
+      if (att < acc) att = acc;
       nchk = (acc + P.K - 1) / P.K;
-      last = nchk ? acc - (nchk - 1) * P.K : 0;
+      // the last segment holds its accepted steps and at most every attempt
+      // the ray did not accept
+      tail = att - (nchk ? (nchk - 1) * P.K : 0u);
     }
   }
   const unsigned long long m = __ballot(disk);
-  const unsigned long long tot = wave_sum(nchk), tl = wave_sum(last), all = wave_sum(acc);
-  uint32_t mx = acc;
-  for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off, 64));
+  const unsigned long long tot = wave_sum(nchk), tl = wave_sum(tail), all = wave_sum(acc), alt = wave_sum(att);
+  uint32_t mx = acc, mr = att - acc;
+  for (int off = 32; off > 0; off >>= 1) {
+    mx = max(mx, (uint32_t)__shfl_xor(mx, off, 64));
+    mr = max(mr, (uint32_t)__shfl_xor(mr, off, 64));
+  }
   constexpr uint32_t NW = SELECT_BLOCK / 64;
-  __shared__ unsigned long long s_cnt[NW], s_tot[NW], s_tl[NW], s_all[NW];
-  __shared__ uint32_t s_mx[NW];
+  __shared__ unsigned long long s_cnt[NW], s_tot[NW], s_tl[NW], s_all[NW], s_alt[NW];
+  __shared__ uint32_t s_mx[NW], s_mr[NW];
   __shared__ unsigned long long s_base;
-  if (lane == 0) { s_cnt[wave] = __popcll(m); s_tot[wave] = tot; s_tl[wave] = tl; s_all[wave] = all; s_mx[wave] = mx; }
+  if (lane == 0) {
+    s_cnt[wave] = __popcll(m); s_tot[wave] = tot; s_tl[wave] = tl; s_all[wave] = all; s_alt[wave] = alt;
+    s_mx[wave] = mx; s_mr[wave] = mr;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long c = 0, t = 0, l = 0, al = 0;
-    uint32_t x = 0;
+    unsigned long long c = 0, t = 0, l = 0, al = 0, at = 0;
+    uint32_t x = 0, xr = 0;
     for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
-      c += s_cnt[w]; t += s_tot[w]; l += s_tl[w]; al += s_all[w]; x = max(x, s_mx[w]);
+      c += s_cnt[w]; t += s_tot[w]; l += s_tl[w]; al += s_all[w]; at += s_alt[w];
+      x = max(x, s_mx[w]); xr = max(xr, s_mr[w]);
     }
     unsigned long long base = 0;
     if (c) {
@@ -500,7 +671,9 @@ __global__ __launch_bounds__(SELECT_BLOCK) void select_kernel(GradSelect P) {
       atomicAdd(&P.cnt->chk_total, t);
       atomicAdd(&P.cnt->tail_total, l);
       atomicAdd(&P.cnt->acc_total, al);
+      atomicAdd(&P.cnt->att_total, at);
       atomicMax(&P.cnt->max_acc, (unsigned long long)x);
+      atomicMax(&P.cnt->max_rej, (unsigned long long)xr);
     }
     s_base = base;
   }
@@ -509,7 +682,7 @@ __global__ __launch_bounds__(SELECT_BLOCK) void select_kernel(GradSelect P) {
     unsigned long long at = s_base + __popcll(m & ((1ull << lane) - 1ull));
     for (uint32_t w = 0; w < wave; ++w) at += s_cnt[w];
     P.list[at] = (uint32_t)i;
-    P.keys[at] = acc;
+    P.keys[at] = att;
   }
 }
 
@@ -613,7 +786,7 @@ hipError_t launch_adjoint_cam(const GradLaunch &g, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Disk rays by accepted count, longest first (radix sort of (count, pixel)).
+// Disk rays by attempt count, longest first (radix sort of (count, pixel)).
 hipError_t sort_rays_desc(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
                           uint64_t n, int key_bits, void *temp, size_t &temp_bytes, hipStream_t s) {
   return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out,

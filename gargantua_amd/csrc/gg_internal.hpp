@@ -41,10 +41,10 @@ struct TraceLaunch {
   Counters *counters;
   uint32_t refill_interval;
   const uint32_t *order;      // camera renders: queue position -> global ray, or null (ray order)
-  // Recording render of gg_grad (Kerr thin disk, FP64, one scene), else
-  // null: every accepted step's record in chunks (GRAD_CHUNK_DOUBLES each:
-  // the index of the ray's previous chunk in a 128-byte line of its own,
-  // then GRAD_CHUNK step records);
+  // Recording render of gg_grad (Kerr thin disk, one scene), else null:
+  // every attempt's step record in chunks (GRAD_CHUNK_DOUBLES each: the
+  // index of the ray's previous chunk in a 128-byte line of its own, then
+  // GRAD_CHUNK step records);
   // rec_last[output index] = the ray's last chunk.
   double *rec_pool;
   unsigned long long rec_cap;       // chunks in the pool
@@ -59,16 +59,18 @@ struct GradCounters {
   unsigned long long chk_cursor;   // checkpoints claimed in the pool
   unsigned long long tail_cursor;  // last-segment step records claimed
   unsigned long long attempts;     // attempts of re-traces and recomputations
-  unsigned long long steps;        // accepted steps back-propagated
+  unsigned long long steps;        // step records (attempts) back-propagated
   unsigned long long done;         // rays whose reverse sweep finished
   unsigned long long bad;          // rays whose re-trace left the rendered path
   unsigned long long rhs;          // RHS evaluations inside attempts
   unsigned long long n_list;       // disk rays found by select
   unsigned long long chk_total;    // checkpoints they need
-  unsigned long long tail_total;   // step records of their last segments
+  unsigned long long tail_total;   // step records their last segments can hold
   unsigned long long acc_total;    // accepted steps of all disk rays
   unsigned long long max_acc;      // the largest accepted count
   unsigned long long rec_cursor;   // chunks claimed by the recording render
+  unsigned long long att_total;    // attempts of all disk rays (their step records)
+  unsigned long long max_rej;      // the most attempts a disk ray did not accept
   unsigned long long pad[1];
 };
 
@@ -76,24 +78,27 @@ struct GradSelect {
   uint64_t n;                      // pixels
   const uint8_t *cls;
   const double *intensity, *target;
-  const uint32_t *accepted;
+  const uint32_t *accepted, *attempts;
   uint32_t K;
   double *resid, *dI;
-  uint32_t *list, *keys;           // disk pixels and their accepted counts (unsorted)
+  uint32_t *list, *keys;           // disk pixels and their attempt counts (unsorted)
   GradCounters *cnt;
 };
 
 struct GradLaunch {
   const DevScene *scene;
   const uint32_t *list;            // disk pixels, longest first
-  const uint32_t *keys;            // their accepted counts
+  const uint32_t *keys;            // their attempt counts (step records)
+  const uint32_t *accepted;        // accepted count per pixel (the render's)
   uint64_t n_list;
   double *rec;                     // per-ray records
   double *chk;                     // checkpoint pool, 9 doubles each
   unsigned long long chk_cap;
   double *tail;                    // last-segment step records, GRAD_STEP_REC doubles each
   unsigned long long tail_cap;
-  double *seg;                     // per-thread segment buffers of the reverse kernel, K step records
+  double *seg;                     // per-thread segment buffers of the reverse kernel, seg_cap step records
+  uint32_t seg_cap;
+  uint32_t live;                   // 1: differentiate the step-size controller; 0: step sizes held fixed
   uint32_t K;
   uint32_t width, height;
   double incl, r_cam;
@@ -107,8 +112,8 @@ struct GradLaunch {
   uint64_t n_pix;
 };
 
-constexpr uint32_t GRAD_REC = 16;       // doubles per ray record
-constexpr uint32_t GRAD_STEP_REC = 32;  // doubles per step record: pos, vel, v_2..v_6, h, padding
+constexpr uint32_t GRAD_REC = 24;       // doubles per ray record
+constexpr uint32_t GRAD_STEP_REC = 32;  // doubles per step record: r, theta, h, kind, a velocity, v_2..v_6, err diff
 constexpr uint32_t GRAD_CHUNK = 16;     // step records per chunk of the recording render
 constexpr uint32_t GRAD_CHUNK_HEAD = 16;   // the back-pointer's 128-byte line
 constexpr uint32_t GRAD_CHUNK_DOUBLES = GRAD_CHUNK_HEAD + GRAD_CHUNK * GRAD_STEP_REC;

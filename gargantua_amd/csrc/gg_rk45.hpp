@@ -149,13 +149,19 @@ struct StageOut {
   bool invalid;
 };
 
-// Step record of an accepted step for the adjoint, 32 doubles (256 bytes,
-// two 128-byte lines; records are 128-byte aligned): start position (0..3),
-// velocity (4..7), the stage velocities v_2..v_6 (REC_VS + 4 (s - 2)), h
-// (REC_H), 3 doubles of padding.  Every 4-vector is 32-byte aligned.
+// Step record of one attempt for the adjoint, 32 doubles (256 bytes, two
+// 128-byte lines; records are 128-byte aligned; every 4-vector 32-byte
+// aligned): start r and theta (REC_R, REC_TH; t and phi enter nothing the
+// adjoint differentiates), h (REC_H), the attempt's kind (REC_KIND: REC_ACCEPT,
+// REC_REJECT or REC_INVALID, as a double), a velocity (REC_V: the start
+// velocity of an accepted attempt, the 5th-order velocity of a rejected one,
+// whose start velocity is that of the accepted attempt that follows it), the
+// stage velocities v_2..v_6 (REC_VS + 4 (s - 2)) and the error-estimate
+// difference new_vel - vel_err (REC_DV).  An invalid attempt (a stage
+// position left the domain) records only h and its kind.
 constexpr int STEP_REC = 32;
-constexpr int REC_VS = 8;
-constexpr int REC_H = 28;
+constexpr int REC_R = 0, REC_TH = 1, REC_H = 2, REC_KIND = 3, REC_V = 4, REC_VS = 8, REC_DV = 28;
+enum { REC_ACCEPT = 0, REC_REJECT = 1, REC_INVALID = 2 };
 
 // The stages of one attempt (geodesic.rs:244-308): k1 when the lane has no
 // valid one (s = 1), then k2..k7, the 5th-order velocity and the error.  One
@@ -256,6 +262,10 @@ __device__ inline StageOut stages(const DevScene &S, const KStore &K, const doub
   d4 ve{madd<ARITH>(h, s4.x, vel.x), madd<ARITH>(h, s4.y, vel.y), madd<ARITH>(h, s4.z, vel.z),
         madd<ARITH>(h, s4.w, vel.w)};
   double dx = nv.x - ve.x, dy = nv.y - ve.y, dz = nv.z - ve.z, dw = nv.w - ve.w;
+  if (REC && rec) {
+    double *dst = (double *)__builtin_assume_aligned(rec + REC_DV, 32);
+    dst[0] = dx; dst[1] = dy; dst[2] = dz; dst[3] = dw;
+  }
   if (S.err_mode == 1) {
     double s0 = dx / (S.atol + S.rtol * fmax(fabs(vel.x), fabs(nv.x)));
     double s1 = dy / (S.atol + S.rtol * fmax(fabs(vel.y), fabs(nv.y)));
@@ -312,8 +322,8 @@ __device__ inline bool fp32_eligible(const DevScene &S, const Lane &L) {
 // states reproduce across launches to ~1e-12, not bitwise, since which
 // eligible steps take which tier depends on the lane grouping.
 // Returns RES_*; `acc` reports acceptance; p0/v0 keep the pre-step state.
-// REC (adjoint kernels, PREC 0): rec, when non-null, receives the step
-// record of the attempt if it is accepted.
+// REC (recording render and adjoint kernels): rec, when non-null, receives
+// the step record of the attempt, whatever its outcome.
 template <int METRIC, int PREC, bool REC = false>
 __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool &acc, double p0[4], d4 &v0,
                               bool use32, bool use_fast = false, double *rec = nullptr) {
@@ -361,7 +371,11 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
     if (need_k1) { L.k1_f32 = false; L.k1_fast = false; }
   }
   L.k1_ok = true;
-  if (o.invalid) { L.h = h * 0.5; return RES_CONTINUE; }
+  if (o.invalid) {
+    if (REC && rec) { rec[REC_H] = h; rec[REC_KIND] = (double)REC_INVALID; }
+    L.h = h * 0.5;
+    return RES_CONTINUE;
+  }
   const double err = o.err, thresh = o.thresh;
   L.n_full++;
   L.err_prev = err / thresh;
@@ -370,9 +384,8 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
     L.pos[2] = madd<BASE>(h, vel.z, p0[2]); L.pos[3] = madd<BASE>(h, vel.w, p0[3]);
     if (REC && rec) {
       double *r = (double *)__builtin_assume_aligned(rec, 32);
-      r[0] = p0[0]; r[1] = p0[1]; r[2] = p0[2]; r[3] = p0[3];
-      r[4] = vel.x; r[5] = vel.y; r[6] = vel.z; r[7] = vel.w;
-      r[REC_H] = h;
+      r[REC_R] = p0[1]; r[REC_TH] = p0[2]; r[REC_H] = h; r[REC_KIND] = (double)REC_ACCEPT;
+      r[REC_V] = vel.x; r[REC_V + 1] = vel.y; r[REC_V + 2] = vel.z; r[REC_V + 3] = vel.w;
     }
     L.vel = o.nv;
     L.lambda = L.lambda + h;
@@ -410,6 +423,11 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
     }
     acc = true;
   } else {                                          // reject (:322-325)
+    if (REC && rec) {
+      double *r = (double *)__builtin_assume_aligned(rec, 32);
+      r[REC_R] = p0[1]; r[REC_TH] = p0[2]; r[REC_H] = h; r[REC_KIND] = (double)REC_REJECT;
+      r[REC_V] = o.nv.x; r[REC_V + 1] = o.nv.y; r[REC_V + 2] = o.nv.z; r[REC_V + 3] = o.nv.w;
+    }
     double hn = h * 0.9 * pow(thresh / err, 0.25);
     L.h = hn >= S.min_step ? hn : S.min_step;       // f64::max
   }

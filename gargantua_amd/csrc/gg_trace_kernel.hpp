@@ -45,10 +45,10 @@ __device__ inline uint32_t find_scene(const DevScene *scenes, uint32_t n, uint64
 }
 
 // Persistent trace kernel.  SRC 0: rays from the camera; 1: states_in;
-// 2: rays from the camera, recording every accepted step (the record the
-// adjoint reads: start state, h, stage velocities) into chunks of
-// GRAD_CHUNK records, each chunk headed by the index of the ray's previous
-// chunk, and each ray's last chunk into rec_last.
+// 2: rays from the camera, recording every attempt (the step record the
+// adjoint reads: start state, h, kind, stage velocities, error difference)
+// into chunks of GRAD_CHUNK records, each chunk headed by the index of the
+// ray's previous chunk, and each ray's last chunk into rec_last.
 // BATCH 0: one scene, its constants wavefront-uniform (scalar registers);
 // BATCH 1: each lane looks up the scene of its ray.
 // DISK 0 none, 1 volumetric, 2 thin; 3 the Hittable scene alone
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
   bool exhausted = false;
   constexpr bool record = (SRC == 2);
   constexpr uint32_t NO_CHUNK = 0xFFFFFFFFu;
-  uint32_t r_chunk = NO_CHUNK;   // record chunk holding step `accepted`
+  uint32_t r_chunk = NO_CHUNK;   // record chunk holding attempt `attempts`
   bool r_need = false;           // a new chunk is due before the next attempt
   // Scene constants are read through the pointer where they are used, so
   // the compiler reloads them (scalar loads for a single scene) instead of
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
       }
       double *recp = nullptr;
       if (record) {
-        // a new chunk every GRAD_CHUNK accepted steps, one atomicAdd per
+        // a new chunk every GRAD_CHUNK attempts, one atomicAdd per
         // wavefront; chunks beyond the pool are counted, not written
         const unsigned long long m = __ballot(r_need);
         if (m) {
@@ -253,15 +253,13 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
           }
         }
         if (r_chunk != NO_CHUNK)
-          recp = T.rec_pool + (uint64_t)r_chunk * GRAD_CHUNK_DOUBLES + GRAD_CHUNK_HEAD + GRAD_STEP_REC * (L.accepted % GRAD_CHUNK);
+          recp = T.rec_pool + (uint64_t)r_chunk * GRAD_CHUNK_DOUBLES + GRAD_CHUNK_HEAD + GRAD_STEP_REC * (L.attempts % GRAD_CHUNK);
       }
       int res = attempt<METRIC, PREC, record>(*Sp, L, K, acc, p0, v0, use32, use_fast, recp);
       L.attempts++;
       n_att++;
-      if (acc) {
-        L.accepted++;
-        if (record && L.accepted % GRAD_CHUNK == 0) r_need = true;
-      }
+      if (record && L.attempts % GRAD_CHUNK == 0) r_need = true;
+      if (acc) L.accepted++;
       // A chord whose two end points lie inside the ball r < scene_rmin stays
       // in it (the ball is convex) and no primitive reaches into it: no test.
       if (DISK >= 3 && acc && fmax(p0[1], L.pos[1]) >= Sp->scene_rmin) {

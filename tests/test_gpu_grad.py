@@ -2,8 +2,12 @@
 
 * VJP of the Kerr RHS vs w^T J of the oracle's dual-number Jacobian.
 * gg_grad's loss, gradient and per-pixel dI vs the oracle's forward-mode
-  dual-number gradient of the same discrete map, and vs central finite
-  differences (h = 1e-5) of the oracle's frozen-step loss: 1e-4 relative.
+  dual-number gradient (step sizes differentiated through the controller,
+  and with frozen_steps held fixed).
+* gg_grad vs central finite differences of the loss the render computes
+  (h = 1e-6 over every ray, h = 1e-5 over the rays whose step decisions do
+  not flip between theta - h and theta + h): 1e-4 relative.  With
+  frozen_steps, vs central differences of the oracle's frozen-step loss.
 * The result does not depend on the checkpoint interval K (bitwise: the
   recomputed segments repeat the forward steps exactly).
 """
@@ -14,6 +18,8 @@ import pytest
 
 import gargantua_amd as gg
 from gargantua_amd import configs
+
+from .gradcheck import central_fd, masked_gradient
 
 pytestmark = pytest.mark.gpu
 
@@ -50,9 +56,11 @@ def test_rhs_vjp_matches_oracle_jacobian(renderer, oracle, spin):
     assert np.all(g["pos_bar"][:, [0, 3]] == 0.0)
 
 
-def test_grad_matches_oracle_dual_gradient(renderer, oracle):
+@pytest.mark.parametrize("frozen", [0, 1])
+def test_grad_matches_oracle_dual_gradient(renderer, oracle, frozen):
     target = _target(oracle)
     g0 = _scene()
+    g0.frozen_steps = frozen
     o = oracle.grad(g0, target, want_dI=True)
     a = renderer.grad(g0, target, checkpoint_every=8, want_dI=True)
     st = a["stats"]
@@ -77,9 +85,28 @@ def test_grad_matches_oracle_dual_gradient(renderer, oracle):
     assert np.all(a["dI"][~disk] == 0.0)
 
 
+@pytest.mark.parametrize("checkpoint_every", [0, 32])
+def test_grad_matches_central_differences_of_the_loss(renderer, oracle, checkpoint_every):
+    """The gradient of the loss the render computes, step sizes included, against
+    central differences of the oracle's loss on the 40 x 40 gradient scene."""
+    target = _target(oracle)
+    a = renderer.grad(_scene(), target, checkpoint_every=checkpoint_every, want_dI=True)
+    fd6 = central_fd(oracle, _scene, target, 1e-6)
+    rel6 = np.abs(a["grad"] - fd6["fd"]) / np.abs(fd6["fd"])
+    print("adjoint", a["grad"], "fd(1e-6)", fd6["fd"], "rel", rel6, "flips", fd6["flips"])
+    assert np.all(rel6 < 1e-4)
+    fd5 = central_fd(oracle, _scene, target, 1e-5)
+    ad5 = masked_gradient(a["dI"], a["intensity"], target, fd5["mask"])
+    rel5 = np.abs(ad5 - fd5["fd_smooth"]) / np.abs(fd5["fd_smooth"])
+    print("fd(1e-5) over smooth rays", fd5["fd_smooth"], "rel", rel5, "flips", fd5["flips"])
+    assert np.all(rel5 < 1e-4)
+    assert np.all(fd5["flips"] <= 0.01 * N * N)
+
+
 def test_grad_matches_frozen_step_central_differences(renderer, oracle):
     target = _target(oracle)
     g0 = _scene()
+    g0.frozen_steps = 1
     a = renderer.grad(g0, target, checkpoint_every=32)
     fields = ["spin", "inclination", "emissivity_q", "r_inner"]
     h = 1e-5
@@ -94,9 +121,11 @@ def test_grad_matches_frozen_step_central_differences(renderer, oracle):
     assert np.all(rel < 1e-4)
 
 
-def test_grad_independent_of_checkpoint_interval(renderer, oracle):
+@pytest.mark.parametrize("frozen", [0, 1])
+def test_grad_independent_of_checkpoint_interval(renderer, oracle, frozen):
     target = _target(oracle)
     g0 = _scene()
+    g0.frozen_steps = frozen
     ref = renderer.grad(g0, target, checkpoint_every=1, want_dI=True)
     assert ref["stats"]["mode"] == 0
     for K in (3, 8, 32, 128, 100000 // 2, 0):
