@@ -85,7 +85,8 @@ class gg_stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("rays", C.c_uint64),
                 ("attempts", C.c_uint64), ("rhs_evals", C.c_uint64), ("rhs_evals_fp32", C.c_uint64),
                 ("rhs_evals_fast", C.c_uint64),
-                ("attempts_full", C.c_uint64), ("grid_blocks", C.c_uint32), ("block_threads", C.c_uint32)]
+                ("attempts_full", C.c_uint64), ("reflections", C.c_uint64), ("grid_blocks", C.c_uint32),
+                ("block_threads", C.c_uint32)]
 
 
 class gg_launch(C.Structure):

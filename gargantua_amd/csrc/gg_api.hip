@@ -879,6 +879,14 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
     hipFunction_t at = nullptr;
     if ((rc = user_kernels(ctx, mode, from_states ? 1 : 0, &user_trace, &at))) return rc;
     if ((rc = user_metric_at(ctx, at, scenes[0].metric, scenes[0].camera.position, um.g))) return rc;
+    // the trace kernel reads the tensor's derivatives from its upper
+    // triangle, which equals the full read only for a symmetric tensor
+    for (int i = 0; i < 4; ++i)
+      for (int j = i + 1; j < 4; ++j)
+        if (!(um.g[4 * i + j] == um.g[4 * j + i]))
+          return fail(GG_ERR_INVALID_ARG, "the user metric tensor is not symmetric at the camera: g[%d][%d] = %.17g, "
+                      "g[%d][%d] = %.17g (both triangles must hold the same value)", i, j, um.g[4 * i + j], j, i,
+                      um.g[4 * j + i]);
     um.has_rh = ctx->user_has_rh;
     um.rh = ctx->user_rh;
   }
@@ -982,8 +990,19 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
   uint32_t block = ctx->launch.block_threads;
   const bool use_graph = (n_scenes > 1) && ctx->launch.graph == 1 && total > 0;
   if (use_graph) {
-    gg_context::GraphKey key{n_scenes, scenes[0].metric.kind, scenes[0].disk.kind, precision, total, ctx->launch,
-                             ctx->scenes.p, ctx->outs.p, ctx->counters.p, (const void *)T.order};
+    // compared with memcmp: zero the padding bytes before filling the fields
+    gg_context::GraphKey key;
+    std::memset(&key, 0, sizeof(key));
+    key.n_scenes = n_scenes;
+    key.metric = scenes[0].metric.kind;
+    key.disk = scenes[0].disk.kind;
+    key.precision = precision;
+    key.n_rays = total;
+    key.launch = ctx->launch;
+    key.scenes = ctx->scenes.p;
+    key.outs = ctx->outs.p;
+    key.counters = ctx->counters.p;
+    key.order = (const void *)T.order;
     if (!ctx->graph_exec || std::memcmp(&key, &ctx->graph_key, sizeof(key)) != 0) {
       if (ctx->graph_exec) { (void)hipGraphExecDestroy(ctx->graph_exec); ctx->graph_exec = nullptr; }
       if (ctx->graph) { (void)hipGraphDestroy(ctx->graph); ctx->graph = nullptr; }
@@ -1001,7 +1020,7 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
       }
       ctx->graph = g;
       HIP_TRY(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
-      ctx->graph_key = key;
+      std::memcpy(&ctx->graph_key, &key, sizeof(key));
     }
     HIP_TRY(hipEventRecord(ctx->ev0, s));
     HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));
@@ -1029,7 +1048,11 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
     }
   }
   if (learn_order) {
-    // the next render's queue: rays by this render's attempt counts, longest first
+    // the next render's queue: rays by this render's attempt counts, longest
+    // first.  The stored order is invalid until the sort below has run: a
+    // failure on the way (a reallocation leaves the buffer uninitialised)
+    // must not leave the previous grid's signature pointing at it.
+    ctx->order_sig = 0;
     if ((rc = ctx->order.ensure(4 * total)) || (rc = ctx->order_keys.ensure(4 * total)) ||
         (rc = ctx->order_ids.ensure(4 * total)) || (rc = ctx->order_keys_out.ensure(4 * total)))
       return rc;
@@ -1068,6 +1091,7 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
     stats->rhs_evals_fp32 = cnt.rhs32;
     stats->rhs_evals_fast = cnt.rhsf;
     stats->attempts_full = cnt.attempts_full;
+    stats->reflections = cnt.reflections;
     stats->grid_blocks = grid;
     stats->block_threads = block;
   }

@@ -328,7 +328,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
   const DevScene &S = *G.scene;
   const uint32_t KK = G.K;
   Lane L;
-  L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0;
+  L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0; L.n_refl = 0;
   bool active = false, exhausted = false;
   uint64_t q = 0, chk_base = 0, tail_base = 0;
   uint32_t pix = 0, acc_target = 0, att_target = 0, last0 = 0, att = 0, tail_cap = 0, seg_att = 0;
@@ -437,7 +437,7 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
   double *const seg = RECOMPUTE ? G.seg + gtid * (uint64_t)G.seg_cap * STEP_REC : nullptr;
 
   Lane L;
-  L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0;
+  L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0; L.n_refl = 0;
   enum { IDLE = 0, REV = 1, RECOMP = 2 };
   int phase = IDLE;
   bool exhausted = false, first = false;

@@ -29,7 +29,7 @@ struct Counters {
   unsigned long long attempts_full;
   unsigned long long retired;
   unsigned long long rhsf;           // RHS evaluations of fast-FP64 attempts (mixed)
-  unsigned long long pad[1];
+  unsigned long long reflections;    // accepted steps reflected at a pole
 };
 
 struct TraceLaunch {

@@ -20,6 +20,7 @@
 
 #include "gargantua.h"
 #include "gg_types.hpp"
+#include "gg_crtrig.hpp"
 
 namespace gg {
 
@@ -81,10 +82,10 @@ __device__ inline Gmn metric64(const DevScene &S, double r_in, double theta) {
   if (METRIC == 1) {
     double m = S.mass, a = S.spin;
     double a2 = a * a;
-    double ct = cos(theta);
+    double st, ct;
+    sincos_cr(theta, &st, &ct);   // the host libm's values (correctly rounded)
     double sigma = r * r + a * a * ct * ct;
     double delta = r * r - 2.0 * m * r + a * a;
-    double st = sin(theta);
     double sin2 = st * st;
     g.tt = -(1.0 - 2.0 * m * r / sigma);
     g.rr = sigma / delta;
@@ -93,7 +94,8 @@ __device__ inline Gmn metric64(const DevScene &S, double r_in, double theta) {
     g.tph = -2.0 * m * a * r * sin2 / sigma;
   } else {
     double f = 1.0 - S.rs / r;
-    double st = sin(theta);
+    double st, ct;
+    sincos_cr(theta, &st, &ct);
     g.tt = -f;
     g.rr = 1.0 / f;
     g.thth = r * r;
@@ -109,10 +111,10 @@ __device__ inline d4 rhs64_kerr(const DevScene &S, double r_in, double theta, d4
   double r = rmax1e10(r_in);
   const double m = S.mass, a = S.spin;
   double a2 = a * a;
-  double cs = cos(theta);
+  double st, cs;
+  sincos_cr(theta, &st, &cs);   // the host libm's values (correctly rounded)
   double sigma = r * r + a * a * cs * cs;
   double delta = r * r - 2.0 * m * r + a * a;
-  double st = sin(theta);
   double ct = cs;  // cos(theta) evaluated twice in the reference; same value
   double sin2 = st * st;
   double sigma2 = sigma * sigma;
@@ -191,8 +193,8 @@ __device__ inline d4 rhs64_kerr(const DevScene &S, double r_in, double theta, d4
 __device__ inline d4 rhs64_schw(const DevScene &S, double r_in, double theta, d4 v) {
 #pragma clang fp contract(off)
   double r = rmax1e10(r_in);
-  double st = sin(theta);
-  double ct = cos(theta);
+  double st, ct;
+  sincos_cr(theta, &st, &ct);   // the host libm's values (correctly rounded)
   const double rs = S.rs;
   double f_denom = r * (r - rs);
   double rmrs = r - rs;
@@ -474,7 +476,8 @@ __device__ inline double redshift64(const DevScene &S, double r, double theta, d
 // AccretionDisk::density (disk.rs:154-173).
 __device__ inline double disk_density(const DevScene &S, double r, double theta) {
 #pragma clang fp contract(off)
-  double st = sin(theta), ct = cos(theta);
+  double st, ct;
+  sincos_cr(theta, &st, &ct);
   double cap_r = r * fabs(st);
   double z = r * ct;
   if (cap_r < S.r_in || cap_r > S.r_out) return 0.0;
@@ -488,7 +491,9 @@ __device__ inline double disk_density(const DevScene &S, double r, double theta)
 __device__ inline double disk_emission(const DevScene &S, double rho, double r, double theta) {
 #pragma clang fp contract(off)
   if (rho == 0.0) return 0.0;
-  double cap_r = r * fabs(sin(theta));
+  double st, ct;
+  sincos_cr(theta, &st, &ct);
+  double cap_r = r * fabs(st);
   if (cap_r < S.r_in) return 0.0;
   double temp_proxy = pow(S.r_in / cap_r, 0.75);
   return rho * rho * temp_proxy;
@@ -507,7 +512,8 @@ __device__ inline void constants_of_motion(const DevScene &S, const double pos[4
   const double pph = g.tph * v.x + g.phph * v.w;
   const double pth = g.thth * v.z;
   const double mu2 = -(pt * v.x + g.rr * v.y * v.y + pth * v.z + pph * v.w);
-  const double st = sin(pos[2]), ct = cos(pos[2]);
+  double st, ct;
+  sincos_cr(pos[2], &st, &ct);
   const double a = METRIC == 1 ? S.spin : 0.0;
   out[0] = -pt;
   out[1] = pph;

@@ -70,6 +70,7 @@ struct Lane {
   double err_prev;   // error ratio of the previous full attempt
   double b2;         // (Q + L_z^2) / E^2 of the ray's start state (mixed kernel)
   uint32_t n_rhs, n_rhs32, n_rhsf, n_full;
+  uint32_t n_refl;   // accepted steps whose theta was reflected at a pole
 };
 
 // Per-lane view of the LDS stage store: slots 0..4 hold k1..k5, structure of
@@ -393,6 +394,7 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
     bool refl = false;
     if (L.pos[2] < 0.0) { L.pos[2] = -L.pos[2]; L.pos[3] = L.pos[3] + PI; refl = true; }
     if (L.pos[2] > PI) { L.pos[2] = 2.0 * PI - L.pos[2]; L.pos[3] = L.pos[3] + PI; refl = true; }
+    L.n_refl += refl ? 1u : 0u;
     // rem_euclid(2 pi); fmod is the identity on [0, 2 pi), where phi almost always is
     const double p3 = L.pos[3];
     if (!(p3 >= 0.0 && p3 < TWO_PI)) {

@@ -58,8 +58,8 @@ __device__ inline double sdot3(const double a[3], const double b[3]) {
 __device__ inline void sph2cart(const double p[4], double c[3]) {
 #pragma clang fp contract(off)
   double st, ct, sp, cp;
-  sincos(p[2], &st, &ct);
-  sincos(p[3], &sp, &cp);
+  sincos_cr(p[2], &st, &ct);   // the host libm's values (correctly rounded)
+  sincos_cr(p[3], &sp, &cp);
   const double r = p[1];
   c[0] = r * st * cp;
   c[1] = r * st * sp;

@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
   extern __shared__ double lds_k[];
   const KStore K{lds_k + threadIdx.x, blockDim.x};
   Lane L;
-  L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0;
+  L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0; L.n_refl = 0;
   uint32_t n_att = 0, n_retired = 0;
   bool active = false;
   uint64_t ray = 0;
@@ -388,7 +388,8 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
   }
   // ---- tallies: one atomic per wavefront ----
   unsigned long long a = wave_sum(n_att), r = wave_sum(L.n_rhs), r32 = wave_sum(L.n_rhs32),
-                     rf = wave_sum(L.n_rhsf), f = wave_sum(L.n_full), ret = wave_sum(n_retired);
+                     rf = wave_sum(L.n_rhsf), f = wave_sum(L.n_full), ret = wave_sum(n_retired),
+                     rl = wave_sum(L.n_refl);
   if (lane == 0) {
     atomicAdd(&T.counters->retired, ret);
     atomicAdd(&T.counters->attempts, a);
@@ -396,6 +397,7 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
     atomicAdd(&T.counters->rhs32, r32);
     atomicAdd(&T.counters->rhsf, rf);
     atomicAdd(&T.counters->attempts_full, f);
+    if (rl) atomicAdd(&T.counters->reflections, rl);
   }
 }
 

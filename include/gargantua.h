@@ -223,6 +223,9 @@ typedef struct gg_stats {
   uint64_t rhs_evals_fp32;  /* of which evaluated in FP32 (mixed)             */
   uint64_t rhs_evals_fast;  /* of which evaluated in fast FP64 (mixed)        */
   uint64_t attempts_full;   /* attempts that reached the error estimate       */
+  uint64_t reflections;     /* accepted steps whose theta crossed a pole and
+                               was reflected (normalize_spherical_coords,
+                               geodesic.rs:337-352)                           */
   uint32_t grid_blocks;
   uint32_t block_threads;
 } gg_stats;
@@ -421,7 +424,8 @@ int gg_set_hittables(gg_context *ctx, uint32_t n, const gg_hittable *objects);
  * filling g[4*mu + nu] = g_mu,nu at x = (t, r, theta, phi) (any helper
  * functions it needs may precede it).  The tensor is symmetric: the
  * Christoffel symbols read its derivatives from the upper triangle
- * (mu <= nu), so g_mu,nu and g_nu,mu should hold the same value.  A source may also
+ * (mu <= nu), so g_mu,nu and g_nu,mu must hold the same value; a render
+ * whose tensor is not symmetric at the camera fails with GG_ERR_INVALID_ARG.  A source may also
  * `#define GG_METRIC_STATIONARY` (g does not depend on t) and/or
  * `#define GG_METRIC_AXISYMMETRIC` (g does not depend on phi): the two
  * tensor evaluations of that coordinate's central difference, which are

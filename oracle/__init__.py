@@ -49,6 +49,7 @@ _SIGS = [
     ("oracle_trace_states", C.c_int, [C.POINTER(_abi.gg_scene), C.c_uint64, _P, C.POINTER(_abi.gg_outputs),
                                       C.c_int]),
     ("oracle_last_rhs", C.c_uint64, []),
+    ("oracle_last_reflections", C.c_uint64, []),
     ("oracle_r_isco", _D, [_D, _D]),
     ("oracle_grad", C.c_int, [C.POINTER(_abi.gg_grad_scene), _P, C.POINTER(_D), _P, _P, _P, C.c_int]),
     ("oracle_loss_frozen", C.c_int, [C.POINTER(_abi.gg_grad_scene), C.POINTER(_abi.gg_grad_scene), _P,
@@ -275,6 +276,7 @@ class Oracle:
         if rc:
             raise RuntimeError(f"oracle_render failed: {rc}")
         o["rhs"] = self.lib.oracle_last_rhs()
+        o["reflections"] = self.lib.oracle_last_reflections()
         return o
 
     def render_rays(self, scene: Scene, idx, state: bool = True) -> dict:
@@ -286,6 +288,7 @@ class Oracle:
         if rc:
             raise RuntimeError(f"oracle_render_rays failed: {rc}")
         o["rhs"] = self.lib.oracle_last_rhs()
+        o["reflections"] = self.lib.oracle_last_reflections()
         return o
 
     def trace_states(self, scene: Scene, states) -> dict:

@@ -168,6 +168,7 @@ template <class T> struct TraceOut {
   double color[3] = {0.0, 0.0, 0.0};   // RayOutcome::Scene { color }
   uint32_t attempts, accepted;
   uint64_t rhs;
+  uint32_t reflections = 0;   // accepted steps whose theta was reflected at a pole
   State<T> st;
 };
 
@@ -345,6 +346,7 @@ TraceOut<T> trace(const MetricP &mp, const T &a, const IntegP &ip, const DiskT<T
     o.decisions = (o.decisions ^ info.decisions) * 1099511628211ull;
     if (log && log->recording) log->acc.push_back(info.accepted ? 1 : 0);
     o.rhs += info.rhs;
+    if (info.reflected) ++o.reflections;
     if (info.accepted) ++o.accepted;
 
     if constexpr (std::is_same<T, double>::value) {
@@ -575,6 +577,7 @@ bool scene_ok(const gg_scene *s) {
 }
 
 thread_local uint64_t g_last_rhs = 0;
+thread_local uint64_t g_last_reflections = 0;   // pole reflections of the last render
 
 }  // namespace
 
@@ -820,7 +823,7 @@ int oracle_render_rays(const gg_scene *s, uint64_t n, const uint64_t *idx, const
   DiskT<double> disk = disk_from<double>(s->disk);
   uint32_t spa = s->image.samples_per_axis;
   uint32_t sw = s->image.width * spa, sh = s->image.height * spa;
-  std::atomic<uint64_t> rhs{0};
+  std::atomic<uint64_t> rhs{0}, refl{0};
   parallel_for(n, nthreads, 16, [&](uint64_t i) {
     uint64_t k = idx ? idx[i] : i;
     uint32_t px = (uint32_t)(k % sw), py = (uint32_t)(k / sw);
@@ -829,8 +832,10 @@ int oracle_render_rays(const gg_scene *s, uint64_t n, const uint64_t *idx, const
     store(out, i, o);
     store_state(out->state, n, i, o.st);
     rhs.fetch_add(o.rhs, std::memory_order_relaxed);
+    refl.fetch_add(o.reflections, std::memory_order_relaxed);
   });
   g_last_rhs = rhs.load();
+  g_last_reflections = refl.load();
   return 0;
 }
 
@@ -847,7 +852,7 @@ int oracle_render(const gg_scene *s, const gg_outputs *out, int nthreads) {
   uint32_t sw = w * spa, sh = hgt * spa;
   uint64_t n = (uint64_t)sw * sh;
   double inv_spp = 1.0 / (double)(spa * spa);
-  std::atomic<uint64_t> rhs{0};
+  std::atomic<uint64_t> rhs{0}, refl{0};
   parallel_for(hgt, nthreads, 1, [&](uint64_t py) {
     for (uint32_t px = 0; px < w; ++px) {
       double acc[3] = {0.0, 0.0, 0.0};
@@ -860,6 +865,7 @@ int oracle_render(const gg_scene *s, const gg_outputs *out, int nthreads) {
           store(out, i, o);
           store_state(out->state, n, i, o.st);
           rhs.fetch_add(o.rhs, std::memory_order_relaxed);
+          refl.fetch_add(o.reflections, std::memory_order_relaxed);
           double lin[3];
           double vel[4] = {o.st.vel[0], o.st.vel[1], o.st.vel[2], o.st.vel[3]};
           shade(o.cls, o.intensity, vel, o.color, lin);
@@ -873,6 +879,7 @@ int oracle_render(const gg_scene *s, const gg_outputs *out, int nthreads) {
     }
   });
   g_last_rhs = rhs.load();
+  g_last_reflections = refl.load();
   return 0;
 }
 
@@ -883,7 +890,7 @@ int oracle_trace_states(const gg_scene *s, uint64_t n, const double *states, con
   IntegP ip = make_integ(s->integrator);
   CameraT<double> cam = camera_from<double>(s->camera);
   DiskT<double> disk = disk_from<double>(s->disk);
-  std::atomic<uint64_t> rhs{0};
+  std::atomic<uint64_t> rhs{0}, refl{0};
   parallel_for(n, nthreads, 16, [&](uint64_t i) {
     State<double> st;
     for (int c = 0; c < 4; ++c) { st.pos[c] = states[9 * i + c]; st.vel[c] = states[9 * i + 4 + c]; }
@@ -892,12 +899,15 @@ int oracle_trace_states(const gg_scene *s, uint64_t n, const double *states, con
     store(out, i, o);
     store_state(out->state, n, i, o.st);
     rhs.fetch_add(o.rhs, std::memory_order_relaxed);
+    refl.fetch_add(o.reflections, std::memory_order_relaxed);
   });
   g_last_rhs = rhs.load();
+  g_last_reflections = refl.load();
   return 0;
 }
 
 uint64_t oracle_last_rhs(void) { return g_last_rhs; }
+uint64_t oracle_last_reflections(void) { return g_last_reflections; }
 
 // Prograde ISCO radius (Bardeen, Press & Teukolsky 1972, eq. 2.21).
 double oracle_r_isco(double mass, double spin) {

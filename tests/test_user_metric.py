@@ -289,6 +289,14 @@ def test_gpu_user_metric_errors(renderer):
     again = renderer.render(_scene(m, w=8, h=8), "fp64")
     np.testing.assert_array_equal(again["cls"], first["cls"])
     np.testing.assert_array_equal(again["r"], first["r"])
+    # a tensor that fills only one triangle of g_t,phi: the device's
+    # upper-triangle derivative read would differ from the full one, so the
+    # render is refused
+    half = KERR.replace("g[3] = g[12] =", "g[3] =")
+    hm = gg.UserMetric(half, 1.0, 0.9, 1.0 + math.sqrt(0.19))
+    renderer.set_user_metric(hm)
+    with pytest.raises(gg.GargantuaError, match="not symmetric"):
+        renderer.render(_scene(hm, w=8, h=8), "fp64")
     renderer.set_user_metric(None)
     sc = _scene(m, w=8, h=8)
     o = gg.alloc_outputs(sc.n_rays)
