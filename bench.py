@@ -7,9 +7,14 @@ memory.  The plain FP64 kernel is timed the same way beside it, and the
 mixed result is compared with the FP64 one (class agreement, final-state
 error).  The roofline object reports algorithmic FLOPs of the trace kernel
 over its hipEvent time against the vector (VALU) peak; the CPU baseline is
-the FP64 oracle on a stratified sample of the same rays.
+the FP64 oracle on every ray of the same scene, whose outputs are also
+compared with both GPU kernels.  The same line carries one timed frame of
+the near-extremal 4096x4096 workload and of the 64-scene batch (FP64 and
+mixed, roofline fractions, oracle on a stride sample), and the adjoint's
+gradients/s on the 512x512 differentiable render with its error against
+central differences of the oracle's loss.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--headline-only]
 """
 from __future__ import annotations
 
@@ -112,7 +117,7 @@ def cpu_baseline(scene, stride=1):
     cores = max(1, min(cores, 16))
     ora = Oracle(threads=cores)
     t0 = time.perf_counter()
-    ora.render_rays(scene, idx, state=False)
+    ref = ora.render_rays(scene, idx, state=False)
     dt = time.perf_counter() - t0
     one = Oracle(threads=1)
     sub = idx[:: max(1, len(idx) // 2048)]
@@ -123,7 +128,7 @@ def cpu_baseline(scene, stride=1):
             "sample": f"every {stride}. row and column of the {scene.width}x{scene.height} jittered Kerr a=0.9 scene "
                       f"({len(idx)} rays, {dt:.1f} s on {cores} threads); 1 thread: {len(sub) / dt1:.1f} geodesics/s "
                       f"on {len(sub)} rays",
-            "value_1core": len(sub) / dt1}
+            "value_1core": len(sub) / dt1}, ref
 
 
 def com_drift(rend, scene, precision):
@@ -140,6 +145,191 @@ def com_drift(rend, scene, precision):
     for k, v in rel.items():
         out[k] = {"median": float(np.median(v)), "p99": float(np.quantile(v, 0.99))}
     return out
+
+
+def _host_at(torch, outs, idx=None):
+    """Per-ray outputs copied to the host (all rays, or the rays of idx)."""
+    if idx is None:
+        h = {k: v.cpu().numpy() for k, v in outs.items()}
+    else:
+        it = torch.from_numpy(np.asarray(idx, dtype=np.int64)).to(next(iter(outs.values())).device)
+        h = {k: v.index_select(0, it).cpu().numpy() for k, v in outs.items()}
+    h["attempts"] = h["attempts"].astype(np.uint32)
+    return h
+
+
+def _roofline_frac(stats, kernel_ms):
+    f32, f64 = flops(stats, "kerr")
+    peak = (f32 + f64) / (f32 / PEAK_FP32 + f64 / PEAK_FP64) if (f32 + f64) else PEAK_FP64
+    return (f32 + f64) / (kernel_ms * 1e-3) / 1e12 / peak, (f32 + f64) / (kernel_ms * 1e-3) / 1e12
+
+
+def _vs_oracle(gpu, ora, band=None):
+    """Class agreement (outside the band too when given), final states within 1e-6, attempt counts."""
+    from tests.parity import state_rel_err
+    eq = gpu["cls"] == ora["cls"]
+    err = state_rel_err(gpu, ora)
+    out = {"rays": int(len(eq)), "class_agree": float(np.mean(eq)),
+           "err_le_1e-6": float(np.mean(err[eq] <= 1e-6)) if eq.any() else 1.0,
+           "err_median": float(np.median(err[eq])) if eq.any() else 0.0,
+           "attempts_equal": float(np.mean(gpu["attempts"] == ora["attempts"]))}
+    if band is not None:
+        ok = eq & ~band
+        out["class_agree_outside_band"] = float(np.mean(eq[~band])) if (~band).any() else 1.0
+        out["err_le_1e-6_outside_band"] = float(np.mean(err[ok] <= 1e-6)) if ok.any() else 1.0
+    return out
+
+
+def _stride_idx(w, h, stride):
+    return np.array([py * w + px for py in range(0, h, stride) for px in range(0, w, stride)], dtype=np.int64)
+
+
+def frame_kerr4096(torch, rend, size=4096, ora_stride=32):
+    """One timed frame of the near-extremal workload per precision (a first frame in tile order, then the
+    frame under the learned longest-first order), with roofline fractions and the oracle on a stride sample."""
+    import gargantua_amd as gg
+    from oracle import Oracle
+    from tests.parity import boundary_band
+    scene = gg.configs.kerr_4096(size, size)
+    n = scene.n_rays
+    idx = _stride_idx(size, size, ora_stride)
+    res, sub = {}, {}
+    band = None
+    for prec in ("fp64", "mixed"):
+        outs = _dev_outputs(torch, n)
+        rend.set_launch(64, 8, 64, 1, 1)
+        first = rend.render(scene, prec, outputs=dict(_ptrs(outs)))["stats"]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = rend.render(scene, prec, outputs=dict(_ptrs(outs)))["stats"]
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        frac, tf = _roofline_frac(st, st["kernel_ms"])
+        res[prec] = {"value": n / wall, "kernel_ms": st["kernel_ms"], "first_frame_kernel_ms": first["kernel_ms"],
+                     "roofline_frac": frac, "achieved_tflops": tf, "attempts_per_ray": st["attempts"] / n,
+                     "fp32_rhs_share": st["rhs_evals_fp32"] / st["rhs_evals"],
+                     "fast_rhs_share": st["rhs_evals_fast"] / st["rhs_evals"]}
+        if prec == "fp64":
+            band = boundary_band(outs["cls"].cpu().numpy().reshape(size, size)).ravel()[idx]
+        sub[prec] = _host_at(torch, outs, idx)
+        _log(f"kerr4096 {prec}: first frame {first['kernel_ms']:.0f} ms, learned order {st['kernel_ms']:.0f} ms")
+        del outs
+        torch.cuda.empty_cache()
+    rend.set_launch()
+    ora = Oracle(threads=max(1, min(16, len(os.sched_getaffinity(0)))))
+    t0 = time.perf_counter()
+    ref = ora.render_rays(scene, idx.astype(np.uint64), state=False)
+    dt = time.perf_counter() - t0
+    return {"workload": f"Kerr a=0.998 {size}x{size}, camera r=1000M i=85deg FOV 20deg, escape 1100M, 50000 attempts",
+            "launch": "64 threads x 8 blocks per CU, refill 64, learned order", "rays": n,
+            "fp64": res["fp64"], "mixed": res["mixed"],
+            "mixed_speedup": res["fp64"]["kernel_ms"] / res["mixed"]["kernel_ms"],
+            "fp64_vs_oracle": _vs_oracle(sub["fp64"], ref, band), "mixed_vs_oracle": _vs_oracle(sub["mixed"], ref, band),
+            "oracle_sample": f"every {ora_stride}th row and column ({len(idx)} rays, {dt:.1f} s on {ora.threads} threads)"}
+
+
+def frame_batch64(torch, rend, ora_stride=16):
+    """One timed frame of the 64-scene batch per precision (graph-captured launch, learned order after a first
+    frame), with roofline fractions and the oracle on a stride sample of every scene."""
+    import gargantua_amd as gg
+    from oracle import Oracle
+    from tests.parity import boundary_band
+    scenes = gg.configs.batch_64(seed=2026)
+    n = sum(s.n_rays for s in scenes)
+    idxs = [_stride_idx(s.width, s.height, ora_stride) for s in scenes]
+    res, sub, bands = {}, {}, []
+    for prec in ("fp64", "mixed"):
+        outs = [_dev_outputs(torch, s.n_rays) for s in scenes]
+        rend.set_launch(64, 8, 64, 1, 1)
+        _, first = rend.render_batch(scenes, prec, outputs=[dict(_ptrs(o)) for o in outs])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, st = rend.render_batch(scenes, prec, outputs=[dict(_ptrs(o)) for o in outs])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        frac, tf = _roofline_frac(st, st["kernel_ms"])
+        res[prec] = {"value": n / wall, "kernel_ms": st["kernel_ms"], "first_frame_kernel_ms": first["kernel_ms"],
+                     "roofline_frac": frac, "achieved_tflops": tf, "attempts_per_ray": st["attempts"] / n,
+                     "fp32_rhs_share": st["rhs_evals_fp32"] / st["rhs_evals"],
+                     "fast_rhs_share": st["rhs_evals_fast"] / st["rhs_evals"]}
+        if prec == "fp64":
+            bands = [boundary_band(o["cls"].cpu().numpy().reshape(s.height, s.width)).ravel()[ix]
+                     for o, s, ix in zip(outs, scenes, idxs)]
+        sub[prec] = [_host_at(torch, o, ix) for o, ix in zip(outs, idxs)]
+        _log(f"batch64 {prec}: first frame {first['kernel_ms']:.0f} ms, learned order {st['kernel_ms']:.0f} ms")
+        del outs
+        torch.cuda.empty_cache()
+    rend.set_launch()
+    ora = Oracle(threads=max(1, min(16, len(os.sched_getaffinity(0)))))
+    t0 = time.perf_counter()
+    refs = [ora.render_rays(s, ix.astype(np.uint64), state=False) for s, ix in zip(scenes, idxs)]
+    dt = time.perf_counter() - t0
+    cat = lambda parts: {k: np.concatenate([p[k] for p in parts]) for k in ("cls", "r", "theta", "phi", "attempts")}
+    ref_all, band_all = cat(refs), np.concatenate(bands)
+    worst = {p: min(_vs_oracle(g, r)["err_le_1e-6"] for g, r in zip(sub[p], refs)) for p in sub}
+    return {"workload": "64 seeded Kerr scenes (generator seed 2026) in one graph-captured launch",
+            "launch": "64 threads x 8 blocks per CU, refill 64, learned order", "rays": n,
+            "fp64": res["fp64"], "mixed": res["mixed"],
+            "mixed_speedup": res["fp64"]["kernel_ms"] / res["mixed"]["kernel_ms"],
+            "fp64_vs_oracle": {**_vs_oracle(cat(sub["fp64"]), ref_all, band_all),
+                               "worst_scene_err_le_1e-6": worst["fp64"]},
+            "mixed_vs_oracle": {**_vs_oracle(cat(sub["mixed"]), ref_all, band_all),
+                                "worst_scene_err_le_1e-6": worst["mixed"]},
+            "oracle_sample": f"every {ora_stride}th row and column of each scene ({len(band_all)} rays, {dt:.1f} s "
+                             f"on {ora.threads} threads)"}
+
+
+def block_adjoint(torch, rend, steps=5, warmup=2, fd_size=40):
+    """Adjoint gradients/s on the 512^2 differentiable render (mixed FP32/FP64 forward recording every attempt,
+    learned ray order, target resident on the device), and the gradient's error against central differences of
+    the oracle's loss on the same scene at fd_size^2 (h = 1e-6 over every ray; h = 1e-5 over the rays whose step
+    decisions do not flip)."""
+    import gargantua_amd as gg
+    from oracle import Oracle
+    from tests.gradcheck import central_fd, masked_gradient
+    cfg = gg.configs
+    tgt = cfg.grad_512_params(512, 512, spin=0.7, inclination_deg=60.0, q=2.5, r_in=cfg.r_isco(1.0, 0.7))
+    target = torch.from_numpy(rend.grad_image(tgt)).to("cuda")
+    gm = cfg.grad_512_params(512, 512, precision="mixed")
+    rend.set_launch(order=1)
+    for _ in range(warmup):
+        rend.grad(gm, target, checkpoint_every=0, want_image=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sts = []
+    for _ in range(steps):
+        out = rend.grad(gm, target, checkpoint_every=0, want_image=False)
+        sts.append(out["stats"])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rend.set_launch()
+    st = sts[-1]
+    # finite differences of the oracle's loss on the small scene
+    ora = Oracle(threads=max(1, min(16, len(os.sched_getaffinity(0)))))
+    mk = lambda: cfg.grad_512_params(fd_size, fd_size)
+    t_small = ora.grad(cfg.grad_512_params(fd_size, fd_size, spin=0.7, inclination_deg=60.0, q=2.5,
+                                           r_in=cfg.r_isco(1.0, 0.7)), np.zeros(fd_size * fd_size))["intensity"]
+    fd6 = central_fd(ora, mk, t_small, 1e-6)
+    fd5 = central_fd(ora, mk, t_small, 1e-5)
+    fd_err = {}
+    for prec in ("fp64", "mixed"):
+        a = rend.grad(cfg.grad_512_params(fd_size, fd_size, precision=prec), t_small, want_dI=True)
+        ad5 = masked_gradient(a["dI"], a["intensity"], t_small, fd5["mask"])
+        fd_err[prec] = {"h1e-6_max_rel": float(np.max(np.abs(a["grad"] - fd6["fd"]) / np.abs(fd6["fd"]))),
+                        "h1e-5_smooth_rays_max_rel": float(np.max(np.abs(ad5 - fd5["fd_smooth"]) /
+                                                                  np.abs(fd5["fd_smooth"]))),
+                        "grad": a["grad"].tolist()}
+    return {"workload": "512x512 Kerr thin disk r_out 20M edge 0.25M, camera r=50M FOV 30deg, rtol 1e-8 atol 1e-10; "
+                        "theta0 (0.5, 45deg, 3.0, r_ISCO+0.5), target (0.7, 60deg, 2.5, r_ISCO(0.7))",
+            "grads_per_s": steps / wall, "ms_per_grad": 1e3 * wall / steps,
+            "forward_ms": float(np.mean([x["forward_ms"] for x in sts])),
+            "adjoint_ms": float(np.mean([x["adjoint_ms"] for x in sts])),
+            "mode": st["mode"], "checkpoint_every": st["checkpoint_every"], "forward_precision": "mixed",
+            "disk_rays": st["disk_rays"], "attempts_backpropagated": st["steps"],
+            "step_sizes": "differentiated through the error estimate and controller",
+            "fd_check": {"scene": f"the same scene at {fd_size}x{fd_size}", "fd_h1e-6": fd6["fd"].tolist(),
+                         "fd_h1e-5_smooth_rays": fd5["fd_smooth"].tolist(),
+                         "rays_with_decision_flips_h1e-5": fd5["flips"].tolist(), **fd_err}}
 
 
 def bench_batch64(args):
@@ -485,6 +675,8 @@ def main():
                     help="copy: one 512 MiB device-to-device copy, the reference kernel that calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters for HBM traffic")
     ap.add_argument("--gd-steps", type=int, default=200, help="grad512: gradient-descent iterations (0: none)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="kerr1024: only the 1024^2 workload (no 4096^2 / batch frames, no adjoint block)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
@@ -595,9 +787,23 @@ def main():
                  "attempts_per_ray": st_m["attempts"] / n},
         "cpu_baseline": None,
     }
+    line["first_frame_value"] = n / (tile_ms["mixed"] * 1e-3)
+    line["first_frame_note"] = ("value is a repeat frame of the same scene under the ray order learned from its "
+                                "warm-up; first_frame_value is the first frame in 8x8-tile order (kernel time)")
     line["constants_of_motion_drift"] = {p: com_drift(rend, scene, p) for p in ("fp64", "mixed")}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(scene)
+        line["cpu_baseline"], ref = cpu_baseline(scene)
+        # every ray of the workload against the FP64 oracle
+        shape = (scene.height, scene.width)
+        keys = ("class_agree", "class_agree_outside_band", "err_le_1e-6", "err_le_1e-6_outside_band", "err_median",
+                "attempts_equal", "psnr_intensity", "state_bit_identical")
+        line["fp64_vs_oracle"] = {k: v for k, v in compare(hd, ref, shape=shape).items() if k in keys}
+        line["mixed_vs_oracle"] = {k: v for k, v in compare(hm, ref, shape=shape).items() if k in keys}
+    if not args.headline_only:
+        # the other workloads' frames and the adjoint, each against the oracle
+        line["workloads"] = {"kerr4096": frame_kerr4096(torch, rend), "batch64": frame_batch64(torch, rend),
+                             "adjoint": block_adjoint(torch, rend)}
+        line["adjoint_grads_per_s"] = line["workloads"]["adjoint"]["grads_per_s"]
     rend.close()
     print(json.dumps(line))
 

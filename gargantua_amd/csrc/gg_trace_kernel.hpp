@@ -121,32 +121,42 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
     active = true;
   };
 
-  __shared__ uint32_t s_cnt[2][4];
+  __shared__ uint32_t s_cnt[3][4];
   __shared__ unsigned long long s_base;
   uint32_t iter = 0;
   for (;;) {
     {
       // ---- block regroup: compaction (and precision sorting when mixed) ----
       // Every refill_interval attempts the block's wavefronts exchange ray
-      // states through LDS: live rays move to the lowest lanes (for the mixed
-      // kernel FP32-eligible rays first, so that whole wavefronts can take
-      // FP32 steps), retired lanes gather at the tail and take new rays with
-      // one atomicAdd for the block.
+      // states through LDS: live rays move to the lowest lanes, retired lanes
+      // gather at the tail and take new rays with one atomicAdd for the
+      // block.  The mixed kernel orders the live rays by the arithmetic their
+      // next attempts can take -- FP32-eligible, then fast-FP64-able, then
+      // those that need exact FP64 (an unsaturated controller with a tiny
+      // previous error, where the error's last bit decides the next h) -- so
+      // that wavefronts are uniform: a wavefront takes the costliest tier
+      // any of its lanes needs.  Rays keep their class for many attempts.
       if (iter % T.refill_interval == 0) {
         const uint32_t tid = threadIdx.x, B = blockDim.x, wave = tid >> 6;
-        const uint32_t key = !active ? 2u : ((PREC == 1 && fp32_eligible(*Sp, L)) ? 0u : 1u);
-        const unsigned long long m0 = __ballot(key == 0), m1 = __ballot(key == 1);
-        if (lane == 0) { s_cnt[0][wave] = __popcll(m0); s_cnt[1][wave] = __popcll(m1); }
+        const uint32_t key = !active ? 3u
+                             : (PREC != 1 ? 1u
+                                          : (fp32_eligible(*Sp, L) ? 0u
+                                                                   : ((L.h < Sp->max_step && L.err_prev < 1e-6) ? 2u : 1u)));
+        const unsigned long long m0 = __ballot(key == 0), m1 = __ballot(key == 1), m2 = __ballot(key == 2);
+        if (lane == 0) { s_cnt[0][wave] = __popcll(m0); s_cnt[1][wave] = __popcll(m1); s_cnt[2][wave] = __popcll(m2); }
         __syncthreads();
-        uint32_t n0 = 0, n1 = 0, o0 = 0, o1 = 0;
+        uint32_t n0 = 0, n1 = 0, n2 = 0, o0 = 0, o1 = 0, o2 = 0;
         for (uint32_t w = 0; w < (B >> 6); ++w) {
-          if (w < wave) { o0 += s_cnt[0][w]; o1 += s_cnt[1][w]; }
+          if (w < wave) { o0 += s_cnt[0][w]; o1 += s_cnt[1][w]; o2 += s_cnt[2][w]; }
           n0 += s_cnt[0][w];
           n1 += s_cnt[1][w];
+          n2 += s_cnt[2][w];
         }
         const unsigned long long below = (1ull << lane) - 1ull;
-        const uint32_t dst = key == 0 ? o0 + __popcll(m0 & below) : n0 + o1 + __popcll(m1 & below);
-        const uint32_t n_act = n0 + n1;
+        const uint32_t dst = key == 0   ? o0 + __popcll(m0 & below)
+                             : key == 1 ? n0 + o1 + __popcll(m1 & below)
+                                        : n0 + n1 + o2 + __popcll(m2 & below);
+        const uint32_t n_act = n0 + n1 + n2;
         // The exchange reuses stage slots 1..4 (free between attempts) as
         // 16 planes of B doubles; two passes of at most 14 fields.
         double *X = lds_k + 4 * B;
