@@ -93,31 +93,37 @@ __device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const
   const d4 rv{rec[REC_V], rec[REC_V + 1], rec[REC_V + 2], rec[REC_V + 3]};
   const d4 v = acc ? rv : A.gv;
   const d4 nv = acc ? A.gv : rv;
+  // derivative coefficients take reciprocals to ~1 ulp (v_rcp_f64 + Newton)
+  const double ih = rcp64(h);
 
   // ---- controller (geodesic.rs:318-325): cotangents of h and of the error ----
   double hbar = 0.0, eb = 0.0;
   const d4 dv{rec[REC_DV], rec[REC_DV + 1], rec[REC_DV + 2], rec[REC_DV + 3]};
-  double err = 0.0;
+  double err = 0.0, ierr = 0.0;
+  double isc[4] = {0.0, 0.0, 0.0, 0.0};   // 1 / (atol + rtol max(|v_c|, |new_vel_c|))
   if (live && A.hb != 0.0) {
     if (S.err_mode == 1) {
-      const double q0 = dv.x / (S.atol + S.rtol * fmax(fabs(v.x), fabs(nv.x)));
-      const double q1 = dv.y / (S.atol + S.rtol * fmax(fabs(v.y), fabs(nv.y)));
-      const double q2 = dv.z / (S.atol + S.rtol * fmax(fabs(v.z), fabs(nv.z)));
-      const double q3 = dv.w / (S.atol + S.rtol * fmax(fabs(v.w), fabs(nv.w)));
+      isc[0] = rcp64(S.atol + S.rtol * fmax(fabs(v.x), fabs(nv.x)));
+      isc[1] = rcp64(S.atol + S.rtol * fmax(fabs(v.y), fabs(nv.y)));
+      isc[2] = rcp64(S.atol + S.rtol * fmax(fabs(v.z), fabs(nv.z)));
+      isc[3] = rcp64(S.atol + S.rtol * fmax(fabs(v.w), fabs(nv.w)));
+      const double q0 = dv.x * isc[0], q1 = dv.y * isc[1], q2 = dv.z * isc[2], q3 = dv.w * isc[3];
       err = sqrt(0.25 * (q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3));
     } else {
       err = sqrt(dv.x * dv.x + dv.y * dv.y + dv.z * dv.z + dv.w * dv.w);
     }
+    // err is 0 exactly when every component of new_vel - vel_err is 0
+    if (err != 0.0) ierr = rcp64(err);
     if (acc) {
       if (err == 0.0) {
         hbar = A.hb;                                   // h unchanged
       } else if (hn != S.max_step && hn != S.min_step) {
-        hbar = A.hb * hn / h;                          // h' = 0.9 h err^-0.2
-        eb = -0.2 * A.hb * hn / err;
+        hbar = A.hb * hn * ih;                         // h' = 0.9 h err^-0.2
+        eb = -0.2 * A.hb * hn * ierr;
       }
     } else if (hn != S.min_step) {
-      hbar = A.hb * hn / h;                            // h' = 0.9 h err^-0.25
-      eb = -0.25 * A.hb * hn / err;
+      hbar = A.hb * hn * ih;                           // h' = 0.9 h err^-0.25
+      eb = -0.25 * A.hb * hn * ierr;
     }
   }
   if (!acc && eb == 0.0) { A.hb = hbar; return; }
@@ -149,11 +155,10 @@ __device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const bool use_v = fabs(vv[c]) >= fabs(nn[c]);
-        const double sc = S.atol + S.rtol * (use_v ? fabs(vv[c]) : fabs(nn[c]));
-        const double q = dd[c] / sc;
-        const double qb = 0.25 * eb * q / err;
-        o[c] = qb / sc;
-        const double scb = -qb * q / sc * S.rtol;     // through max(|v_c|, |new_vel_c|)
+        const double q = dd[c] * isc[c];
+        const double qb = 0.25 * eb * q * ierr;
+        o[c] = qb * isc[c];
+        const double scb = -qb * q * isc[c] * S.rtol;  // through max(|v_c|, |new_vel_c|)
         if (use_v) vs_[c] = vv[c] > 0.0 ? scb : (vv[c] < 0.0 ? -scb : 0.0);
         else ns_[c] = nn[c] > 0.0 ? scb : (nn[c] < 0.0 ? -scb : 0.0);
       }
@@ -161,7 +166,7 @@ __device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const
       vbar.x += vs_[0]; vbar.y += vs_[1]; vbar.z += vs_[2]; vbar.w += vs_[3];
       nvb.x += ns_[0]; nvb.y += ns_[1]; nvb.z += ns_[2]; nvb.w += ns_[3];
     } else {
-      const double k = eb / err;
+      const double k = eb * ierr;
       db = d4{k * dv.x, k * dv.y, k * dv.z, k * dv.w};
     }
     nvb.x += db.x; nvb.y += db.y; nvb.z += db.z; nvb.w += db.w;
@@ -175,7 +180,7 @@ __device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const
     pth += jb.th;
     vbar.y += h * jb.r;
     vbar.z += h * jb.th;
-    hbar += jb.r * v.y + jb.th * v.z + 0.5 * (jb.v.x * nv.x + jb.v.y * nv.y + jb.v.z * nv.z + jb.v.w * nv.w) / h;
+    hbar += jb.r * v.y + jb.th * v.z + 0.5 * ih * (jb.v.x * nv.x + jb.v.y * nv.y + jb.v.z * nv.z + jb.v.w * nv.w);
     nvb.x += jb.v.x; nvb.y += jb.v.y; nvb.z += jb.v.z; nvb.w += jb.v.w;
   }
   // new_vel = v + h sum_i c_i k_i, vel_err = v + h sum_i d_i k_i
@@ -205,7 +210,8 @@ __device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const
     pr += jb.r;
     pth += jb.th;
     vbar.x += jb.v.x; vbar.y += jb.v.y; vbar.z += jb.v.z; vbar.w += jb.v.w;
-    if (live) hbar += 0.5 * (jb.v.x * vs.x + jb.v.y * vs.y + jb.v.z * vs.z + jb.v.w * vs.w) / h;
+    // the combination's h-derivative (Euler: w . a(x, u) = (J_u^T w) . u / 2)
+    if (live) hbar += 0.5 * ih * (jb.v.x * vs.x + jb.v.y * vs.y + jb.v.z * vs.z + jb.v.w * vs.w);
     if (i >= 2) {
       vbar.y += ha * jb.r;
       vbar.z += ha * jb.th;
