@@ -1,13 +1,14 @@
 // gg_adjoint.hpp -- device pieces of the hand-written adjoint (gg_grad).
 //
-// Discretise-then-differentiate: the step-size sequence, the accept pattern
-// and the disk-crossing step of every ray are those of the forward render
-// and are held fixed; the adjoint differentiates the map from (spin,
-// inclination, emissivity index q, inner radius r_in) to each ray's disk
-// intensity through the camera ray, every accepted Dormand-Prince step and
-// the crossing.  The per-stage RHS vector-Jacobian product is reverse mode
-// (gg_vjp.hpp); the crossing and the camera ray, evaluated once per ray,
-// are differentiated here in forward mode with one-tangent dual numbers.
+// The adjoint differentiates the map from (spin, inclination, emissivity
+// index q, inner radius r_in) to each ray's disk intensity through the camera
+// ray, every attempt of the Dormand-Prince integrator (its stages, error
+// estimate and step-size controller, or with the step sizes held fixed) and
+// the disk crossing; the discrete decisions (accept/reject, clamps, the
+// crossing step) are those of the forward render.  The per-stage RHS
+// vector-Jacobian product is reverse mode (gg_vjp.hpp); the crossing and the
+// camera ray, evaluated once per ray, are differentiated here in forward mode
+// with one-tangent dual numbers.
 #pragma once
 
 #include "gg_physics.hpp"
@@ -115,7 +116,7 @@ __device__ inline Tn kerr_redshift_d(double m, Tn a, Tn r, Tn p0, Tn p3, Tn sqrt
 // Camera ray of pixel (px, py) of the inclination camera
 // (t, r, theta, phi) = (0, r_cam, i, 0) looking at -r_hat, up = +z, with one
 // tangent through (a, i): Camera::pixel_direction and from_camera
-// (camera.rs:20-29, ray.rs:44-91).  Returns (k^t, k^r, k^theta, k^phi).
+// (camera.rs:20-29, ray.rs:14-61).  Returns (k^t, k^r, k^theta, k^phi).
 struct CamD {
   Tn k[4];
 };

@@ -431,7 +431,7 @@ void build_dev_scene(const gg_scene &s, DevScene &D, const UserMetricAt *um = nu
   D.tol = s.integrator.tolerance;
   D.rtol = s.integrator.rtol;
   D.atol = s.integrator.atol;
-  // camera basis (camera.rs:21-26) and spherical frame at the camera (ray.rs:59-65)
+  // camera basis (camera.rs:21-26) and spherical frame at the camera (ray.rs:29-35)
   const gg_camera &c = s.camera;
   for (int i = 0; i < 4; ++i) D.cam_pos[i] = c.position[i];
   double fwd[3] = {c.look_at[0], c.look_at[1], c.look_at[2]};

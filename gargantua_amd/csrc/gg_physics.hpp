@@ -473,7 +473,7 @@ __device__ inline double redshift64(const DevScene &S, double r, double theta, d
   return e_emit > 0.0 ? e_obs / e_emit : 0.0;
 }
 
-// AccretionDisk::density (disk.rs:154-173).
+// AccretionDisk::density (disk.rs:40-59).
 __device__ inline double disk_density(const DevScene &S, double r, double theta) {
 #pragma clang fp contract(off)
   double st, ct;
@@ -487,7 +487,7 @@ __device__ inline double disk_density(const DevScene &S, double r, double theta)
   return S.density_0 * radial * vertical;
 }
 
-// AccretionDisk::emission given the density just computed (disk.rs:179-195).
+// AccretionDisk::emission given the density just computed (disk.rs:65-81).
 __device__ inline double disk_emission(const DevScene &S, double rho, double r, double theta) {
 #pragma clang fp contract(off)
   if (rho == 0.0) return 0.0;
@@ -502,7 +502,7 @@ __device__ inline double disk_emission(const DevScene &S, double rho, double r, 
 // Constants of motion of a geodesic at (pos, vel): E = -p_t, L_z = p_phi,
 // Carter's Q = p_theta^2 + cos^2(theta) (a^2 (mu^2 - E^2) + L_z^2 / sin^2(theta))
 // and the mass shell mu^2 = -g(v, v) (0 for a null ray; the camera rays of
-// ray.rs:81-83 drop g_tphi, so Kerr camera rays carry mu^2 = -2 g_tphi k^t k^phi).
+// ray.rs:51-53 drop g_tphi, so Kerr camera rays carry mu^2 = -2 g_tphi k^t k^phi).
 // Exact for the geodesic flow; the integrator's forward-Euler positions
 // make them drift at O(h), which is what they diagnose.
 template <int METRIC>
@@ -530,7 +530,7 @@ __device__ inline uint64_t splitmix64(uint64_t seed, uint64_t n) {
 }
 
 // Camera::pixel_direction + GeodesicRay::from_camera (camera.rs:20-29,
-// ray.rs:44-91) with the per-scene basis hoisted to the host.
+// ray.rs:14-61) with the per-scene basis hoisted to the host.
 __device__ inline void camera_ray(const DevScene &S, uint32_t px, uint32_t py, double pos[4], d4 &vel) {
 #pragma clang fp contract(off)
   double fx = (double)px + 0.5, fy = (double)py + 0.5;

@@ -29,7 +29,7 @@
  *   gg_grad_scene_to_scene, gg_geodesic_acceleration_vjp
  *                        no reference counterpart: the scene gg_grad renders
  *                        and the RHS vector-Jacobian product it uses.
- *   gg_camera_rays       GeodesicRay::from_camera (ray.rs:44-91) per pixel.
+ *   gg_camera_rays       GeodesicRay::from_camera (ray.rs:14-61) per pixel.
  *   gg_constants_of_motion
  *                        no reference counterpart: E, L_z, Carter Q, mu^2.
  *
@@ -136,7 +136,7 @@ typedef struct gg_image {
   uint32_t width;
   uint32_t height;
   uint32_t samples_per_axis; /* RenderOptions::samples_per_axis (>= 1)        */
-  uint32_t jitter;           /* 0: pixel centres (ray.rs:52-53); 1: centre +
+  uint32_t jitter;           /* 0: pixel centres (ray.rs:22-23); 1: centre +
                                 U[-0.5,0.5)^2 from SplitMix64(seed, ray)      */
   uint64_t seed;
 } gg_image;
@@ -295,7 +295,7 @@ int gg_geodesic_acceleration(gg_context *ctx, const gg_metric *metric,
                              int precision, uint64_t n, const double *pos,
                              const double *vel, double *acc);
 
-/* GeodesicRay::from_camera (crates/raytracer/src/curved/ray.rs:44-91) for
+/* GeodesicRay::from_camera (crates/raytracer/src/curved/ray.rs:14-61) for
  * every sub-pixel ray of the scene: states[9*i..] = (t r th ph vt vr vth vph
  * lambda=0), i = sub_py*(width*spa) + sub_px (with jitter when set). */
 int gg_camera_rays(gg_context *ctx, const gg_scene *scene, double *states);
@@ -349,16 +349,17 @@ typedef struct gg_grad_stats {
   double total_ms;          /* whole call                                     */
   uint64_t rays;            /* width*height                                   */
   uint64_t disk_rays;       /* rays differentiated (class GG_CLASS_DISK)      */
-  uint64_t steps;           /* accepted steps back-propagated                 */
+  uint64_t steps;           /* step records (attempts) back-propagated        */
   uint64_t attempts;        /* attempts re-run by the adjoint (re-trace +
                                segment recomputation)                         */
   uint64_t checkpoints;     /* checkpoints stored                             */
   uint64_t checkpoint_bytes;
   uint64_t segment_bytes;   /* per-thread segment buffers of the reverse pass */
-  uint64_t record_bytes;    /* step records stored (256 bytes each): each disk
-                               ray's last segment by the re-trace (modes 0, 1),
-                               or every ray's steps in 4224-byte chunks of 16
-                               by the render (mode 2)                         */
+  uint64_t record_bytes;    /* step records stored (256 bytes each, one per
+                               attempt): room for each disk ray's last segment
+                               by the re-trace (modes 0, 1), or every ray's
+                               attempts in 4224-byte chunks of 16 by the
+                               render (mode 2)                                */
   uint32_t grid_blocks;      /* reverse kernel launch                        */
   uint32_t block_threads;
   uint32_t checkpoint_every; /* interval used (mode 1: at least the longest
@@ -371,15 +372,19 @@ typedef struct gg_grad_stats {
 /* Loss L = mean_p (I_p - target_p)^2 over the width*height pixel-centre rays
  * of gg_grad_scene_to_scene(scene), rendered in FP64.  grad[0..3] =
  * dL/d(spin, inclination, emissivity_q, r_inner) of that discrete algorithm
- * with each ray's step sizes, accept/reject pattern and disk-crossing step
- * held fixed (reverse mode; checkpoints every checkpoint_every accepted
- * steps, segments recomputed between them; the result does not depend on
- * checkpoint_every).  checkpoint_every = 0: the render records every
- * accepted step of every ray (256 bytes each, in chunks of 16) and nothing
+ * (reverse mode): each attempt's step size is the controller's function of
+ * the previous step size and error estimate (geodesic.rs:308-325) and
+ * carries its derivative, or, with scene->frozen_steps, is held at the
+ * render's value; the accept/reject pattern, the clamps of h and the
+ * disk-crossing step are the render's.  Checkpoints every checkpoint_every
+ * accepted steps, segments recomputed between them; the result does not
+ * depend on checkpoint_every.  checkpoint_every = 0: the render records
+ * every attempt of every ray (256 bytes each, in chunks of 16) and nothing
  * is re-traced or recomputed, when the records fit in the memory of
- * gg_set_grad_memory (default a quarter of the free device memory); else a re-trace stores every disk ray's step records when
- * those fit, else checkpoints every 32 steps.  An interval at least the
- * longest ray's accepted count also stores every step record (re-trace).
+ * gg_set_grad_memory (default a quarter of the free device memory); else a
+ * re-trace stores every disk ray's step records when those fit, else
+ * checkpoints every 32 accepted steps.  An interval at least the longest
+ * ray's accepted count also stores every step record (re-trace).
  * Only rays of class GG_CLASS_DISK carry a gradient.
  * target: [N] host or device.  Optional outputs: intensity [N] (the image),
  * d_intensity [4*N] (dI_p/d(spin, inclination, q, r_inner), AoS).

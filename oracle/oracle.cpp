@@ -6,10 +6,10 @@
 //
 // Reference files restated here (paths relative to the reference root):
 //   crates/raytracer/src/curved/camera.rs:20-29     pixel_direction
-//   crates/raytracer/src/curved/ray.rs:44-91        GeodesicRay::from_camera
+//   crates/raytracer/src/curved/ray.rs:14-61        GeodesicRay::from_camera
 //   crates/raytracer/src/curved/tracer.rs:54-152    trace_ray, trace_ray_with_disk
 //   crates/raytracer/src/curved/tracer.rs:185-189   spatial_dir
-//   crates/raytracer/src/curved/disk.rs:154-195     density, emission
+//   crates/raytracer/src/curved/disk.rs:40-81     density, emission
 //   crates/raytracer/src/curved/renderer.rs:44-80   shade_outcome_with, disk_color, encode_srgb
 //   crates/raytracer/src/curved/renderer.rs:256-345 render_inner, render_row
 //   crates/raytracer/src/curved/sky.rs:25-41        ProceduralSky::sample
@@ -87,7 +87,7 @@ inline uint64_t splitmix64(uint64_t seed, uint64_t n) {
 }
 inline double unit53(uint64_t x) { return (double)(x >> 11) * (1.0 / 9007199254740992.0); }
 
-// GeodesicRay::from_camera (ray.rs:44-91); with jitter the pixel centre
+// GeodesicRay::from_camera (ray.rs:14-61); with jitter the pixel centre
 // offset +0.5 becomes +0.5 + j, j = U[-0.5, 0.5) from SplitMix64(seed, ray).
 template <class T>
 State<T> from_camera(const MetricP &mp, const T &a, const CameraT<T> &cam, const gg_image &img, uint32_t px,
@@ -99,8 +99,8 @@ State<T> from_camera(const MetricP &mp, const T &a, const CameraT<T> &cam, const
     fx = fx + (unit53(splitmix64(img.seed, 2 * k + 1)) - 0.5);
     fy = fy + (unit53(splitmix64(img.seed, 2 * k + 2)) - 0.5);
   }
-  double u = fx / (double)width * 2.0 - 1.0;   // ray.rs:52
-  double v = 1.0 - fy / (double)height * 2.0;  // ray.rs:53
+  double u = fx / (double)width * 2.0 - 1.0;   // ray.rs:22
+  double v = 1.0 - fy / (double)height * 2.0;  // ray.rs:23
   V3<T> n = pixel_direction(cam, u, v);
   V4<T> pos = cam.pos;
   T theta = pos[2], phi = pos[3];
@@ -113,8 +113,8 @@ State<T> from_camera(const MetricP &mp, const T &a, const CameraT<T> &cam, const
   T kr = n_r / sqrt(g.rr);
   T kth = n_t / sqrt(g.thth);
   T kph = n_p / sqrt(g.phph);
-  T spatial = g.rr * kr * kr + g.thth * kth * kth + g.phph * kph * kph;  // ray.rs:81-82
-  T kt = sqrt(-spatial / g.tt);                                          // ray.rs:83
+  T spatial = g.rr * kr * kr + g.thth * kth * kth + g.phph * kph * kph;  // ray.rs:51-52
+  T kt = sqrt(-spatial / g.tt);                                          // ray.rs:53
   State<T> s;
   s.pos = pos;
   s.vel[0] = kt; s.vel[1] = kr; s.vel[2] = kth; s.vel[3] = kph;
@@ -137,7 +137,7 @@ template <class T> DiskT<T> disk_from(const gg_disk &d) {
   return k;
 }
 
-// AccretionDisk::density (disk.rs:154-173).
+// AccretionDisk::density (disk.rs:40-59).
 double disk_density(const DiskT<double> &d, const V4<double> &pos) {
   double r = pos[1], theta = pos[2];
   double s = std::sin(theta), c = std::cos(theta);
@@ -150,7 +150,7 @@ double disk_density(const DiskT<double> &d, const V4<double> &pos) {
   return d.density_0 * radial * vertical;
 }
 
-// AccretionDisk::emission (disk.rs:179-195).
+// AccretionDisk::emission (disk.rs:65-81).
 double disk_emission(const DiskT<double> &d, const V4<double> &pos) {
   double rho = disk_density(d, pos);
   if (rho == 0.0) return 0.0;

@@ -49,7 +49,7 @@ def test_gpu_matches_golden(renderer, name, precision):
     if escaped_px.any():
         assert np.mean(close[escaped_px]) >= 0.9
     # Disk intensities within 1e-6 relative.  The volumetric disk weighs each
-    # step by exp(-z^2 / 2H^2) (disk.rs:170), whose log-derivative is -z/H^2:
+    # step by exp(-z^2 / 2H^2) (disk.rs:56), whose log-derivative is -z/H^2:
     # the ~1e-7 position noise of the mixed kernel's FP32 steps inside the
     # disk band becomes ~1e-6 of intensity there, so mixed volumetric scenes
     # are held to 1e-5.

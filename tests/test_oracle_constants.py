@@ -46,7 +46,7 @@ def test_equatorial_ray_has_zero_carter_q(oracle):
 
 
 def test_camera_ray_mass_shell(oracle):
-    """ray.rs:81-83 sets k^t from the diagonal metric only: null at a = 0,
+    """ray.rs:51-53 sets k^t from the diagonal metric only: null at a = 0,
     mu^2 = -2 g_tphi k^t k^phi for Kerr."""
     for spin in (0.0, 0.9):
         sc = _scene(spin, 1.0)
