@@ -129,6 +129,8 @@ EXPORTS = [
     ("gg_render", C.c_int, [_P, C.POINTER(gg_scene), C.c_int, C.POINTER(gg_outputs), C.POINTER(gg_stats)]),
     ("gg_render_batch", C.c_int, [_P, C.POINTER(gg_scene), C.c_uint32, C.c_int, C.POINTER(gg_outputs),
                                   C.POINTER(gg_stats)]),
+    ("gg_render_batch_noalloc", C.c_int, [_P, C.POINTER(gg_scene), C.c_uint32, C.c_int, C.POINTER(gg_outputs),
+                                          C.POINTER(gg_stats)]),
     ("gg_trace_states", C.c_int, [_P, C.POINTER(gg_scene), C.c_int, C.c_uint64, _P, C.POINTER(gg_outputs),
                                   C.POINTER(gg_stats)]),
     ("gg_geodesic_acceleration", C.c_int, [_P, C.POINTER(gg_metric), C.c_int, C.c_uint64, _P, _P, _P]),

@@ -73,11 +73,47 @@ struct DevBuf {
   }
 };
 
+// ---- the scratch of gg_render_batch_noalloc ----------------------------------
+// One static device array per GPU: the trace kernel's counters, the scenes'
+// constants and their output tables, each at a 256-byte boundary.  In host
+// code `scratch` names a host-side stand-in: the device address is taken with
+// hipGetSymbolAddress once per device (scratch_address) and only that address
+// is handed to copies and kernels.
+constexpr size_t SCRATCH_COUNTERS = 0;
+constexpr size_t SCRATCH_SCENES = 256;
+constexpr size_t SCRATCH_OUTS = SCRATCH_SCENES + ((sizeof(DevScene) * GG_NOALLOC_MAX_SCENES + 255) / 256) * 256;
+constexpr size_t SCRATCH_END = SCRATCH_OUTS + ((sizeof(DevOut) * GG_NOALLOC_MAX_SCENES + 255) / 256) * 256;
+static_assert(sizeof(Counters) <= SCRATCH_SCENES, "counters fit their slot");
+static_assert(SCRATCH_END <= GG_NOALLOC_SCRATCH_BYTES, "scratch layout fits the stated maximum");
+static __device__ __align__(256) unsigned char scratch[GG_NOALLOC_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[SCRATCH_MAX_DEVICES] = {};
+
+// The device address of `scratch` on `device` (which must be current), taken
+// once per device.
+int scratch_address(int device, void **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= SCRATCH_MAX_DEVICES)
+    return fail(GG_ERR_UNSUPPORTED, "device %d beyond the %d the scratch table holds", device, SCRATCH_MAX_DEVICES);
+  if (!g_scratch_addr[device]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(GG_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_addr[device] = p;
+  }
+  *out = g_scratch_addr[device];
+  return GG_OK;
+}
+
 }  // namespace
 
 struct gg_context {
   int device = 0;
   hipStream_t stream = nullptr;
+  void *scratch = nullptr;   // this device's address of `scratch` (gg_render_batch_noalloc)
   bool own_stream = false;
   uint32_t num_cus = 256;
   gg_launch launch{64, 8, 64, 1, 0};
@@ -573,6 +609,7 @@ int gg_context_create(int device, void *stream, gg_context **out) {
   }
   int rc = c->counters.ensure(sizeof(Counters));
   if (rc) { delete c; return rc; }
+  if ((rc = scratch_address(device, &c->scratch))) { delete c; return rc; }
   *out = c;
   return GG_OK;
 }
@@ -1109,6 +1146,109 @@ int gg_render(gg_context *ctx, const gg_scene *scene, int precision, const gg_ou
 int gg_render_batch(gg_context *ctx, const gg_scene *scenes, uint32_t n, int precision, const gg_outputs *out,
                     gg_stats *stats) {
   return render_impl(ctx, scenes, n, precision, out, nullptr, 0, stats);
+}
+
+// gg_render_batch without allocating: the per-call device data go to this
+// GPU's `scratch` (its address taken by gg_context_create), host-side tables
+// live on the stack, the outputs are the caller's device buffers, and the
+// launch is direct.  No hipMalloc, hipFree or allocating library call.
+int gg_render_batch_noalloc(gg_context *ctx, const gg_scene *scenes, uint32_t n, int precision,
+                            const gg_outputs *outs, gg_stats *stats) {
+  auto t_start = std::chrono::steady_clock::now();
+  if (!ctx) return fail(GG_ERR_INVALID_ARG, "context is NULL");
+  if (!scenes || n == 0) return fail(GG_ERR_INVALID_ARG, "no scenes");
+  if (!outs) return fail(GG_ERR_INVALID_ARG, "outputs are NULL");
+  if (n > GG_NOALLOC_MAX_SCENES)
+    return fail(GG_ERR_INVALID_ARG, "gg_render_batch_noalloc takes at most %d scenes (%u given): its scratch holds "
+                                    "%d scenes' tables", GG_NOALLOC_MAX_SCENES, n, GG_NOALLOC_MAX_SCENES);
+  if (precision < GG_PREC_FP64 || precision > GG_PREC_FP64_FAST)
+    return fail(GG_ERR_INVALID_ARG, "precision %d unknown", precision);
+  if (!ctx->scratch) return fail(GG_ERR_INTERNAL, "no scratch address for device %d", ctx->device);
+  int rc;
+  for (uint32_t i = 0; i < n; ++i) {
+    if ((rc = validate_scene(&scenes[i], true))) return rc;
+    if (scenes[i].metric.kind == GG_METRIC_USER)
+      return fail(GG_ERR_UNSUPPORTED, "gg_render_batch_noalloc: a user metric compiles its kernel (allocates)");
+    if (scenes[i].metric.kind != scenes[0].metric.kind || scenes[i].disk.kind != scenes[0].disk.kind)
+      return fail(GG_ERR_UNSUPPORTED, "a batch must share metric kind and disk kind (scene %u differs)", i);
+  }
+  if (ctx->n_hit > 0 && !ctx->hit_off)
+    return fail(GG_ERR_UNSUPPORTED, "gg_render_batch_noalloc: hittables are set (scene mode renders one scene per "
+                                    "gg_render call)");
+  HIP_TRY(hipSetDevice(ctx->device));
+  DevScene ds[GG_NOALLOC_MAX_SCENES];
+  DevOut dout[GG_NOALLOC_MAX_SCENES];
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const gg_outputs &o = outs[i];
+    if (o.rgb || o.color)
+      return fail(GG_ERR_UNSUPPORTED, "gg_render_batch_noalloc: rgb and color outputs need staging buffers");
+    const void *ptrs[8] = {o.cls, o.r, o.theta, o.phi, o.intensity, o.attempts, o.accepted, o.state};
+    for (const void *p : ptrs)
+      if (p && !is_device_ptr(p))
+        return fail(GG_ERR_INVALID_ARG, "gg_render_batch_noalloc: scene %u has a host output pointer (device "
+                                        "memory only)", i);
+    build_dev_scene(scenes[i], ds[i]);
+    const uint64_t rays = (uint64_t)ds[i].sub_w * ds[i].sub_h;
+    ds[i].ray_begin = total;
+    ds[i].out_index = i;
+    total += rays;
+    DevOut &d = dout[i];
+    d.cls = o.cls;
+    d.r = o.r;
+    d.theta = o.theta;
+    d.phi = o.phi;
+    d.intensity = o.intensity;
+    d.attempts = o.attempts;
+    d.accepted = o.accepted;
+    d.state = o.state;
+    d.color = nullptr;
+    d.n_rays = rays;
+  }
+  char *base = (char *)ctx->scratch;
+  Counters *cnt_d = (Counters *)(base + SCRATCH_COUNTERS);
+  DevScene *ds_d = (DevScene *)(base + SCRATCH_SCENES);
+  DevOut *do_d = (DevOut *)(base + SCRATCH_OUTS);
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(ds_d, ds, sizeof(DevScene) * n, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(do_d, dout, sizeof(DevOut) * n, hipMemcpyHostToDevice, s));
+  TraceLaunch T;
+  std::memset(&T, 0, sizeof(T));
+  T.scenes = ds_d;
+  T.outs = do_d;
+  T.n_scenes = n;
+  T.n_rays = total;
+  T.counters = cnt_d;
+  T.refill_interval = ctx->launch.refill_interval;
+  const uint32_t grid = grid_for(ctx, total);
+  const uint32_t block = ctx->launch.block_threads;
+  HIP_TRY(launch_reset(cnt_d, s));       // the counters start at zero (a kernel, not a memset)
+  HIP_TRY(hipEventRecord(ctx->ev0, s));
+  if (total > 0) HIP_TRY(launch_trace(scenes[0].metric.kind, scenes[0].disk.kind, precision, T, grid, block, s));
+  HIP_TRY(hipEventRecord(ctx->ev1, s));
+  Counters cnt;
+  HIP_TRY(hipMemcpyAsync(&cnt, cnt_d, sizeof(Counters), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  // exact-once coverage: every ray handed out by the queue retired once
+  if (cnt.retired != total)
+    return fail(GG_ERR_INTERNAL, "ray coverage: %llu of %llu rays retired", (unsigned long long)cnt.retired,
+                (unsigned long long)total);
+  if (stats) {
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    stats->kernel_ms = ms;
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    stats->rays = total;
+    stats->attempts = cnt.attempts;
+    stats->rhs_evals = cnt.rhs;
+    stats->rhs_evals_fp32 = cnt.rhs32;
+    stats->rhs_evals_fast = cnt.rhsf;
+    stats->attempts_full = cnt.attempts_full;
+    stats->reflections = cnt.reflections;
+    stats->grid_blocks = grid;
+    stats->block_threads = block;
+  }
+  return GG_OK;
 }
 
 int gg_autotune(gg_context *ctx, const gg_scene *scenes, uint32_t n, int precision, const gg_launch *candidates,

@@ -272,6 +272,26 @@ int gg_render(gg_context *ctx, const gg_scene *scene, int precision,
 int gg_render_batch(gg_context *ctx, const gg_scene *scenes, uint32_t n,
                     int precision, const gg_outputs *out, gg_stats *stats);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  gg_render_batch for at most GG_NOALLOC_MAX_SCENES scenes
+ * that allocates nothing: the scenes' constants, their output tables and
+ * the trace kernel's counters live in one static device array per GPU
+ * (GG_NOALLOC_SCRATCH_BYTES, held until the process ends) instead of
+ * context buffers that grow on demand, and the launch is direct (no graph,
+ * tile order: gg_launch.order and .graph are not used).  Every output
+ * pointer must be device memory (host outputs would need staging); rgb and
+ * color outputs, user metrics and hittables are not supported
+ * (GG_ERR_UNSUPPORTED); more than GG_NOALLOC_MAX_SCENES scenes is
+ * GG_ERR_INVALID_ARG.  The counters are cleared by a kernel at the start of
+ * each call.  All calls on one GPU share its array: calls that overlap on
+ * one GPU (from several threads or contexts) race and must be serialised by
+ * the caller.  The results are those of gg_render_batch. */
+#define GG_NOALLOC_MAX_SCENES 64
+#define GG_NOALLOC_SCRATCH_BYTES (64u * 1024u)
+int gg_render_batch_noalloc(gg_context *ctx, const gg_scene *scenes, uint32_t n, int precision,
+                            const gg_outputs *out, gg_stats *stats);
+
 /* Times the batch (outputs discarded) under each candidate launch shape and
  * installs the fastest on the context (each candidate renders twice, the
  * second launch is timed).  candidates == NULL: a built-in grid of block

@@ -132,6 +132,8 @@ extern "C" {
                      stats: *mut GgStats) -> c_int;
     pub fn gg_render_batch(ctx: *mut GgContext, scenes: *const GgScene, n: u32, precision: c_int,
                            out: *const GgOutputs, stats: *mut GgStats) -> c_int;
+    pub fn gg_render_batch_noalloc(ctx: *mut GgContext, scenes: *const GgScene, n: u32, precision: c_int,
+                                   out: *const GgOutputs, stats: *mut GgStats) -> c_int;
     pub fn gg_autotune(ctx: *mut GgContext, scenes: *const GgScene, n: u32, precision: c_int,
                        candidates: *const GgLaunch, n_candidates: u32, best: *mut GgLaunch,
                        best_kernel_ms: *mut f64) -> c_int;
