@@ -219,21 +219,24 @@ class Oracle:
         import hashlib
         import subprocess
         import tempfile
-        tag = hashlib.sha1(metric.source.encode()).hexdigest()[:16]
-        d = os.path.join(tempfile.gettempdir(), "gg_oracle_user_metrics")
-        os.makedirs(d, exist_ok=True)
-        so = os.path.join(d, f"um_{tag}.so")
+        text = ("#include <cmath>\nusing namespace std;\n#define __device__\n#define __host__\n"
+                "namespace gg {\n" + metric.source + "\n}\n"
+                "extern \"C\" void gg_user_metric_host(const double *x, double m, double a, double *g) "
+                "{ gg::gg_user_metric(x, m, a, g); }\n")
+        cmd = ["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared"]
+        # keyed by the whole translation unit and the compile line, in a
+        # directory private to this oracle (mkdtemp: mode 0700)
+        tag = hashlib.sha256((text + "\0" + " ".join(cmd)).encode()).hexdigest()[:32]
+        if getattr(self, "_user_dir", None) is None:
+            self._user_dir = tempfile.mkdtemp(prefix="gg_oracle_user_metrics_")
+        so = os.path.join(self._user_dir, f"um_{tag}.so")
         if not os.path.exists(so):
-            src = os.path.join(d, f"um_{tag}_{os.getpid()}.cpp")
+            src = os.path.join(self._user_dir, f"um_{tag}.cpp")
             with open(src, "w") as f:
-                f.write("#include <cmath>\nusing namespace std;\n#define __device__\n#define __host__\n"
-                        "namespace gg {\n" + metric.source + "\n}\n"
-                        "extern \"C\" void gg_user_metric_host(const double *x, double m, double a, double *g) "
-                        "{ gg::gg_user_metric(x, m, a, g); }\n")
-            tmp = f"{so}.{os.getpid()}.tmp"
-            subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", tmp, src],
-                           check=True)
-            os.replace(tmp, so)              # atomic: concurrent test processes see a whole library
+                f.write(text)
+            tmp = f"{so}.tmp"
+            subprocess.run(cmd + ["-o", tmp, src], check=True)
+            os.replace(tmp, so)
         self._user_lib = C.CDLL(so)
         fn = C.cast(self._user_lib.gg_user_metric_host, C.c_void_p)
         rh = metric.event_horizon
