@@ -1011,6 +1011,7 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
   }
 
   TraceLaunch T;
+  T.scene = ds[0];
   T.scenes = (const DevScene *)ctx->scenes.p;
   T.outs = (const DevOut *)ctx->outs.p;
   T.n_scenes = n_scenes;
@@ -1214,6 +1215,7 @@ int gg_render_batch_noalloc(gg_context *ctx, const gg_scene *scenes, uint32_t n,
   HIP_TRY(hipMemcpyAsync(do_d, dout, sizeof(DevOut) * n, hipMemcpyHostToDevice, s));
   TraceLaunch T;
   std::memset(&T, 0, sizeof(T));
+  T.scene = ds[0];
   T.scenes = ds_d;
   T.outs = do_d;
   T.n_scenes = n;
@@ -1461,7 +1463,7 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
   if ((rc = B[0].ensure(n)) || (rc = B[1].ensure(8 * n)) || (rc = B[2].ensure(4 * n)) ||
       (rc = B[3].ensure(8 * n)) || (rc = B[4].ensure(8 * n)) || (rc = B[5].ensure(32 * n)) ||
       (rc = B[6].ensure(4 * n)) || (rc = B[7].ensure(sizeof(GradCounters))) || (rc = B[8].ensure(64 + grad_reduce_scratch_bytes())) ||
-      (rc = B[9].ensure(sizeof(DevScene))) || (rc = B[12].ensure(4 * n)) || (rc = B[13].ensure(4 * n)) ||
+      (rc = B[12].ensure(4 * n)) || (rc = B[13].ensure(4 * n)) ||
       (rc = B[14].ensure(4 * n)) || (rc = B[21].ensure(4 * n)))
     return rc;
   uint8_t *d_cls = (uint8_t *)B[0].p;
@@ -1527,9 +1529,6 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
     ctx->rec_req = {};
     if (rc) return rc;
 
-    DevScene D;
-    build_dev_scene(sc, D);
-    HIP_TRY(hipMemcpyAsync(B[9].p, &D, sizeof(D), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_target, target, 8 * n, hipMemcpyDefault, s));
     GradSelect sel{n, d_cls, d_int, d_target, d_acc, d_att, K, d_resid, d_dI, d_list_in, d_keys_in, d_cnt};
     HIP_TRY(launch_grad_select(sel, s));
@@ -1596,7 +1595,7 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
     HIP_TRY(sort_rays_desc(d_keys_in, d_keys, d_list_in, d_list, n_list, key_bits, nullptr, temp_bytes, s));
     if ((rc = B[17].ensure(temp_bytes ? temp_bytes : 1))) return rc;
     GradLaunch G;
-    G.scene = (const DevScene *)B[9].p;
+    build_dev_scene(sc, G.scene);
     G.list = d_list;
     G.keys = d_keys;
     G.accepted = d_acc;

@@ -331,7 +331,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
   extern __shared__ double lds_k[];
   const KStore K{lds_k + threadIdx.x, blockDim.x};
   const uint32_t lane = threadIdx.x & 63;
-  const DevScene &S = *G.scene;
+  const DevScene &S = G.scene;
   const uint32_t KK = G.K;
   Lane L;
   L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0; L.n_refl = 0;
@@ -437,7 +437,7 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
   const KStore K{lds_k + threadIdx.x, blockDim.x};
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const DevScene &S = *G.scene;
+  const DevScene &S = G.scene;
   const uint32_t KK = G.K;
   const bool live = G.live != 0;
   double *const seg = RECOMPUTE ? G.seg + gtid * (uint64_t)G.seg_cap * STEP_REC : nullptr;
@@ -588,7 +588,7 @@ __global__ __launch_bounds__(256) void adjoint_seed_kernel(GradLaunch G) {
       const uint64_t n = G.n_pix;
       const double *fs = G.final_state;
       ok = r[REC_KIND] == (double)REC_ACCEPT &&
-           crossing_seeds(G, *G.scene, rec, r[REC_R], r[REC_TH], r[REC_V], r[REC_V + 3], fs[n + pix],
+           crossing_seeds(G, G.scene, rec, r[REC_R], r[REC_TH], r[REC_V], r[REC_V + 3], fs[n + pix],
                           fs[2 * n + pix], fs[4 * n + pix], fs[7 * n + pix]);
       put_gv(rec, d4{fs[4 * n + pix], fs[5 * n + pix], fs[6 * n + pix], fs[7 * n + pix]});
       rec[R_CHK] = as_f64(chunk);
@@ -608,7 +608,7 @@ __global__ __launch_bounds__(256) void adjoint_cam_kernel(GradLaunch G) {
   const double *rec = G.rec + REC * q;
   const double *o0 = G.dI + 4 * (uint64_t)pix;
   if (!(o0[0] == o0[0])) return;   // flagged ray keeps its NaNs
-  const DevScene &S = *G.scene;
+  const DevScene &S = G.scene;
   const uint32_t px = pix % G.width, py = pix / G.width;
   const CamD ca = incl_camera_ray_d(S.mass, Tn{S.spin, 1.0}, dconst(G.incl), G.r_cam, S.half_h, S.half_w, px, py,
                                     G.width, G.height);

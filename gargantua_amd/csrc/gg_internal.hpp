@@ -50,6 +50,11 @@ struct TraceLaunch {
   unsigned long long rec_cap;       // chunks in the pool
   unsigned long long *rec_cursor;   // chunks claimed; beyond rec_cap nothing is written
   uint32_t *rec_last;
+  // A one-scene launch reads its scene from here, by value in the kernel
+  // arguments: the constant kernel-argument segment lets every read of a
+  // scene constant be a scalar load, where a read through `scenes` (global
+  // memory the kernel also writes) is a vector load the wavefront waits on.
+  DevScene scene;
 };
 
 // Tallies of gg_grad (zeroed by launch_grad_reset).
@@ -86,7 +91,7 @@ struct GradSelect {
 };
 
 struct GradLaunch {
-  const DevScene *scene;
+  DevScene scene;                  // by value: scene constants are scalar loads from the kernel arguments
   const uint32_t *list;            // disk pixels, longest first
   const uint32_t *keys;            // their attempt counts (step records)
   const uint32_t *accepted;        // accepted count per pixel (the render's)

@@ -223,6 +223,49 @@ __device__ inline d4 rhs64(const DevScene &S, double r, double theta, d4 v) {
 
 // ----------------------------------------------------------- fast FP64 ----
 
+// Inverse metric and the metric's r and theta derivatives at one point of
+// a stationary, axisymmetric metric with g_t_phi (Kerr, kerr.rs:61-94).
+template <class F>
+struct KerrDeriv {
+  F gi_tt, gi_tph, gi_rr, gi_thth, gi_phph;
+  F tt_r, tt_th, rr_r, rr_th, thth_r, thth_th, pp_r, pp_th, tp_r, tp_th;
+};
+
+// Geodesic acceleration -Gamma^mu_ab v^a v^b with the index lowered,
+// -g^{mu nu} (d_a g_{nu b} v^a v^b - 1/2 d_nu g_{ab} v^a v^b): the same
+// quantity as the Christoffel contraction of metric.rs:37-52 (Gamma^mu_ab =
+// 1/2 g^{mu nu} (d_a g_{nu b} + d_b g_{nu a} - d_nu g_{ab})), in about half
+// the operations.  With the metric depending on r and theta only,
+// d_a g_{nu b} v^a v^b = v^r w_nu + v^theta u_nu for w_nu = d_r g_{nu b} v^b
+// and u_nu = d_theta g_{nu b} v^b, and 1/2 d_nu g_{ab} v^a v^b is
+// 1/2 w.v for nu = r, 1/2 u.v for nu = theta, 0 for t and phi.
+__device__ inline float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// (Every fused operation is written out, so that each kernel instantiation
+// rounds the same way: the FP64 kernels' results do not depend on whether a
+// scene's constants are scalar or per lane.)
+template <class F, class V>
+__device__ inline V kerr_accel(const KerrDeriv<F> &D, V v) {
+#pragma clang fp contract(off)
+  const F vt = v.x, vr = v.y, vth = v.z, vph = v.w;
+  const F w_t = fma_t(D.tt_r, vt, D.tp_r * vph), w_p = fma_t(D.tp_r, vt, D.pp_r * vph);
+  const F w_r = D.rr_r * vr, w_th = D.thth_r * vth;
+  const F u_t = fma_t(D.tt_th, vt, D.tp_th * vph), u_p = fma_t(D.tp_th, vt, D.pp_th * vph);
+  const F u_r = D.rr_th * vr, u_th = D.thth_th * vth;
+  const F wv = fma_t(w_t, vt, fma_t(w_p, vph, fma_t(w_r, vr, w_th * vth)));
+  const F uv = fma_t(u_t, vt, fma_t(u_p, vph, fma_t(u_r, vr, u_th * vth)));
+  const F f_t = fma_t(vr, w_t, vth * u_t), f_p = fma_t(vr, w_p, vth * u_p);
+  const F f_r = fma_t(vr, w_r, fma_t(vth, u_r, F(-0.5) * wv));
+  const F f_th = fma_t(vr, w_th, fma_t(vth, u_th, F(-0.5) * uv));
+  V acc;
+  acc.x = -fma_t(D.gi_tt, f_t, D.gi_tph * f_p);
+  acc.y = -(D.gi_rr * f_r);
+  acc.z = -(D.gi_thth * f_th);
+  acc.w = -fma_t(D.gi_tph, f_t, D.gi_phph * f_p);
+  return acc;
+}
+
 // 1/x to ~1 ulp: v_rcp_f64 refined by two Newton steps (x is a positive,
 // normal quantity here: Sigma, Delta or sin^2 theta).
 __device__ inline double rcp64(double x) {
@@ -233,88 +276,70 @@ __device__ inline double rcp64(double x) {
   return __builtin_fma(r, e, r);
 }
 
-// Kerr RHS in FP64 with the algebra of rhs32_kerr: the same Christoffel
-// symbols through 1/Sigma, 1/Delta and 1/sin^2 instead of fifteen divisions,
-// FMA contraction and merged symmetric pairs.  Agrees with rhs64_kerr to a
-// few ulp; used by the fast-FP64 and mixed kernels.
+// Kerr RHS in FP64 with the algebra of rhs32_kerr: the metric derivatives
+// and inverse metric of kerr.rs:100-424 through 1/Sigma, 1/Delta and
+// 1/sin^2 instead of fifteen divisions, explicit FMAs, and the contraction
+// with lowered index (kerr_accel) instead of twenty Christoffel symbols.
+// Agrees with rhs64_kerr to rounding (a few ulp of the largest term); used
+// by the fast-FP64 and mixed kernels.
 __device__ inline d4 rhs64f_kerr(double m, double a, double r_in, double theta, d4 v) {
-#pragma clang fp contract(fast)
-  double r = rmax1e10(r_in);
+#pragma clang fp contract(off)
+  const double r = rmax1e10(r_in);
   double st, ct;
   sincos(theta, &st, &ct);
-  double a2 = a * a;
-  double r2 = r * r;
-  double sin2 = st * st;
-  double sc = st * ct;
-  double sigma = r2 + a2 * ct * ct;
-  double twom = 2.0 * m;
-  double delta = r2 - twom * r + a2;
-  double is = rcp64(sigma);
-  double id = rcp64(delta);
-  double is2 = is * is;
-  double isd = is * id;
-  double dsig_th = -2.0 * a2 * sc;
-  double r2pa2 = r2 + a2;
-  double big_a = r2pa2 * r2pa2 - a2 * delta * sin2;
-  double ginv_tt = -big_a * isd;
-  double ginv_tph = -twom * a * r * isd;
-  double ginv_rr = delta * is;
-  double ginv_thth = is;
-  double ginv_phph = (delta - a2 * sin2) * isd * rcp64(sin2);
-  double sig_m2r2 = sigma - 2.0 * r2;
-  double dtt_dr = twom * sig_m2r2 * is2;
-  double dtt_dth = -twom * r * dsig_th * is2;
-  double drr_dr = (2.0 * r * delta - sigma * (2.0 * r - twom)) * id * id;
-  double drr_dth = dsig_th * id;
-  double dthth_dr = 2.0 * r;
-  double dthth_dth = dsig_th;
-  double big_p = twom * a2 * r;
-  double dpp_dr = 2.0 * r * sin2 + sin2 * sin2 * twom * a2 * sig_m2r2 * is2;
-  double dpp_dth = 2.0 * r2pa2 * sc + big_p * (4.0 * sin2 * sc * sigma - sin2 * sin2 * dsig_th) * is2;
-  double dtp_dr = -twom * a * sin2 * sig_m2r2 * is2;
-  double dtp_dth = -twom * a * r * (2.0 * sc * sigma - sin2 * dsig_th) * is2;
-
-  double G001 = 0.5 * (ginv_tt * dtt_dr + ginv_tph * dtp_dr);
-  double G002 = 0.5 * (ginv_tt * dtt_dth + ginv_tph * dtp_dth);
-  double G013 = 0.5 * (ginv_tt * dtp_dr + ginv_tph * dpp_dr);
-  double G023 = 0.5 * (ginv_tt * dtp_dth + ginv_tph * dpp_dth);
-  double hrr = 0.5 * ginv_rr;
-  double G100 = -hrr * dtt_dr, G111 = hrr * drr_dr, G122 = -hrr * dthth_dr, G133 = -hrr * dpp_dr;
-  double G112 = hrr * drr_dth, G103 = -hrr * dtp_dr;
-  double hth = 0.5 * ginv_thth;
-  double G200 = -hth * dtt_dth, G211 = -hth * drr_dth, G222 = hth * dthth_dth, G233 = -hth * dpp_dth;
-  double G212 = hth * dthth_dr, G203 = -hth * dtp_dth;
-  double G301 = 0.5 * (ginv_tph * dtt_dr + ginv_phph * dtp_dr);
-  double G302 = 0.5 * (ginv_tph * dtt_dth + ginv_phph * dtp_dth);
-  double G313 = 0.5 * (ginv_tph * dtp_dr + ginv_phph * dpp_dr);
-  double G323 = 0.5 * (ginv_tph * dtp_dth + ginv_phph * dpp_dth);
-
-  const double v0 = v.x, v1 = v.y, v2 = v.z, v3 = v.w;
-  d4 acc;
-  acc.x = -2.0 * (G001 * v0 * v1 + G002 * v0 * v2 + G013 * v1 * v3 + G023 * v2 * v3);
-  acc.y = -(G100 * v0 * v0 + G111 * v1 * v1 + G122 * v2 * v2 + G133 * v3 * v3 +
-            2.0 * (G112 * v1 * v2 + G103 * v0 * v3));
-  acc.z = -(G200 * v0 * v0 + G211 * v1 * v1 + G222 * v2 * v2 + G233 * v3 * v3 +
-            2.0 * (G212 * v1 * v2 + G203 * v0 * v3));
-  acc.w = -2.0 * (G301 * v0 * v1 + G302 * v0 * v2 + G313 * v1 * v3 + G323 * v2 * v3);
-  return acc;
+  const double a2 = a * a;
+  const double r2 = r * r;
+  const double sin2 = st * st;
+  const double sc = st * ct;
+  const double sigma = __builtin_fma(a2 * ct, ct, r2);
+  const double twom = 2.0 * m;
+  const double delta = __builtin_fma(-twom, r, r2 + a2);
+  const double is = rcp64(sigma);
+  const double id = rcp64(delta);
+  const double is2 = is * is;
+  const double isd = is * id;
+  const double dsig_th = (-2.0 * a2) * sc;
+  const double r2pa2 = r2 + a2;
+  const double big_a = __builtin_fma(r2pa2, r2pa2, -((a2 * delta) * sin2));
+  const double two_r = 2.0 * r;
+  const double s4 = sin2 * sin2;
+  KerrDeriv<double> D;
+  D.gi_tt = -big_a * isd;
+  D.gi_tph = -(twom * a * r) * isd;
+  D.gi_rr = delta * is;
+  D.gi_thth = is;
+  D.gi_phph = __builtin_fma(-a2, sin2, delta) * isd * rcp64(sin2);
+  const double sig_m2r2 = __builtin_fma(-2.0, r2, sigma);
+  D.tt_r = twom * sig_m2r2 * is2;
+  D.tt_th = -(twom * r) * dsig_th * is2;
+  D.rr_r = __builtin_fma(two_r, delta, -(sigma * (two_r - twom))) * id * id;
+  D.rr_th = dsig_th * id;
+  D.thth_r = two_r;
+  D.thth_th = dsig_th;
+  const double big_p = twom * a2 * r;
+  D.pp_r = __builtin_fma(two_r, sin2, s4 * (twom * a2) * sig_m2r2 * is2);
+  D.pp_th = __builtin_fma(2.0 * r2pa2, sc, big_p * __builtin_fma(4.0 * sin2 * sc, sigma, -(s4 * dsig_th)) * is2);
+  D.tp_r = -(twom * a) * sin2 * sig_m2r2 * is2;
+  D.tp_th = -(twom * a * r) * __builtin_fma(2.0 * sc, sigma, -(sin2 * dsig_th)) * is2;
+  return kerr_accel(D, v);
 }
 
 __device__ inline d4 rhs64f_schw(double rs, double r_in, double theta, d4 v) {
-#pragma clang fp contract(fast)
-  double r = rmax1e10(r_in);
+#pragma clang fp contract(off)
+  const double r = rmax1e10(r_in);
   double st, ct;
   sincos(theta, &st, &ct);
-  double ir = rcp64(r);
-  double rmrs = r - rs;
-  double ifd = rcp64(r * rmrs);
-  double g_t_tr = 0.5 * rs * ifd;
-  double g_r_tt = 0.5 * rs * rmrs * ir * ir * ir;
+  const double ir = rcp64(r);
+  const double rmrs = r - rs;
+  const double ifd = rcp64(r * rmrs);
+  const double g_t_tr = 0.5 * rs * ifd;
+  const double g_r_tt = 0.5 * rs * rmrs * ir * ir * ir;
   d4 acc;
   acc.x = -2.0 * g_t_tr * v.x * v.y;
-  acc.y = -g_r_tt * v.x * v.x + g_t_tr * v.y * v.y + rmrs * (v.z * v.z + st * st * v.w * v.w);
-  acc.z = -2.0 * ir * v.y * v.z + st * ct * v.w * v.w;
-  acc.w = -2.0 * ir * v.y * v.w - 2.0 * (ct * rcp64(st)) * v.z * v.w;
+  acc.y = __builtin_fma(-g_r_tt * v.x, v.x,
+                        __builtin_fma(g_t_tr * v.y, v.y, rmrs * __builtin_fma(v.z, v.z, st * st * v.w * v.w)));
+  acc.z = __builtin_fma(-2.0 * ir * v.y, v.z, st * ct * v.w * v.w);
+  acc.w = __builtin_fma(-2.0 * ir * v.y, v.w, -(2.0 * (ct * rcp64(st)) * v.z * v.w));
   return acc;
 }
 
@@ -328,9 +353,9 @@ __device__ inline d4 rhs64f(const DevScene &S, double r, double theta, d4 v) {
 
 __device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// Kerr geodesic acceleration in FP32: the same Christoffel symbols with the
-// divisions folded into 1/Sigma, 1/Delta, 1/sin^2 and the symmetric pairs of
-// the contraction merged.
+// Kerr geodesic acceleration in FP32: the metric derivatives and inverse
+// metric with the divisions folded into 1/Sigma, 1/Delta, 1/sin^2, and the
+// contraction with lowered index (kerr_accel).
 // psi = theta - pi/2, formed in FP64 by the caller: near the equatorial
 // plane FP32 keeps full relative precision in cos(theta) = -sin(psi), which
 // a rounded FP32 theta would not (its 6e-8 absolute error feeds straight
@@ -354,48 +379,25 @@ __device__ inline f4 rhs32_kerr(float m, float a, float r, float psi, f4 v) {
   float r2pa2 = r2 + a2;
   float big_a = r2pa2 * r2pa2 - a2 * delta * sin2;
   float twom = 2.0f * m;
-  float ginv_tt = -big_a * isd;
-  float ginv_tph = -twom * a * r * isd;
-  float ginv_rr = delta * is;
-  float ginv_thth = is;
-  float ginv_phph = (delta - a2 * sin2) * isd * frcp(sin2);
+  KerrDeriv<float> D;
+  D.gi_tt = -big_a * isd;
+  D.gi_tph = -twom * a * r * isd;
+  D.gi_rr = delta * is;
+  D.gi_thth = is;
+  D.gi_phph = (delta - a2 * sin2) * isd * frcp(sin2);
   float sig_m2r2 = sigma - 2.0f * r2;
-  float dtt_dr = twom * sig_m2r2 * is2;
-  float dtt_dth = -twom * r * dsig_th * is2;
-  float drr_dr = (2.0f * r * delta - sigma * (2.0f * r - twom)) * id * id;
-  float drr_dth = dsig_th * id;
-  float dthth_dr = 2.0f * r;
-  float dthth_dth = dsig_th;
+  D.tt_r = twom * sig_m2r2 * is2;
+  D.tt_th = -twom * r * dsig_th * is2;
+  D.rr_r = (2.0f * r * delta - sigma * (2.0f * r - twom)) * id * id;
+  D.rr_th = dsig_th * id;
+  D.thth_r = 2.0f * r;
+  D.thth_th = dsig_th;
   float big_p = twom * a2 * r;
-  float dpp_dr = 2.0f * r * sin2 + sin2 * sin2 * twom * a2 * sig_m2r2 * is2;
-  float dpp_dth = 2.0f * r2pa2 * sc + big_p * (4.0f * sin2 * sc * sigma - sin2 * sin2 * dsig_th) * is2;
-  float dtp_dr = -twom * a * sin2 * sig_m2r2 * is2;
-  float dtp_dth = -twom * a * r * (2.0f * sc * sigma - sin2 * dsig_th) * is2;
-
-  float G001 = 0.5f * (ginv_tt * dtt_dr + ginv_tph * dtp_dr);
-  float G002 = 0.5f * (ginv_tt * dtt_dth + ginv_tph * dtp_dth);
-  float G013 = 0.5f * (ginv_tt * dtp_dr + ginv_tph * dpp_dr);
-  float G023 = 0.5f * (ginv_tt * dtp_dth + ginv_tph * dpp_dth);
-  float hrr = 0.5f * ginv_rr;
-  float G100 = -hrr * dtt_dr, G111 = hrr * drr_dr, G122 = -hrr * dthth_dr, G133 = -hrr * dpp_dr;
-  float G112 = hrr * drr_dth, G103 = -hrr * dtp_dr;
-  float hth = 0.5f * ginv_thth;
-  float G200 = -hth * dtt_dth, G211 = -hth * drr_dth, G222 = hth * dthth_dth, G233 = -hth * dpp_dth;
-  float G212 = hth * dthth_dr, G203 = -hth * dtp_dth;
-  float G301 = 0.5f * (ginv_tph * dtt_dr + ginv_phph * dtp_dr);
-  float G302 = 0.5f * (ginv_tph * dtt_dth + ginv_phph * dtp_dth);
-  float G313 = 0.5f * (ginv_tph * dtp_dr + ginv_phph * dpp_dr);
-  float G323 = 0.5f * (ginv_tph * dtp_dth + ginv_phph * dpp_dth);
-
-  const float v0 = v.x, v1 = v.y, v2 = v.z, v3 = v.w;
-  f4 acc;
-  acc.x = -2.0f * (G001 * v0 * v1 + G002 * v0 * v2 + G013 * v1 * v3 + G023 * v2 * v3);
-  acc.y = -(G100 * v0 * v0 + G111 * v1 * v1 + G122 * v2 * v2 + G133 * v3 * v3 +
-            2.0f * (G112 * v1 * v2 + G103 * v0 * v3));
-  acc.z = -(G200 * v0 * v0 + G211 * v1 * v1 + G222 * v2 * v2 + G233 * v3 * v3 +
-            2.0f * (G212 * v1 * v2 + G203 * v0 * v3));
-  acc.w = -2.0f * (G301 * v0 * v1 + G302 * v0 * v2 + G313 * v1 * v3 + G323 * v2 * v3);
-  return acc;
+  D.pp_r = 2.0f * r * sin2 + sin2 * sin2 * twom * a2 * sig_m2r2 * is2;
+  D.pp_th = 2.0f * r2pa2 * sc + big_p * (4.0f * sin2 * sc * sigma - sin2 * sin2 * dsig_th) * is2;
+  D.tp_r = -twom * a * sin2 * sig_m2r2 * is2;
+  D.tp_th = -twom * a * r * (2.0f * sc * sigma - sin2 * dsig_th) * is2;
+  return kerr_accel(D, v);
 }
 
 __device__ inline f4 rhs32_schw(float rs, float r, float psi, f4 v) {

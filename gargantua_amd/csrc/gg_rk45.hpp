@@ -164,22 +164,45 @@ constexpr int STEP_REC = 32;
 constexpr int REC_R = 0, REC_TH = 1, REC_H = 2, REC_KIND = 3, REC_V = 4, REC_VS = 8, REC_DV = 28;
 enum { REC_ACCEPT = 0, REC_REJECT = 1, REC_INVALID = 2 };
 
+// Tableau entries for a stage number known at compile time (the unrolled
+// loop): the same double quotients as kA, kB, kC, kD, folded into the code.
+__device__ constexpr double tabA(int s) {
+  return s == 2 ? A2 : s == 3 ? A3 : s == 4 ? A4 : s == 5 ? A5 : s == 6 ? 1.0 : 0.0;
+}
+__device__ constexpr double tabB(int s, int j) {
+  return s == 3 ? (j == 1 ? B31 : B32)
+       : s == 4 ? (j == 1 ? B41 : j == 2 ? B42 : B43)
+       : s == 5 ? (j == 1 ? B51 : j == 2 ? B52 : j == 3 ? B53 : B54)
+       : s == 6 ? (j == 1 ? B61 : j == 2 ? B62 : j == 3 ? B63 : j == 4 ? B64 : B65)
+       : 0.0;
+}
+__device__ constexpr double tabC(int s) {
+  return s == 1 ? C1 : s == 3 ? C3 : s == 4 ? C4 : s == 5 ? C5 : s == 6 ? C6 : 0.0;
+}
+__device__ constexpr double tabD(int s) {
+  return s == 1 ? D1 : s == 3 ? D3 : s == 4 ? D4 : s == 5 ? D5 : s == 6 ? D6 : s == 7 ? D7 : 0.0;
+}
+
 // The stages of one attempt (geodesic.rs:244-308): k1 when the lane has no
 // valid one (s = 1), then k2..k7, the 5th-order velocity and the error.  One
-// copy of the loop per arithmetic (ARITH_EXACT, ARITH_F32, ARITH_FAST).
+// copy of the loop per arithmetic (ARITH_EXACT, ARITH_F32, ARITH_FAST).  The
+// FP32 loop is unrolled (stage number, tableau and branches resolved at
+// compile time); the FP64 loops stay rolled, their right-hand sides being
+// several times larger.
 // REC: write the stage velocities into rec (when non-null) as they form.
 template <int METRIC, int ARITH, bool REC = false>
 __device__ inline StageOut stages(const DevScene &S, const KStore &K, const double p0[4], const d4 vel, double h,
                                   bool need_k1, double *rec = nullptr) {
 #pragma clang fp contract(off)
+  constexpr bool UNROLL = (ARITH != ARITH_EXACT);
   StageOut o;
   d4 k1{0, 0, 0, 0}, s5{0, 0, 0, 0}, s4{0, 0, 0, 0};
   o.nv = d4{0, 0, 0, 0};
   o.k7 = d4{0, 0, 0, 0};
   o.nr = 0;
   o.invalid = false;
-#pragma unroll 1
-  for (int s = 1; s <= 7; ++s) {
+  // one stage; false when a stage position left the domain
+  auto stage = [&] __device__(const int s) -> bool {
     double qr, qth;
     double qt = 0.0, qph = 0.0;      // METRIC 2: the metric may depend on t and phi too
     d4 vs;
@@ -192,23 +215,29 @@ __device__ inline StageOut stages(const DevScene &S, const KStore &K, const doub
       eval = need_k1;
     } else if (s <= 6) {
       // stage position pos + (h a_s) vel on the forward-Euler line (:248-284)
-      double ha = h * kA[s];
+      double ha = h * (UNROLL ? tabA(s) : kA[s]);
       double q0 = madd<ARITH>(ha, vel.x, p0[0]), q1 = madd<ARITH>(ha, vel.y, p0[1]);
       double q2 = madd<ARITH>(ha, vel.z, p0[2]), q3 = madd<ARITH>(ha, vel.w, p0[3]);
-      if (!valid4(q0, q1, q2, q3)) { o.invalid = true; break; }   // :250-291 halve h
+      if (!valid4(q0, q1, q2, q3)) { o.invalid = true; return false; }   // :250-291 halve h
       if (s == 2) {
         double hb = h * B21;                                        // vel + (h b21) k1 (:249)
         vs = d4{madd<ARITH>(hb, k1.x, vel.x), madd<ARITH>(hb, k1.y, vel.y), madd<ARITH>(hb, k1.z, vel.z),
                 madd<ARITH>(hb, k1.w, vel.w)};
       } else {
-        double b = kB[s][1];
+        double b = UNROLL ? tabB(s, 1) : kB[s][1];
         d4 sum{b * k1.x, b * k1.y, b * k1.z, b * k1.w};
-#pragma unroll 1
-        for (int j = 2; j < s; ++j) {
-          double bj = kB[s][j];
+        auto term = [&] __device__(int j) {
+          double bj = UNROLL ? tabB(s, j) : kB[s][j];
           d4 kj = K.get(j - 1);
           sum.x = madd<ARITH>(bj, kj.x, sum.x); sum.y = madd<ARITH>(bj, kj.y, sum.y);
           sum.z = madd<ARITH>(bj, kj.z, sum.z); sum.w = madd<ARITH>(bj, kj.w, sum.w);
+        };
+        if constexpr (UNROLL) {
+#pragma unroll
+          for (int j = 2; j < s; ++j) term(j);
+        } else {
+#pragma unroll 1
+          for (int j = 2; j < s; ++j) term(j);
         }
         vs = d4{madd<ARITH>(h, sum.x, vel.x), madd<ARITH>(h, sum.y, vel.y), madd<ARITH>(h, sum.z, vel.z),
                 madd<ARITH>(h, sum.w, vel.w)};
@@ -245,11 +274,11 @@ __device__ inline StageOut stages(const DevScene &S, const KStore &K, const doub
       s4 = d4{D1 * k1.x, D1 * k1.y, D1 * k1.z, D1 * k1.w};       // sum d_i k_i (:299-305)
     } else if (s >= 3) {
       if (s <= 6) {
-        double c = kC[s];
+        double c = UNROLL ? tabC(s) : kC[s];
         s5.x = madd<ARITH>(c, k.x, s5.x); s5.y = madd<ARITH>(c, k.y, s5.y);
         s5.z = madd<ARITH>(c, k.z, s5.z); s5.w = madd<ARITH>(c, k.w, s5.w);
       }
-      double d = kD[s];
+      double d = UNROLL ? tabD(s) : kD[s];
       s4.x = madd<ARITH>(d, k.x, s4.x); s4.y = madd<ARITH>(d, k.y, s4.y);
       s4.z = madd<ARITH>(d, k.z, s4.z); s4.w = madd<ARITH>(d, k.w, s4.w);
     }
@@ -257,6 +286,16 @@ __device__ inline StageOut stages(const DevScene &S, const KStore &K, const doub
       o.nv = d4{madd<ARITH>(h, s5.x, vel.x), madd<ARITH>(h, s5.y, vel.y), madd<ARITH>(h, s5.z, vel.z),
                 madd<ARITH>(h, s5.w, vel.w)};
     if (s == 7) o.k7 = k;
+    return true;
+  };
+  if constexpr (UNROLL) {
+#pragma unroll
+    for (int s = 1; s <= 7; ++s)
+      if (!stage(s)) break;
+  } else {
+#pragma unroll 1
+    for (int s = 1; s <= 7; ++s)
+      if (!stage(s)) break;
   }
   if (o.invalid) return o;
   const d4 nv = o.nv;

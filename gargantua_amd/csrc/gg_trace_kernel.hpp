@@ -80,10 +80,18 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
   constexpr uint32_t NO_CHUNK = 0xFFFFFFFFu;
   uint32_t r_chunk = NO_CHUNK;   // record chunk holding attempt `attempts`
   bool r_need = false;           // a new chunk is due before the next attempt
-  // Scene constants are read through the pointer where they are used, so
-  // the compiler reloads them (scalar loads for a single scene) instead of
-  // holding ~75 doubles live across the persistent loop.
-  const DevScene *__restrict__ Sp = T.scenes;
+  // Scene constants are read through the pointer where they are used:
+  // scalar loads from the kernel arguments for one scene, vector loads of
+  // the lane's own scene in a batch.  The kernel-argument pointer passes
+  // through an empty asm statement at the top of every iteration of the
+  // persistent loop, so that the compiler reloads the constants each
+  // iteration instead of hoisting ~75 invariant loads out of the loop and
+  // holding (spilling) them across it.
+  // (T is the kernel's only explicit argument: it starts the segment.)
+  typedef const __attribute__((address_space(4))) DevScene KScene;
+  typedef const __attribute__((address_space(4))) char KByte;
+  KScene *scene_k = (KScene *)((KByte *)__builtin_amdgcn_kernarg_segment_ptr() + __builtin_offsetof(TraceLaunch, scene));
+  const DevScene *__restrict__ Sp = single ? (const DevScene *)scene_k : T.scenes;
 
   // Start ray `ray` in this lane.
   auto init_ray = [&](uint64_t rid) {
@@ -125,6 +133,10 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
   __shared__ unsigned long long s_base;
   uint32_t iter = 0;
   for (;;) {
+    if constexpr (single) {
+      asm volatile("" : "+s"(scene_k));
+      Sp = (const DevScene *)scene_k;
+    }
     {
       // ---- block regroup: compaction (and precision sorting when mixed) ----
       // Every refill_interval attempts the block's wavefronts exchange ray
