@@ -1570,11 +1570,10 @@ int gg_grad(gg_context *ctx, const gg_grad_scene *scene, const double *target, u
   if (mode == 0 && seg_cap > 0xFFFFFFFFull) return fail(GG_ERR_INVALID_ARG, "segment buffer too large");
 
   // launch shapes: forward 256-thread blocks, 2 per CU; reverse 64-thread
-  // blocks, 4 per CU when it recomputes segments (its LDS and registers allow
-  // one wavefront per SIMD), else 8 (two wavefronts per SIMD); never more
-  // wavefronts than disk rays
+  // blocks, 4 per CU (one wavefront per SIMD: its registers hold the record
+  // it is loading ahead); never more wavefronts than disk rays
   const uint64_t waves = (n_list + 63) / 64;
-  uint64_t grid_f = (uint64_t)ctx->num_cus * 2, grid_r = (uint64_t)ctx->num_cus * (mode == 0 ? 4 : 8);
+  uint64_t grid_f = (uint64_t)ctx->num_cus * 2, grid_r = (uint64_t)ctx->num_cus * 4;
   if ((waves + 3) / 4 < grid_f) grid_f = (waves + 3) / 4;
   if (waves < grid_r) grid_r = waves;
   if (grid_f < 1) grid_f = 1;

@@ -58,11 +58,38 @@ struct Adj {
   double hn;        // h of the next attempt
 };
 
+// The RHS VJP at (r, theta0 + d), sin and cos of the angle from those of
+// theta0 (s0, c0) when d is small.
+__device__ inline RhsBar vjp_near(double m, double a, double r, double th0, double s0, double c0, double d, d4 v,
+                                  d4 lam) {
+  double s, c;
+  if (!sincos_near(s0, c0, d, s, c)) sincos(th0 + d, &s, &c);
+  return kerr_rhs_vjp_sc(m, a, r, s, c, v, lam);
+}
+
+// One step record in registers: the reverse sweep loads the record of the
+// attempt before the current one while it back-propagates the current one,
+// so that the load's latency overlaps the arithmetic.
+struct StepRec {
+  double d[STEP_REC];
+};
+
+__device__ inline void load_rec(StepRec &R, const double *src) {
+  const double *p = (const double *)__builtin_assume_aligned(src, 32);
+#pragma unroll
+  for (int k = 0; k < STEP_REC; ++k) R.d[k] = p[k];
+}
+
+// LDS slots 6..10 of the reverse kernels: the current record's stage
+// velocities v_2..v_6, read by the rolled stage loop.
+constexpr int SLOT_VS = 6;
+
 // Adjoint of one recorded attempt (geodesic.rs:185-328), whose record holds
 // its start (r, theta), h, kind, a velocity, the stage velocities v_2..v_6
 // and the error difference new_vel - vel_err.  On entry A holds the
 // cotangents of the attempt's outputs (state and next h); on exit those of
-// its inputs.  LDS slots 0..5 hold the stage cotangents kbar_1..kbar_6.
+// its inputs.  LDS slots 0..5 hold the stage cotangents kbar_1..kbar_6,
+// slots 6..10 the stage velocities.
 //
 // Accepted: pos' = p + h v (normalised), v' = v + h sum_i c_i k_i,
 //   k_i = a(p + h a_i v, v_i).  Rejected: p, v pass through.  Invalid: h/2.
@@ -78,9 +105,9 @@ struct Adj {
 //   VJP's own velocity output dotted with u.
 // !live: the step sizes are held fixed; rejected and invalid attempts then
 //   move nothing.
-__device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const double *rec_in, bool live, Adj &A) {
+__device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const StepRec &R, bool live, Adj &A) {
   GG_CONTRACT_FAST
-  const double *rec = (const double *)__builtin_assume_aligned(rec_in, 32);
+  const double *rec = R.d;
   const double h = rec[REC_H];
   const int kind = (int)rec[REC_KIND];
   const double hn = A.hn;
@@ -93,6 +120,8 @@ __device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const
   const d4 rv{rec[REC_V], rec[REC_V + 1], rec[REC_V + 2], rec[REC_V + 3]};
   const d4 v = acc ? rv : A.gv;
   const d4 nv = acc ? A.gv : rv;
+  double s0, c0;                         // the stage angles are theta0 + h a_i v^theta
+  sincos(th0, &s0, &c0);
   // derivative coefficients take reciprocals to ~1 ulp (v_rcp_f64 + Newton)
   const double ih = rcp64(h);
 
@@ -173,8 +202,8 @@ __device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const
     veb = d4{-db.x, -db.y, -db.z, -db.w};
     // k7 = a(p + h v, new_vel) (geodesic.rs:298), weight h d7 in vel_err
     const double hd7 = h * D7;
-    const RhsBar jb = kerr_rhs_vjp(m, a, h * v.y + r0, h * v.z + th0, nv,
-                                   d4{hd7 * veb.x, hd7 * veb.y, hd7 * veb.z, hd7 * veb.w});
+    const RhsBar jb = vjp_near(m, a, h * v.y + r0, th0, s0, c0, h * v.z, nv,
+                               d4{hd7 * veb.x, hd7 * veb.y, hd7 * veb.z, hd7 * veb.w});
     A.ga += jb.a;
     pr += jb.r;
     pth += jb.th;
@@ -183,29 +212,34 @@ __device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const
     hbar += jb.r * v.y + jb.th * v.z + 0.5 * ih * (jb.v.x * nv.x + jb.v.y * nv.y + jb.v.z * nv.z + jb.v.w * nv.w);
     nvb.x += jb.v.x; nvb.y += jb.v.y; nvb.z += jb.v.z; nvb.w += jb.v.w;
   }
+#pragma unroll
+  for (int i = 2; i <= 6; ++i) {
+    const double *o = rec + REC_VS + 4 * (i - 2);
+    K.put(SLOT_VS + i - 2, d4{o[0], o[1], o[2], o[3]});
+  }
+  // (kB rows are zero beyond the diagonal: the k_j cotangent updates below
+  // run over j = 1..5 for every stage, their LDS reads issued together)
   // new_vel = v + h sum_i c_i k_i, vel_err = v + h sum_i d_i k_i
   vbar.x += nvb.x + veb.x; vbar.y += nvb.y + veb.y; vbar.z += nvb.z + veb.z; vbar.w += nvb.w + veb.w;
-#pragma unroll 1
+#pragma unroll
   for (int i = 1; i <= 6; ++i) {
-    const double hc = h * kC[i], hd = h * kD[i];
+    const double hc = h * tabC(i), hd = h * tabD(i);
     K.put(i - 1, d4{hc * nvb.x + hd * veb.x, hc * nvb.y + hd * veb.y, hc * nvb.z + hd * veb.z,
                     hc * nvb.w + hd * veb.w});
   }
 #pragma unroll 1
   for (int i = 6; i >= 1; --i) {
     const d4 w = K.get(i - 1);
-    double qr = r0, qth = th0;
+    double qr = r0;
     d4 vs = v;
     double ha = 0.0;
     if (i >= 2) {
       // stage position p + h a_i v, stage velocity v + h sum_j b_ij k_j
       ha = h * kA[i];
       qr = ha * v.y + r0;
-      qth = ha * v.z + th0;
-      const double *o = rec + REC_VS + 4 * (i - 2);
-      vs = d4{o[0], o[1], o[2], o[3]};
+      vs = K.get(SLOT_VS + i - 2);
     }
-    const RhsBar jb = kerr_rhs_vjp(m, a, qr, qth, vs, w);
+    const RhsBar jb = vjp_near(m, a, qr, th0, s0, c0, ha * v.z, vs, w);
     A.ga += jb.a;
     pr += jb.r;
     pth += jb.th;
@@ -216,12 +250,14 @@ __device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const
       vbar.y += ha * jb.r;
       vbar.z += ha * jb.th;
       if (live) hbar += kA[i] * (jb.r * v.y + jb.th * v.z);
-#pragma unroll 1
-      for (int j = 1; j < i; ++j) {
-        double hb = h * kB[i][j];
-        d4 kb = K.get(j - 1);
-        kb.x += hb * jb.v.x; kb.y += hb * jb.v.y; kb.z += hb * jb.v.z; kb.w += hb * jb.v.w;
-        K.put(j - 1, kb);
+      d4 kb[5];
+#pragma unroll
+      for (int j = 1; j <= 5; ++j) kb[j - 1] = K.get(j - 1);
+#pragma unroll
+      for (int j = 1; j <= 5; ++j) {
+        const double hb = h * kB[i][j];
+        kb[j - 1].x += hb * jb.v.x; kb[j - 1].y += hb * jb.v.y; kb[j - 1].z += hb * jb.v.z; kb[j - 1].w += hb * jb.v.w;
+        K.put(j - 1, kb[j - 1]);
       }
     }
   }
@@ -449,7 +485,10 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
   bool exhausted = false, first = false;
   uint64_t q = 0, chk_base = 0;
   uint32_t pix = 0, c = 0, j = 0, att = 0, chunk = 0, seg_j = 0;
+  uint32_t chunk_prev = 0;   // MODE 2: the chunk before `chunk` (its back-pointer, read on entering it)
   const double *src = nullptr;
+  StepRec cur;               // the record of attempt j, once cur_ok
+  bool cur_ok = false;
   Adj A;
   A.pr = 0; A.pth = 0; A.vb = d4{0, 0, 0, 0}; A.hb = 0; A.ga = 0; A.gv = d4{0, 0, 0, 0}; A.hn = 0;
   unsigned long long n_att = 0, n_adj = 0, n_done = 0, n_bad = 0;
@@ -469,6 +508,7 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
         c = nchk - 1;
         if (MODE == 2) {
           chunk = (uint32_t)as_u64(rec[R_CHK]);
+          chunk_prev = (uint32_t)as_u64(G.pool[(uint64_t)chunk * GRAD_CHUNK_DOUBLES]);
           j = G.keys[q] - 1;
         } else {
           chk_base = as_u64(rec[R_CHK]);
@@ -482,6 +522,7 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
         A.hb = 0.0;       // the crossing ends the ray: its next h enters nothing
         A.hn = 0.0;
         first = true;
+        cur_ok = false;
         phase = REV;
       }
     }
@@ -517,6 +558,7 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
             } else {
               j = seg_j - 1;
               src = seg;
+              cur_ok = false;
               phase = REV;
             }
           }
@@ -526,9 +568,20 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
 
     // ---- REV: one step record backwards ----
     if (phase == REV) {
-      const double *sp = MODE == 2 ? G.pool + (uint64_t)chunk * GRAD_CHUNK_DOUBLES + GRAD_CHUNK_HEAD + STEP_REC * (j % GRAD_CHUNK)
-                                   : src + STEP_REC * (uint64_t)j;
-      attempt_adjoint(K, S, sp, live, A);
+      // the record of attempt j (loaded one attempt ahead, except the first
+      // of a ray or segment) and, while it is back-propagated, that of j - 1
+      auto rec_at = [&](uint32_t ch, uint32_t jj) -> const double * {
+        return MODE == 2 ? G.pool + (uint64_t)ch * GRAD_CHUNK_DOUBLES + GRAD_CHUNK_HEAD + STEP_REC * (jj % GRAD_CHUNK)
+                         : src + STEP_REC * (uint64_t)jj;
+      };
+      if (!cur_ok) load_rec(cur, rec_at(chunk, j));
+      // (issued on every path, the last record loading itself again, so that
+      // the wait before the current record's first use does not also wait
+      // for this load)
+      StepRec nxt;
+      const uint32_t nchunk = (MODE == 2 && j > 0 && j % GRAD_CHUNK == 0) ? chunk_prev : chunk;
+      load_rec(nxt, rec_at(nchunk, j > 0 ? j - 1 : j));
+      attempt_adjoint(K, S, cur, live, A);
       ++n_adj;
       if (first) {
         const double *rec = G.rec + REC * q;
@@ -536,8 +589,13 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
         first = false;
       }
       if (j > 0) {
-        if (MODE == 2 && j % GRAD_CHUNK == 0) chunk = (uint32_t)as_u64(G.pool[(uint64_t)chunk * GRAD_CHUNK_DOUBLES]);
+        if (MODE == 2 && nchunk != chunk) {
+          chunk = nchunk;
+          chunk_prev = (uint32_t)as_u64(G.pool[(uint64_t)chunk * GRAD_CHUNK_DOUBLES]);
+        }
         --j;
+        cur = nxt;
+        cur_ok = true;
       } else if (RECOMPUTE && c > 0) {
         --c;
         get_state(G.chk + 9 * (chk_base + c), L.pos, L.vel, L.h);
@@ -568,8 +626,8 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
 }
 
 __global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) { adjoint_rev_body<0>(G); }
-__global__ __launch_bounds__(64, 2) void adjoint_rev_stored_kernel(GradLaunch G) { adjoint_rev_body<1>(G); }
-__global__ __launch_bounds__(64, 2) void adjoint_rev_chunked_kernel(GradLaunch G) { adjoint_rev_body<2>(G); }
+__global__ __launch_bounds__(64, 1) void adjoint_rev_stored_kernel(GradLaunch G) { adjoint_rev_body<1>(G); }
+__global__ __launch_bounds__(64, 1) void adjoint_rev_chunked_kernel(GradLaunch G) { adjoint_rev_body<2>(G); }
 
 // Seeds of the recorded rays (the recording render replaces the re-trace):
 // the crossing step is the ray's last record, its end state the render's
@@ -762,7 +820,7 @@ __global__ __launch_bounds__(256) void vjp_kernel(const DevScene *S, uint64_t n,
 // LDS of the reverse kernel: six slots of 4 doubles (the recomputation's
 // k_1..k_5, the reverse step's cotangents kbar_1..kbar_6); of the forward
 // kernel: the trace kernel's k_1..k_5.
-size_t adjoint_lds_bytes(uint32_t block) { return (size_t)block * 6 * 4 * sizeof(double); }
+size_t adjoint_lds_bytes(uint32_t block) { return (size_t)block * 11 * 4 * sizeof(double); }
 size_t adjoint_fwd_lds_bytes(uint32_t block) { return (size_t)block * 5 * 4 * sizeof(double); }
 
 hipError_t launch_adjoint_fwd(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s) {

@@ -2,15 +2,18 @@
 // mode written out by hand.
 //
 // a^mu = -Gamma^mu_ab(r, theta; a) v^a v^b (metric.rs:37-52 with the
-// Christoffel symbols of kerr.rs:100-424).  For a cotangent w the adjoint
-// needs w . a and its gradient with respect to r, theta, the spin a and v.
-// With C_i(v, w) the coefficient of the i-th distinct symbol in w . a, the
-// value is sum_i C_i Gamma_i; the v-gradient is sum_i Gamma_i dC_i/dv
-// (exact: a is quadratic in v); the (r, theta, a)-gradient comes from one
-// backward sweep through the symbol algebra, seeded with Gamma_i-bar = C_i,
-// which costs about twice the forward algebra instead of three tangent
-// passes.  Plain C++ on value types: the same source compiles for the GPU
-// and for a host-side check.
+// Christoffel symbols of kerr.rs:100-424).  For a cotangent lam the adjoint
+// needs the gradient of L = lam . a with respect to r, theta, the spin a and
+// v.  With the index lowered, a^mu = -g^{mu nu} f_nu and
+//   f_nu = v^r w_nu + v^theta u_nu - 1/2 delta_nu^r (w . v) - 1/2 delta_nu^theta (u . v),
+//   w_nu = d_r g_{nu b} v^b,  u_nu = d_theta g_{nu b} v^b
+// (the metric depends on r and theta only), so L = -Lam . f with
+// Lam_nu = lam_mu g^{mu nu}.  The v-gradient is exact in closed form (f is
+// quadratic in v); the (r, theta, a)-gradient comes from the cotangents of
+// the five inverse-metric components and the ten metric derivatives, swept
+// backwards through the metric algebra: about twice the forward algebra
+// instead of three tangent passes.  Plain C++ on value types: the same
+// source compiles for the GPU and for a host-side check.
 #pragma once
 
 #include <math.h>
@@ -19,77 +22,37 @@
 
 namespace gg {
 
-// Coefficient of symbol i in w . a.  Symbol order:
-// 001 002 013 023 100 111 122 133 112 103 200 211 222 233 212 203 301 302 313 323
-GG_HD inline double gamma_coef(int i, d4 v, d4 w) {
-  GG_CONTRACT_FAST
-  switch (i) {
-    case 0: return -2.0 * w.x * v.x * v.y;
-    case 1: return -2.0 * w.x * v.x * v.z;
-    case 2: return -2.0 * w.x * v.y * v.w;
-    case 3: return -2.0 * w.x * v.z * v.w;
-    case 4: return -w.y * v.x * v.x;
-    case 5: return -w.y * v.y * v.y;
-    case 6: return -w.y * v.z * v.z;
-    case 7: return -w.y * v.w * v.w;
-    case 8: return -2.0 * w.y * v.y * v.z;
-    case 9: return -2.0 * w.y * v.x * v.w;
-    case 10: return -w.z * v.x * v.x;
-    case 11: return -w.z * v.y * v.y;
-    case 12: return -w.z * v.z * v.z;
-    case 13: return -w.z * v.w * v.w;
-    case 14: return -2.0 * w.z * v.y * v.z;
-    case 15: return -2.0 * w.z * v.x * v.w;
-    case 16: return -2.0 * w.w * v.x * v.y;
-    case 17: return -2.0 * w.w * v.x * v.z;
-    case 18: return -2.0 * w.w * v.y * v.w;
-    default: return -2.0 * w.w * v.z * v.w;
-  }
-}
-
-// gv += Gamma_i dC_i/dv.
-GG_HD inline void gamma_coef_dv(int i, double g, d4 v, d4 w, d4 &gv) {
-  GG_CONTRACT_FAST
-  switch (i) {
-    case 0: { double t = -2.0 * w.x * g; gv.x += t * v.y; gv.y += t * v.x; break; }
-    case 1: { double t = -2.0 * w.x * g; gv.x += t * v.z; gv.z += t * v.x; break; }
-    case 2: { double t = -2.0 * w.x * g; gv.y += t * v.w; gv.w += t * v.y; break; }
-    case 3: { double t = -2.0 * w.x * g; gv.z += t * v.w; gv.w += t * v.z; break; }
-    case 4: gv.x += -2.0 * w.y * g * v.x; break;
-    case 5: gv.y += -2.0 * w.y * g * v.y; break;
-    case 6: gv.z += -2.0 * w.y * g * v.z; break;
-    case 7: gv.w += -2.0 * w.y * g * v.w; break;
-    case 8: { double t = -2.0 * w.y * g; gv.y += t * v.z; gv.z += t * v.y; break; }
-    case 9: { double t = -2.0 * w.y * g; gv.x += t * v.w; gv.w += t * v.x; break; }
-    case 10: gv.x += -2.0 * w.z * g * v.x; break;
-    case 11: gv.y += -2.0 * w.z * g * v.y; break;
-    case 12: gv.z += -2.0 * w.z * g * v.z; break;
-    case 13: gv.w += -2.0 * w.z * g * v.w; break;
-    case 14: { double t = -2.0 * w.z * g; gv.y += t * v.z; gv.z += t * v.y; break; }
-    case 15: { double t = -2.0 * w.z * g; gv.x += t * v.w; gv.w += t * v.x; break; }
-    case 16: { double t = -2.0 * w.w * g; gv.x += t * v.y; gv.y += t * v.x; break; }
-    case 17: { double t = -2.0 * w.w * g; gv.x += t * v.z; gv.z += t * v.x; break; }
-    case 18: { double t = -2.0 * w.w * g; gv.y += t * v.w; gv.w += t * v.y; break; }
-    default: { double t = -2.0 * w.w * g; gv.z += t * v.w; gv.w += t * v.z; break; }
-  }
+// 1/x: on the device v_rcp_f64 and two Newton steps (~1 ulp; x is a
+// positive normal quantity here: Sigma, Delta or sin^2 theta), on the host
+// (the check build) the division.
+GG_HD inline double vjp_rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+#else
+  return 1.0 / x;
+#endif
 }
 
 struct RhsBar {
-  double r, th, a;   // w^T da/dr, w^T da/dtheta, w^T da/dspin
-  d4 v;              // w^T da/dv
+  double r, th, a;   // lam^T da/dr, lam^T da/dtheta, lam^T da/dspin
+  d4 v;              // lam^T da/dv
 };
 
-GG_HD inline RhsBar kerr_rhs_vjp(double m, double a, double r, double th, d4 v, d4 w) {
+// The VJP at (r, theta) given st = sin theta, ct = cos theta.
+GG_HD inline RhsBar kerr_rhs_vjp_sc(double m, double a, double r, double st, double ct, d4 v, d4 lam) {
   GG_CONTRACT_FAST
-  // ---- forward: the symbol algebra of rhs64f_kerr, intermediates kept ----
-  const double st = sin(th), ct = cos(th);
+  // ---- forward: inverse metric and metric derivatives, intermediates kept ----
   const double tm = 2.0 * m;
   const double a2 = a * a, r2 = r * r, sin2 = st * st, sc = st * ct, ct2 = ct * ct;
   const double sigma = r2 + a2 * ct2;
   const double delta = r2 - tm * r + a2;
-  const double is = 1.0 / sigma, id = 1.0 / delta;
+  const double is = vjp_rcp(sigma), id = vjp_rcp(delta);
   const double is2 = is * is, isd = is * id;
-  const double dsig = -2.0 * a2 * sc;             // d sigma / d theta
+  const double dsig = -2.0 * a2 * sc;             // d sigma / d theta = d g_thth / d theta
   const double rp = r2 + a2;
   const double sm = sigma - 2.0 * r2;
   const double dtt_r = tm * sm * is2;             // d g_tt / dr
@@ -107,57 +70,64 @@ GG_HD inline RhsBar kerr_rhs_vjp(double m, double a, double r, double th, d4 v, 
   const double gtt = -Y * isd;                    // g^tt
   const double gtp = -tm * a * r * isd;           // g^t phi
   const double X = delta - a2 * sin2;
-  const double isn = 1.0 / sin2;
+  const double isn = vjp_rcp(sin2);
   const double gpp = X * isd * isn;               // g^phi phi
-  const double drr_t = dsig * id;
-  const double hrr = 0.5 * delta * is;            // g^rr / 2
+  const double grr = delta * is;                  // g^rr
+  const double drr_t = dsig * id;                 // d g_rr / d theta
   const double N = 2.0 * r * delta - sigma * (2.0 * r - tm);
   const double id2 = id * id;
-  const double drr_r = N * id2;
-  const double hth = 0.5 * is;                    // g^theta theta / 2
+  const double drr_r = N * id2;                   // d g_rr / dr
+  const double two_r = 2.0 * r;                   // d g_thth / dr
 
-  const double G[20] = {
-      0.5 * (gtt * dtt_r + gtp * dtp_r), 0.5 * (gtt * dtt_t + gtp * dtp_t),
-      0.5 * (gtt * dtp_r + gtp * dpp_r), 0.5 * (gtt * dtp_t + gtp * dpp_t),
-      -hrr * dtt_r, hrr * drr_r, -hrr * (2.0 * r), -hrr * dpp_r, hrr * drr_t, -hrr * dtp_r,
-      -hth * dtt_t, -hth * drr_t, hth * dsig, -hth * dpp_t, hth * (2.0 * r), -hth * dtp_t,
-      0.5 * (gtp * dtt_r + gpp * dtp_r), 0.5 * (gtp * dtt_t + gpp * dtp_t),
-      0.5 * (gtp * dtp_r + gpp * dpp_r), 0.5 * (gtp * dtp_t + gpp * dpp_t)};
-  double C[20];
+  // ---- the contraction with lowered index ----
+  const double vt = v.x, vr = v.y, vth = v.z, vph = v.w;
+  const double w_t = dtt_r * vt + dtp_r * vph, w_p = dtp_r * vt + dpp_r * vph;
+  const double w_r = drr_r * vr, w_th = two_r * vth;
+  const double u_t = dtt_t * vt + dtp_t * vph, u_p = dtp_t * vt + dpp_t * vph;
+  const double u_r = drr_t * vr, u_th = dsig * vth;
+  const double wv = w_t * vt + w_p * vph + w_r * vr + w_th * vth;
+  const double uv = u_t * vt + u_p * vph + u_r * vr + u_th * vth;
+  const double f_t = vr * w_t + vth * u_t, f_p = vr * w_p + vth * u_p;
+  const double f_r = vr * w_r + vth * u_r - 0.5 * wv;
+  const double f_th = vr * w_th + vth * u_th - 0.5 * uv;
+  const double L_t = lam.x * gtt + lam.w * gtp, L_p = lam.x * gtp + lam.w * gpp;
+  const double L_r = lam.y * grr, L_th = lam.z * is;
+
+  // ---- v-gradient: -Lam . df/dv ----
+  const double Lw = L_t * w_t + L_p * w_p + L_r * w_r + L_th * w_th;
+  const double Lu = L_t * u_t + L_p * u_p + L_r * u_r + L_th * u_th;
   RhsBar o;
-  o.v = d4{0.0, 0.0, 0.0, 0.0};
-  for (int i = 0; i < 20; ++i) {
-    C[i] = gamma_coef(i, v, w);
-    gamma_coef_dv(i, G[i], v, w, o.v);
-  }
+  o.v.x = -vr * (dtt_r * L_t + dtp_r * L_p) - vth * (dtt_t * L_t + dtp_t * L_p) + L_r * w_t + L_th * u_t;
+  o.v.w = -vr * (dtp_r * L_t + dpp_r * L_p) - vth * (dtp_t * L_t + dpp_t * L_p) + L_r * w_p + L_th * u_p;
+  o.v.y = -Lw - vr * (drr_r * L_r) - vth * (drr_t * L_r) + L_r * w_r + L_th * u_r;
+  o.v.z = -Lu - vr * (two_r * L_th) - vth * (dsig * L_th) + L_r * w_th + L_th * u_th;
 
-  // ---- backward: Gamma_i-bar = C_i ----
-  const double b_gtt = 0.5 * (C[0] * dtt_r + C[1] * dtt_t + C[2] * dtp_r + C[3] * dtp_t);
-  const double b_gtp = 0.5 * (C[0] * dtp_r + C[1] * dtp_t + C[2] * dpp_r + C[3] * dpp_t + C[16] * dtt_r +
-                              C[17] * dtt_t + C[18] * dtp_r + C[19] * dtp_t);
-  const double b_gpp = 0.5 * (C[16] * dtp_r + C[17] * dtp_t + C[18] * dpp_r + C[19] * dpp_t);
-  const double b_dtt_r = 0.5 * (C[0] * gtt + C[16] * gtp) - C[4] * hrr;
-  const double b_dtt_t = 0.5 * (C[1] * gtt + C[17] * gtp) - C[10] * hth;
-  const double b_dtp_r = 0.5 * (C[0] * gtp + C[2] * gtt + C[16] * gpp + C[18] * gtp) - C[9] * hrr;
-  const double b_dtp_t = 0.5 * (C[1] * gtp + C[3] * gtt + C[17] * gpp + C[19] * gtp) - C[15] * hth;
-  const double b_dpp_r = 0.5 * (C[2] * gtp + C[18] * gpp) - C[7] * hrr;
-  const double b_dpp_t = 0.5 * (C[3] * gtp + C[19] * gpp) - C[13] * hth;
-  const double b_hrr = -C[4] * dtt_r + C[5] * drr_r - C[6] * (2.0 * r) - C[7] * dpp_r + C[8] * drr_t - C[9] * dtp_r;
-  const double b_drr_r = C[5] * hrr;
-  const double b_drr_t = C[8] * hrr - C[11] * hth;
-  const double b_hth =
-      -C[10] * dtt_t - C[11] * drr_t + C[12] * dsig - C[13] * dpp_t + C[14] * (2.0 * r) - C[15] * dtp_t;
-  double b_r = -2.0 * C[6] * hrr + 2.0 * C[14] * hth;
-  double b_dsig = C[12] * hth;
+  // ---- cotangents of the inverse metric and the metric derivatives ----
+  const double b_gtt = -lam.x * f_t;
+  const double b_gtp = -(lam.x * f_p + lam.w * f_t);
+  const double b_gpp = -lam.w * f_p;
+  const double b_grr = -lam.y * f_r;
+  double b_is = -lam.z * f_th;                                // g^thth = is
+  const double vt2 = vt * vt, vp2 = vph * vph, vtp = vt * vph;
+  const double b_dtt_r = -L_t * vr * vt + 0.5 * L_r * vt2;
+  const double b_dtt_t = -L_t * vth * vt + 0.5 * L_th * vt2;
+  const double b_dtp_r = -(L_t * vph + L_p * vt) * vr + L_r * vtp;
+  const double b_dtp_t = -(L_t * vph + L_p * vt) * vth + L_th * vtp;
+  const double b_dpp_r = -L_p * vr * vph + 0.5 * L_r * vp2;
+  const double b_dpp_t = -L_p * vth * vph + 0.5 * L_th * vp2;
+  const double b_drr_r = -0.5 * L_r * vr * vr;
+  const double b_drr_t = -L_r * vth * vr + 0.5 * L_th * vr * vr;
+  double b_r = 2.0 * (-L_th * vr * vth + 0.5 * L_r * vth * vth);      // d g_thth / dr = 2 r
+  double b_dsig = -0.5 * L_th * vth * vth;                            // d g_thth / d theta = dsig
 
-  double b_is = 0.5 * b_hth;                                  // hth = is / 2
+  // ---- backward through the metric algebra ----
   const double b_N = b_drr_r * id2;                           // drr_r = N id^2
   double b_id = b_drr_r * N * 2.0 * id;
   b_r += b_N * (2.0 * delta - 2.0 * sigma);                   // N = 2 r delta - sigma (2 r - 2m)
   double b_delta = b_N * 2.0 * r;
   double b_sigma = -b_N * (2.0 * r - tm);
-  b_delta += 0.5 * b_hrr * is;                                // hrr = delta is / 2
-  b_is += 0.5 * b_hrr * delta;
+  b_delta += b_grr * is;                                      // grr = delta is
+  b_is += b_grr * delta;
   b_dsig += b_drr_t * id;                                     // drr_t = dsig id
   b_id += b_drr_t * dsig;
   const double b_X = b_gpp * isd * isn;                       // gpp = X isd isn
@@ -239,6 +209,26 @@ GG_HD inline RhsBar kerr_rhs_vjp(double m, double a, double r, double th, d4 v, 
   o.th = b_st * ct - b_ct * st;                               // st = sin th, ct = cos th
   o.a = b_a;
   return o;
+}
+
+GG_HD inline RhsBar kerr_rhs_vjp(double m, double a, double r, double th, d4 v, d4 lam) {
+  return kerr_rhs_vjp_sc(m, a, r, sin(th), cos(th), v, lam);
+}
+
+// sin and cos of theta0 + d from s0 = sin theta0, c0 = cos theta0 by the
+// angle-addition formulas, with sin d and cos d from their Taylor series
+// through d^9 and d^10 (truncation below 2^-60 relative for |d| <= 1/16);
+// false when |d| > 1/16.  The adjoint's stage angles differ from the
+// attempt's start angle by h a_i v^theta, nearly always far below that.
+GG_HD inline bool sincos_near(double s0, double c0, double d, double &s, double &c) {
+  GG_CONTRACT_FAST
+  if (!(fabs(d) <= 0.0625)) return false;
+  const double d2 = d * d;
+  const double sd = d * (1.0 + d2 * (-1.0 / 6 + d2 * (1.0 / 120 + d2 * (-1.0 / 5040 + d2 * (1.0 / 362880)))));
+  const double cm = d2 * (-0.5 + d2 * (1.0 / 24 + d2 * (-1.0 / 720 + d2 * (1.0 / 40320 + d2 * (-1.0 / 3628800)))));
+  s = s0 + (s0 * cm + c0 * sd);                      // cos d = 1 + cm
+  c = c0 + (c0 * cm - s0 * sd);
+  return true;
 }
 
 }  // namespace gg

@@ -14,3 +14,7 @@ extern "C" void host_kerr_rhs_vjp(double m, double a, uint64_t n, const double *
     o[0] = b.r; o[1] = b.th; o[2] = b.v.x; o[3] = b.v.y; o[4] = b.v.z; o[5] = b.v.w; o[6] = b.a;
   }
 }
+
+extern "C" int host_sincos_near(double s0, double c0, double d, double *out2) {
+  return gg::sincos_near(s0, c0, d, out2[0], out2[1]) ? 1 : 0;
+}

@@ -143,7 +143,29 @@ def host_vjp(tmp_path_factory):
     lib = C.CDLL(so)
     lib.host_kerr_rhs_vjp.argtypes = [C.c_double, C.c_double, C.c_uint64] + [C.c_void_p] * 4
     lib.host_kerr_rhs_vjp.restype = None
+    lib.host_sincos_near.argtypes = [C.c_double, C.c_double, C.c_double, C.c_void_p]
+    lib.host_sincos_near.restype = C.c_int
     return lib
+
+
+def test_sincos_near_matches_libm(host_vjp):
+    """The adjoint's stage angles: sin and cos of theta0 + d by angle addition
+    with the Taylor series of sin d, cos d, against libm at theta0 + d (d
+    within 1/16), and the refusal beyond."""
+    rng = np.random.default_rng(5)
+    out = np.zeros(2)
+    worst = 0.0
+    for _ in range(20000):
+        th0 = rng.uniform(0.0, math.pi)
+        d = rng.uniform(-0.0625, 0.0625) * rng.choice([1.0, 1e-3, 1e-8])
+        assert host_vjp.host_sincos_near(math.sin(th0), math.cos(th0), d, out.ctypes.data) == 1
+        worst = max(worst, abs(out[0] - math.sin(th0 + d)), abs(out[1] - math.cos(th0 + d)))
+    print(f"max abs error {worst:.2e}")
+    assert worst < 5e-16
+    assert host_vjp.host_sincos_near(math.sin(1.0), math.cos(1.0), 0.0, out.ctypes.data) == 1
+    assert out[0] == math.sin(1.0) and out[1] == math.cos(1.0)
+    for d in (0.0626, -0.07, 1.0, float("nan")):
+        assert host_vjp.host_sincos_near(0.5, 0.8, d, out.ctypes.data) == 0
 
 
 @pytest.mark.parametrize("spin", [0.0, 0.3, 0.9, 0.998])
