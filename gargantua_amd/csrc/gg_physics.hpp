@@ -420,6 +420,12 @@ __device__ inline f4 rhs32_schw(float rs, float r, float psi, f4 v) {
 }
 
 template <int METRIC>
+__device__ inline f4 rhs32f(const DevScene &S, float r, float psi, f4 v) {
+  if (METRIC == 1) return rhs32_kerr((float)S.mass, (float)S.spin, r, psi, v);
+  return rhs32_schw((float)S.rs, r, psi, v);
+}
+
+template <int METRIC>
 __device__ inline f4 rhs32(const DevScene &S, double r, double theta, f4 v) {
   const float psi = (float)(theta - 1.5707963267948966);
   if (METRIC == 1) return rhs32_kerr((float)S.mass, (float)S.spin, (float)r, psi, v);

@@ -326,6 +326,100 @@ __device__ inline StageOut stages(const DevScene &S, const KStore &K, const doub
   return o;
 }
 
+// The stages of an FP32 attempt: the right-hand sides, stage positions r,
+// stage velocities and the sums sum c_i k_i and sum (c_i - d_i) k_i in FP32
+// with the k_i in registers; each stage's psi = theta - pi/2 formed in FP64
+// before its rounding (near the disk plane a rounded FP32 theta would lose
+// psi's relative precision); new_vel = v + h sum c_i k_i, the error
+// difference h sum (c_i - d_i) k_i and its norm in FP64.  The FP32 rounding
+// of the sums is a fraction of that of the right-hand sides, which
+// fp32_noise bounds.  An invalid stage (r <= 0 or not finite) sends the
+// attempt to FP64 like an error ratio near a decision.
+template <int METRIC, bool REC = false>
+__device__ inline StageOut stages32(const DevScene &S, const KStore &K, const double p0[4], const d4 vel, double h,
+                                    bool need_k1, double *rec = nullptr) {
+#pragma clang fp contract(fast)
+  StageOut o;
+  o.nr = 0;
+  o.invalid = false;
+  o.nv = d4{0, 0, 0, 0};
+  const float hf = (float)h;
+  const f4 vf{(float)vel.x, (float)vel.y, (float)vel.z, (float)vel.w};
+  const float r0 = (float)p0[1];
+  const double psi0 = p0[2] - HALF_PI;
+  f4 k[7];
+  if (need_k1) {
+    k[0] = rhs32f<METRIC>(S, r0, (float)psi0, vf);
+    ++o.nr;
+  } else {
+    const d4 kd = K.get(0);
+    k[0] = f4{(float)kd.x, (float)kd.y, (float)kd.z, (float)kd.w};
+  }
+  const float c1 = (float)C1, e1 = (float)(C1 - D1);
+  f4 s5{c1 * k[0].x, c1 * k[0].y, c1 * k[0].z, c1 * k[0].w};
+  f4 se{e1 * k[0].x, e1 * k[0].y, e1 * k[0].z, e1 * k[0].w};
+#pragma unroll
+  for (int s = 2; s <= 7; ++s) {
+    float rs, psis;
+    f4 vs;
+    if (s <= 6) {
+      const double ha = h * tabA(s);
+      rs = r0 + (float)ha * vf.y;
+      psis = (float)(psi0 + ha * vel.z);
+      if (!(rs > 0.0f && isfinite(rs) && isfinite(psis))) { o.invalid = true; return o; }
+      f4 sum{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int j = 1; j < s; ++j) {
+        const float b = (float)tabB(s, j);
+        sum.x += b * k[j - 1].x; sum.y += b * k[j - 1].y; sum.z += b * k[j - 1].z; sum.w += b * k[j - 1].w;
+      }
+      if (s == 2) sum = f4{(float)B21 * k[0].x, (float)B21 * k[0].y, (float)B21 * k[0].z, (float)B21 * k[0].w};
+      vs = f4{vf.x + hf * sum.x, vf.y + hf * sum.y, vf.z + hf * sum.z, vf.w + hf * sum.w};
+      if (REC && rec) {
+        double *d = (double *)__builtin_assume_aligned(rec + REC_VS + 4 * (s - 2), 32);
+        d[0] = vs.x; d[1] = vs.y; d[2] = vs.z; d[3] = vs.w;
+      }
+    } else {
+      rs = r0 + hf * vf.y;                                          // k7 at (pos6, new_vel)
+      psis = (float)(psi0 + h * vel.z);
+      vs = f4{(float)o.nv.x, (float)o.nv.y, (float)o.nv.z, (float)o.nv.w};
+    }
+    k[s - 1] = rhs32f<METRIC>(S, rs, psis, vs);
+    ++o.nr;
+    const f4 &kk = k[s - 1];
+    if (s >= 3 && s <= 6) {
+      const float c = (float)tabC(s);
+      s5.x += c * kk.x; s5.y += c * kk.y; s5.z += c * kk.z; s5.w += c * kk.w;
+    }
+    if (s >= 3) {
+      const float e = (float)(tabC(s) - tabD(s));
+      se.x += e * kk.x; se.y += e * kk.y; se.z += e * kk.z; se.w += e * kk.w;
+    }
+    if (s == 6)
+      o.nv = d4{__builtin_fma(h, (double)s5.x, vel.x), __builtin_fma(h, (double)s5.y, vel.y),
+                __builtin_fma(h, (double)s5.z, vel.z), __builtin_fma(h, (double)s5.w, vel.w)};
+  }
+  o.k7 = to64(k[6]);
+  const double dx = h * (double)se.x, dy = h * (double)se.y, dz = h * (double)se.z, dw = h * (double)se.w;
+  if (REC && rec) {
+    double *dst = (double *)__builtin_assume_aligned(rec + REC_DV, 32);
+    dst[0] = dx; dst[1] = dy; dst[2] = dz; dst[3] = dw;
+  }
+  const d4 nv = o.nv;
+  if (S.err_mode == 1) {
+    const double q0 = dx / (S.atol + S.rtol * fmax(fabs(vel.x), fabs(nv.x)));
+    const double q1 = dy / (S.atol + S.rtol * fmax(fabs(vel.y), fabs(nv.y)));
+    const double q2 = dz / (S.atol + S.rtol * fmax(fabs(vel.z), fabs(nv.z)));
+    const double q3 = dw / (S.atol + S.rtol * fmax(fabs(vel.w), fabs(nv.w)));
+    o.err = sqrt(0.25 * (q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3));
+    o.thresh = 1.0;
+  } else {
+    o.err = sqrt(dx * dx + dy * dy + dz * dz + dw * dw);
+    o.thresh = S.tol;
+  }
+  return o;
+}
+
 // Whether the next attempt of this ray may run its stages in FP32: the
 // controller is saturated (h == max_step, previous error ratio < 0.3), the
 // ray is far enough from the photon-orbit region that FP32 rounding is not
@@ -382,7 +476,7 @@ __device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool 
   bool done = false;
   if (PREC == 1 && use32) {
     const bool need_k1 = !L.k1_ok;
-    o = stages<METRIC, ARITH_F32, REC>(S, K, p0, vel, h, need_k1, rec);
+    o = stages32<METRIC, REC>(S, K, p0, vel, h, need_k1, rec);
     L.n_rhs += o.nr;
     L.n_rhs32 += o.nr;
     if (need_k1) { L.k1_ok = true; L.k1_f32 = true; }   // slot 0 now holds an FP32 k1
