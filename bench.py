@@ -617,32 +617,15 @@ def bench_grad512(args):
     # gradient-descent recovery (Adam on the raw parameters)
     gd = None
     if args.gd_steps > 0:
-        theta = np.array([g0.spin, g0.inclination, g0.emissivity_q, g0.r_inner])
-        truth = np.array([tgt_scene.spin, tgt_scene.inclination, tgt_scene.emissivity_q, tgt_scene.r_inner])
-        lr, b1, b2 = 0.02, 0.9, 0.999
-        m = np.zeros(4)
-        v = np.zeros(4)
-        hist = []
         t0 = time.perf_counter()
-        for it in range(1, args.gd_steps + 1):
-            g = cfg.grad_512_params(size, size, spin=theta[0], inclination_deg=math.degrees(theta[1]), q=theta[2],
-                                    r_in=theta[3], precision="mixed")
-            out = rend.grad(g, target_dev, checkpoint_every=0, want_image=False)
-            grad = out["grad"]
-            m = b1 * m + (1 - b1) * grad
-            v = b2 * v + (1 - b2) * grad * grad
-            step = lr * (m / (1 - b1 ** it)) / (np.sqrt(v / (1 - b2 ** it)) + 1e-12)
-            theta = theta - step
-            theta[0] = min(max(theta[0], -0.99), 0.99)
-            theta[1] = min(max(theta[1], 0.05), math.pi - 0.05)
-            theta[3] = max(theta[3], 1.0)
-            hist.append(out["loss"])
-            if it % 20 == 0 or it == 1:
-                _log(f"gd {it}: loss {out['loss']:.3e} theta {np.round(theta, 4).tolist()}")
+        fit = rend.fit(g0m, target_dev, args.gd_steps,
+                       log=lambda it, loss, th: _log(f"gd {it}: loss {loss:.3e} theta {np.round(th, 4).tolist()}"))
+        theta = fit["theta"][-1]
+        truth = np.array([tgt_scene.spin, tgt_scene.inclination, tgt_scene.emissivity_q, tgt_scene.r_inner])
         gd = {"steps": args.gd_steps, "optimizer": "Adam lr 0.02 (0.9, 0.999)", "seconds": time.perf_counter() - t0,
               "theta0": [g0.spin, g0.inclination, g0.emissivity_q, g0.r_inner], "theta_final": theta.tolist(),
               "theta_true": truth.tolist(), "abs_err": np.abs(theta - truth).tolist(),
-              "loss_first": hist[0], "loss_last": hist[-1]}
+              "loss_first": float(fit["loss"][0]), "loss_last": float(fit["loss"][-1])}
     rend.close()
     best = per_k["auto+order mixed"]
     return {"metric": f"adjoint gradient evaluations/s, {size}x{size} Kerr thin disk, dL/d(a, i, q, r_in)",

@@ -620,6 +620,34 @@ class Renderer:
                                             C.byref(st)), "gg_render")
         return o["intensity"]
 
+    def fit(self, gscene, target, steps: int, lr: float = 0.02, betas=(0.9, 0.999), log=None) -> dict:
+        """Gradient descent on theta = (spin, inclination, q, r_inner) of `gscene` towards the image `target`
+        (Adam on the raw parameters, bias-corrected; each step one grad() with records kept by the render).
+        The iterate stays inside |spin| <= 0.99, inclination in [0.05, pi - 0.05] and r_inner >= 1.
+        Returns theta after each step and the loss before it; `log(it, loss, theta)` is called on steps 1, 20, 40..."""
+        import copy
+        b1, b2 = betas
+        theta = np.array([gscene.spin, gscene.inclination, gscene.emissivity_q, gscene.r_inner], float)
+        m = np.zeros(4)
+        v = np.zeros(4)
+        thetas, losses = [], []
+        g = copy.copy(gscene)
+        for it in range(1, steps + 1):
+            g.spin, g.inclination, g.emissivity_q, g.r_inner = (float(x) for x in theta)
+            out = self.grad(g, target, checkpoint_every=0, want_image=False)
+            grad = out["grad"]
+            m = b1 * m + (1 - b1) * grad
+            v = b2 * v + (1 - b2) * grad * grad
+            theta = theta - lr * (m / (1 - b1 ** it)) / (np.sqrt(v / (1 - b2 ** it)) + 1e-12)
+            theta[0] = min(max(theta[0], -0.99), 0.99)
+            theta[1] = min(max(theta[1], 0.05), math.pi - 0.05)
+            theta[3] = max(theta[3], 1.0)
+            thetas.append(theta.copy())
+            losses.append(out["loss"])
+            if log is not None and (it == 1 or it % 20 == 0):
+                log(it, out["loss"], theta)
+        return {"theta": np.array(thetas), "loss": np.array(losses)}
+
 
 _default_renderer: Optional[Renderer] = None
 

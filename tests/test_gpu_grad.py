@@ -288,3 +288,25 @@ def test_grad_rejects_bad_arguments(renderer):
         renderer.grad(bad, np.zeros(N * N))
     with pytest.raises(gg.GargantuaError):
         renderer.geodesic_acceleration_vjp(gg.Schwarzschild(1.0), np.ones((1, 4)), np.ones((1, 4)), np.ones((1, 4)))
+
+
+def test_gradient_descent_recovers_parameters(renderer):
+    """200 Adam steps from theta0 = (0.5, 45 deg, 3.0, r_ISCO + 0.5) on the
+    image at (0.7, 60 deg, 2.5, r_ISCO(0.7)), mixed forward render: the loss
+    falls by orders of magnitude and the well-determined parameters
+    (inclination, q) approach their true values closely.  Spin and r_in
+    move the image in nearly the same way and converge last."""
+    n = 96
+    tgt = _scene(n, spin=0.7, inclination_deg=60.0, q=2.5, r_in=configs.r_isco(1.0, 0.7))
+    truth = np.array([tgt.spin, tgt.inclination, tgt.emissivity_q, tgt.r_inner])
+    target = renderer.grad_image(tgt)
+    g0 = _scene(n, precision="mixed")
+    theta0 = np.array([g0.spin, g0.inclination, g0.emissivity_q, g0.r_inner])
+    fit = renderer.fit(g0, target, 200, log=lambda it, loss, th: print(f"step {it}: loss {loss:.3e} theta {th}"))
+    theta = fit["theta"][-1]
+    err0, err = np.abs(theta0 - truth), np.abs(theta - truth)
+    print("loss", fit["loss"][0], "->", fit["loss"][-1], "|theta - truth|", err0, "->", err)
+    # measured: loss 3.3e-5 -> 1.5e-9; |theta - truth| (0.2, 0.26, 0.5, 1.34) -> (0.031, 0.0030, 0.0007, 0.021)
+    assert fit["loss"][-1] < 1e-3 * fit["loss"][0]
+    assert err[1] < 0.05 * err0[1] and err[2] < 0.01 * err0[2]
+    assert err[0] < 0.25 * err0[0] and err[3] < 0.05 * err0[3]

@@ -149,3 +149,24 @@ def test_horizon_shadow_and_escape(oracle):
     rgb = oracle.render(scene, rgb=True)["rgb"]
     assert np.any(np.all(rgb == 0, axis=1)), "expected at least one black pixel (horizon shadow)"
     assert np.any(np.any(rgb != 0, axis=1)), "expected at least one non-black pixel (escaped rays)"
+
+
+def test_kerr_spin_clamp(oracle):
+    # kerr.rs:27-32: |spin| > mass is clamped to +-0.9999 mass, so the
+    # horizon stays M + sqrt(M^2 - a^2) with a = 0.9999 M, and every metric
+    # quantity equals that of the clamped spin.
+    a_max = 0.9999
+    assert oracle.event_horizon(gg.Kerr(1.0, 1.5)) == 1.0 + math.sqrt(1.0 - a_max * a_max)
+    assert oracle.event_horizon(gg.Kerr(2.0, -7.0)) == 2.0 + math.sqrt(4.0 - (2.0 * a_max) ** 2)
+    rng = np.random.default_rng(11)
+    for sign in (1.0, -1.0):
+        over, at = gg.Kerr(1.0, sign * 1.5), gg.Kerr(1.0, sign * a_max)
+        for _ in range(8):
+            pos = (0.0, rng.uniform(2.0, 50.0), rng.uniform(0.1, math.pi - 0.1), rng.uniform(0.0, 6.0))
+            assert np.array_equal(oracle.metric_tensor(over, pos), oracle.metric_tensor(at, pos))
+        pos = np.array([[0.0, 7.0, 1.1, 0.3]])
+        vel = np.array([[1.0, -0.4, 0.01, 0.02]])
+        assert np.array_equal(oracle.geodesic_acceleration(over, pos, vel),
+                              oracle.geodesic_acceleration(at, pos, vel))
+    # inside the bound the spin is kept as given
+    assert oracle.event_horizon(gg.Kerr(1.0, 0.5)) == 1.0 + math.sqrt(0.75)
