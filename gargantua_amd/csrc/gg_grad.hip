@@ -364,8 +364,8 @@ __device__ inline void put_gv(double *rec, d4 v) {
 
 __global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
 #pragma clang fp contract(off)
-  extern __shared__ double lds_k[];
-  const KStore K{lds_k + threadIdx.x, blockDim.x};
+  extern __shared__ __align__(16) double lds_k[];
+  const KStore K{reinterpret_cast<double2 *>(lds_k) + threadIdx.x, blockDim.x};
   const uint32_t lane = threadIdx.x & 63;
   const DevScene &S = G.scene;
   const uint32_t KK = G.K;
@@ -469,8 +469,8 @@ template <int MODE>
 __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
 #pragma clang fp contract(off)
   constexpr bool RECOMPUTE = (MODE == 0);
-  extern __shared__ double lds_k[];
-  const KStore K{lds_k + threadIdx.x, blockDim.x};
+  extern __shared__ __align__(16) double lds_k[];
+  const KStore K{reinterpret_cast<double2 *>(lds_k) + threadIdx.x, blockDim.x};
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const DevScene &S = G.scene;

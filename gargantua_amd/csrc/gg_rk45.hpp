@@ -73,15 +73,21 @@ struct Lane {
   uint32_t n_refl;   // accepted steps whose theta was reflected at a pole
 };
 
-// Per-lane view of the LDS stage store: slots 0..4 hold k1..k5, structure of
-// arrays [slot][component][lane] so each ds_read_b64 of a wavefront touches
-// 64 consecutive doubles.
+// Per-lane view of the LDS stage store: slots 0..4 hold k1..k5.  Slot s is
+// two planes of 16-byte pairs, (x, y) of every lane in plane 2s and (z, w) in
+// plane 2s + 1, so a slot moves with two ds_read_b128 / ds_write_b128 per
+// lane and each plane a wavefront touches is 1 KiB of consecutive LDS.
 struct KStore {
-  double *p;        // &lds[threadIdx.x]
-  uint32_t stride;  // blockDim.x
-  __device__ double &at(int slot, int c) const { return p[(slot * 4 + c) * stride]; }
-  __device__ d4 get(int slot) const { return d4{at(slot, 0), at(slot, 1), at(slot, 2), at(slot, 3)}; }
-  __device__ void put(int slot, d4 k) const { at(slot, 0) = k.x; at(slot, 1) = k.y; at(slot, 2) = k.z; at(slot, 3) = k.w; }
+  double2 *p;       // &plane0[threadIdx.x] (LDS base 16-byte aligned)
+  uint32_t stride;  // blockDim.x: pairs per plane
+  __device__ d4 get(int slot) const {
+    const double2 a = p[(2 * slot) * stride], b = p[(2 * slot + 1) * stride];
+    return d4{a.x, a.y, b.x, b.y};
+  }
+  __device__ void put(int slot, d4 k) const {
+    p[(2 * slot) * stride] = double2{k.x, k.y};
+    p[(2 * slot + 1) * stride] = double2{k.z, k.w};
+  }
 };
 
 __device__ inline bool valid4(double p0, double p1, double p2, double p3) {

@@ -66,8 +66,8 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
   const uint64_t n = T.n_rays;
   constexpr bool single = (BATCH == 0);
 
-  extern __shared__ double lds_k[];
-  const KStore K{lds_k + threadIdx.x, blockDim.x};
+  extern __shared__ __align__(16) double lds_k[];
+  const KStore K{reinterpret_cast<double2 *>(lds_k) + threadIdx.x, blockDim.x};
   Lane L;
   L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0; L.n_refl = 0;
   uint32_t n_att = 0, n_retired = 0;

@@ -233,7 +233,9 @@ typedef struct gg_stats {
 /* Launch shape of the persistent trace kernel. 0 = library default. */
 typedef struct gg_launch {
   uint32_t block_threads;     /* 64..256, multiple of 64 (default 64)         */
-  uint32_t blocks_per_cu;     /* blocks launched per CU (default 8)           */
+  uint32_t blocks_per_cu;     /* blocks launched per CU; 0 (default) = by ray
+                                 count: 8 wavefronts per CU when every lane
+                                 gets at least 3 rays, else 4 (one per SIMD) */
   uint32_t refill_interval;   /* attempts between retired-lane refills (64)   */
   uint32_t graph;             /* gg_render_batch: 1 = capture the launch in a
                                  hipGraph and replay it on later calls with the
