@@ -58,14 +58,9 @@ struct Adj {
   double hn;        // h of the next attempt
 };
 
-// The RHS VJP at (r, theta0 + d), sin and cos of the angle from those of
-// theta0 (s0, c0) when d is small.
-__device__ inline RhsBar vjp_near(double m, double a, double r, double th0, double s0, double c0, double d, d4 v,
-                                  d4 lam) {
-  double s, c;
-  if (!sincos_near(s0, c0, d, s, c)) sincos(th0 + d, &s, &c);
-  return kerr_rhs_vjp_sc(m, a, r, s, c, v, lam);
-}
+// Block sizes of the re-trace and reverse kernels, fixed at compile time (the
+// LDS slot offsets are instruction immediates); the launches use these.
+constexpr uint32_t FWD_BLOCK = 256, REV_BLOCK = 64;
 
 // One step record in registers: the reverse sweep loads the record of the
 // attempt before the current one while it back-propagates the current one,
@@ -80,16 +75,16 @@ __device__ inline void load_rec(StepRec &R, const double *src) {
   for (int k = 0; k < STEP_REC; ++k) R.d[k] = p[k];
 }
 
-// LDS slots 6..10 of the reverse kernels: the current record's stage
-// velocities v_2..v_6, read by the rolled stage loop.
-constexpr int SLOT_VS = 6;
+// LDS slots 5..8 of the reverse kernels: the current record's stage
+// velocities v_2..v_5, read by the rolled stage loop.
+constexpr int SLOT_VS = 5;
 
 // Adjoint of one recorded attempt (geodesic.rs:185-328), whose record holds
 // its start (r, theta), h, kind, a velocity, the stage velocities v_2..v_6
 // and the error difference new_vel - vel_err.  On entry A holds the
 // cotangents of the attempt's outputs (state and next h); on exit those of
-// its inputs.  LDS slots 0..5 hold the stage cotangents kbar_1..kbar_6,
-// slots 6..10 the stage velocities.
+// its inputs.  LDS slots 0..4 hold the stage cotangents kbar_1..kbar_5,
+// slots 5..8 the stage velocities v_2..v_5.
 //
 // Accepted: pos' = p + h v (normalised), v' = v + h sum_i c_i k_i,
 //   k_i = a(p + h a_i v, v_i).  Rejected: p, v pass through.  Invalid: h/2.
@@ -105,7 +100,10 @@ constexpr int SLOT_VS = 6;
 //   VJP's own velocity output dotted with u.
 // !live: the step sizes are held fixed; rejected and invalid attempts then
 //   move nothing.
-__device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const StepRec &R, bool live, Adj &A) {
+// Shared work: k7 and stage 6 (a_6 = 1) sit at the same point p + h v, so
+// one metric algebra (kerr_point) serves both products.
+template <class KS>
+__device__ inline void attempt_adjoint(const KS &K, const DevScene &S, const StepRec &R, bool live, Adj &A) {
   GG_CONTRACT_FAST
   const double *rec = R.d;
   const double h = rec[REC_H];
@@ -157,6 +155,12 @@ __device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const
   }
   if (!acc && eb == 0.0) { A.hb = hbar; return; }
 
+  // the end point p + h v (before normalisation): k7's point and stage 6's
+  const double dth7 = h * v.z;
+  double s7, c7;
+  if (!sincos_near(s0, c0, dth7, s7, c7)) sincos(th0 + dth7, &s7, &c7);
+  const KerrPoint P7 = kerr_point(m, a, h * v.y + r0, s7, c7);
+
   double pr = A.pr, pth = A.pth;
   d4 vbar = acc ? d4{0.0, 0.0, 0.0, 0.0} : A.vb;       // a rejection passes p, v through
   d4 nvb{0.0, 0.0, 0.0, 0.0}, veb{0.0, 0.0, 0.0, 0.0}; // cotangents of new_vel, vel_err
@@ -202,8 +206,7 @@ __device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const
     veb = d4{-db.x, -db.y, -db.z, -db.w};
     // k7 = a(p + h v, new_vel) (geodesic.rs:298), weight h d7 in vel_err
     const double hd7 = h * D7;
-    const RhsBar jb = vjp_near(m, a, h * v.y + r0, th0, s0, c0, h * v.z, nv,
-                               d4{hd7 * veb.x, hd7 * veb.y, hd7 * veb.z, hd7 * veb.w});
+    const RhsBar jb = kerr_rhs_vjp_pt(P7, nv, d4{hd7 * veb.x, hd7 * veb.y, hd7 * veb.z, hd7 * veb.w});
     A.ga += jb.a;
     pr += jb.r;
     pth += jb.th;
@@ -213,54 +216,67 @@ __device__ inline void attempt_adjoint(const KStore &K, const DevScene &S, const
     nvb.x += jb.v.x; nvb.y += jb.v.y; nvb.z += jb.v.z; nvb.w += jb.v.w;
   }
 #pragma unroll
-  for (int i = 2; i <= 6; ++i) {
+  for (int i = 2; i <= 5; ++i) {
     const double *o = rec + REC_VS + 4 * (i - 2);
     K.put(SLOT_VS + i - 2, d4{o[0], o[1], o[2], o[3]});
   }
-  // (kB rows are zero beyond the diagonal: the k_j cotangent updates below
-  // run over j = 1..5 for every stage, their LDS reads issued together)
   // new_vel = v + h sum_i c_i k_i, vel_err = v + h sum_i d_i k_i
   vbar.x += nvb.x + veb.x; vbar.y += nvb.y + veb.y; vbar.z += nvb.z + veb.z; vbar.w += nvb.w + veb.w;
+  d4 w6;
 #pragma unroll
   for (int i = 1; i <= 6; ++i) {
     const double hc = h * tabC(i), hd = h * tabD(i);
-    K.put(i - 1, d4{hc * nvb.x + hd * veb.x, hc * nvb.y + hd * veb.y, hc * nvb.z + hd * veb.z,
-                    hc * nvb.w + hd * veb.w});
+    const d4 wi{hc * nvb.x + hd * veb.x, hc * nvb.y + hd * veb.y, hc * nvb.z + hd * veb.z, hc * nvb.w + hd * veb.w};
+    if (i == 6) w6 = wi;
+    else K.put(i - 1, wi);
   }
-#pragma unroll 1
-  for (int i = 6; i >= 1; --i) {
-    const d4 w = K.get(i - 1);
-    double qr = r0;
-    d4 vs = v;
-    double ha = 0.0;
-    if (i >= 2) {
-      // stage position p + h a_i v, stage velocity v + h sum_j b_ij k_j
-      ha = h * kA[i];
-      qr = ha * v.y + r0;
-      vs = K.get(SLOT_VS + i - 2);
-    }
-    const RhsBar jb = vjp_near(m, a, qr, th0, s0, c0, ha * v.z, vs, w);
+  // one stage's product: the velocity, position and h cotangents, and
+  // h b_ij vbar_i into the cotangents of the earlier stages j < i (k_j); the
+  // next stage's (j = i - 1) is returned in registers, the others go back to LDS
+  auto stage = [&](int i, const KerrPoint &P, double ha, const d4 vs, const d4 w) {
+    const RhsBar jb = kerr_rhs_vjp_pt(P, vs, w);
     A.ga += jb.a;
     pr += jb.r;
     pth += jb.th;
     vbar.x += jb.v.x; vbar.y += jb.v.y; vbar.z += jb.v.z; vbar.w += jb.v.w;
     // the combination's h-derivative (Euler: w . a(x, u) = (J_u^T w) . u / 2)
     if (live) hbar += 0.5 * ih * (jb.v.x * vs.x + jb.v.y * vs.y + jb.v.z * vs.z + jb.v.w * vs.w);
+    d4 wn{0.0, 0.0, 0.0, 0.0};
     if (i >= 2) {
       vbar.y += ha * jb.r;
       vbar.z += ha * jb.th;
       if (live) hbar += kA[i] * (jb.r * v.y + jb.th * v.z);
       d4 kb[5];
 #pragma unroll
-      for (int j = 1; j <= 5; ++j) kb[j - 1] = K.get(j - 1);
+      for (int j = 1; j <= 5; ++j)
+        if (j < i) kb[j - 1] = K.get(j - 1);
 #pragma unroll
       for (int j = 1; j <= 5; ++j) {
-        const double hb = h * kB[i][j];
-        kb[j - 1].x += hb * jb.v.x; kb[j - 1].y += hb * jb.v.y; kb[j - 1].z += hb * jb.v.z; kb[j - 1].w += hb * jb.v.w;
-        K.put(j - 1, kb[j - 1]);
+        if (j < i) {
+          const double hb = h * kB[i][j];
+          kb[j - 1].x += hb * jb.v.x; kb[j - 1].y += hb * jb.v.y; kb[j - 1].z += hb * jb.v.z; kb[j - 1].w += hb * jb.v.w;
+          if (j == i - 1) wn = kb[j - 1];
+          else K.put(j - 1, kb[j - 1]);
+        }
       }
     }
+    return wn;
+  };
+  // stage 6 at the end point (a_6 = 1), its velocity from the record
+  d4 w;
+  {
+    const double *o = rec + REC_VS + 4 * 4;
+    w = stage(6, P7, h, d4{o[0], o[1], o[2], o[3]}, w6);
   }
+#pragma unroll 1
+  for (int i = 5; i >= 2; --i) {
+    // stage position p + h a_i v, stage velocity v + h sum_j b_ij k_j
+    const double ha = h * kA[i];
+    double s, c;
+    if (!sincos_near(s0, c0, ha * v.z, s, c)) sincos(th0 + ha * v.z, &s, &c);
+    w = stage(i, kerr_point(m, a, ha * v.y + r0, s, c), ha, K.get(SLOT_VS + i - 2), w);
+  }
+  stage(1, kerr_point(m, a, r0, s0, c0), 0.0, v, w);
   A.pr = pr;
   A.pth = pth;
   A.vb = vbar;
@@ -362,10 +378,10 @@ __device__ inline void put_gv(double *rec, d4 v) {
   rec[R_GV0] = v.x; rec[R_GV1] = v.y; rec[R_GV2] = v.z; rec[R_GV3] = v.w;
 }
 
-__global__ __launch_bounds__(256, 2) void adjoint_fwd_kernel(GradLaunch G) {
+__global__ __launch_bounds__(FWD_BLOCK, 2) void adjoint_fwd_kernel(GradLaunch G) {
 #pragma clang fp contract(off)
   extern __shared__ __align__(16) double lds_k[];
-  const KStore K{reinterpret_cast<double2 *>(lds_k) + threadIdx.x, blockDim.x};
+  const KStoreT<FWD_BLOCK> K{reinterpret_cast<double2 *>(lds_k) + threadIdx.x, FWD_BLOCK};
   const uint32_t lane = threadIdx.x & 63;
   const DevScene &S = G.scene;
   const uint32_t KK = G.K;
@@ -470,7 +486,7 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
 #pragma clang fp contract(off)
   constexpr bool RECOMPUTE = (MODE == 0);
   extern __shared__ __align__(16) double lds_k[];
-  const KStore K{reinterpret_cast<double2 *>(lds_k) + threadIdx.x, blockDim.x};
+  const KStoreT<REV_BLOCK> K{reinterpret_cast<double2 *>(lds_k) + threadIdx.x, REV_BLOCK};
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const DevScene &S = G.scene;
@@ -625,9 +641,9 @@ __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
   }
 }
 
-__global__ __launch_bounds__(64) void adjoint_rev_kernel(GradLaunch G) { adjoint_rev_body<0>(G); }
-__global__ __launch_bounds__(64, 1) void adjoint_rev_stored_kernel(GradLaunch G) { adjoint_rev_body<1>(G); }
-__global__ __launch_bounds__(64, 1) void adjoint_rev_chunked_kernel(GradLaunch G) { adjoint_rev_body<2>(G); }
+__global__ __launch_bounds__(REV_BLOCK) void adjoint_rev_kernel(GradLaunch G) { adjoint_rev_body<0>(G); }
+__global__ __launch_bounds__(REV_BLOCK, 1) void adjoint_rev_stored_kernel(GradLaunch G) { adjoint_rev_body<1>(G); }
+__global__ __launch_bounds__(REV_BLOCK, 1) void adjoint_rev_chunked_kernel(GradLaunch G) { adjoint_rev_body<2>(G); }
 
 // Seeds of the recorded rays (the recording render replaces the re-trace):
 // the crossing step is the ray's last record, its end state the render's
@@ -817,18 +833,21 @@ __global__ __launch_bounds__(256) void vjp_kernel(const DevScene *S, uint64_t n,
 
 }  // namespace
 
-// LDS of the reverse kernel: six slots of 4 doubles (the recomputation's
-// k_1..k_5, the reverse step's cotangents kbar_1..kbar_6); of the forward
-// kernel: the trace kernel's k_1..k_5.
-size_t adjoint_lds_bytes(uint32_t block) { return (size_t)block * 11 * 4 * sizeof(double); }
+// LDS of the reverse kernels: nine slots of 4 doubles (the recomputation's
+// k_1..k_5, the reverse step's cotangents kbar_1..kbar_5 in the same slots,
+// its stage velocities v_2..v_5); of the forward kernel: the trace kernel's
+// k_1..k_5.
+size_t adjoint_lds_bytes(uint32_t block) { return (size_t)block * 9 * 4 * sizeof(double); }
 size_t adjoint_fwd_lds_bytes(uint32_t block) { return (size_t)block * 5 * 4 * sizeof(double); }
 
 hipError_t launch_adjoint_fwd(const GradLaunch &g, uint32_t grid, uint32_t block, hipStream_t s) {
+  if (block != FWD_BLOCK) return hipErrorInvalidValue;
   hipLaunchKernelGGL(adjoint_fwd_kernel, dim3(grid), dim3(block), adjoint_fwd_lds_bytes(block), s, g);
   return hipGetLastError();
 }
 
 hipError_t launch_adjoint_rev(const GradLaunch &g, int mode, uint32_t grid, uint32_t block, hipStream_t s) {
+  if (block != REV_BLOCK) return hipErrorInvalidValue;
   if (mode == 0)
     hipLaunchKernelGGL(adjoint_rev_kernel, dim3(grid), dim3(block), adjoint_lds_bytes(block), s, g);
   else if (mode == 1)

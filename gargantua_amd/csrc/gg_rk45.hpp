@@ -76,19 +76,24 @@ struct Lane {
 // Per-lane view of the LDS stage store: slots 0..4 hold k1..k5.  Slot s is
 // two planes of 16-byte pairs, (x, y) of every lane in plane 2s and (z, w) in
 // plane 2s + 1, so a slot moves with two ds_read_b128 / ds_write_b128 per
-// lane and each plane a wavefront touches is 1 KiB of consecutive LDS.
-struct KStore {
+// lane and each plane a wavefront touches is 1 KiB of consecutive LDS.  N > 0
+// fixes the block size (pairs per plane) at compile time, so that every slot
+// offset is an instruction immediate.
+template <uint32_t N = 0>
+struct KStoreT {
   double2 *p;       // &plane0[threadIdx.x] (LDS base 16-byte aligned)
-  uint32_t stride;  // blockDim.x: pairs per plane
+  uint32_t stride;  // blockDim.x: pairs per plane (N when N > 0)
+  __device__ uint32_t st() const { return N ? N : stride; }
   __device__ d4 get(int slot) const {
-    const double2 a = p[(2 * slot) * stride], b = p[(2 * slot + 1) * stride];
+    const double2 a = p[(2 * slot) * st()], b = p[(2 * slot + 1) * st()];
     return d4{a.x, a.y, b.x, b.y};
   }
   __device__ void put(int slot, d4 k) const {
-    p[(2 * slot) * stride] = double2{k.x, k.y};
-    p[(2 * slot + 1) * stride] = double2{k.z, k.w};
+    p[(2 * slot) * st()] = double2{k.x, k.y};
+    p[(2 * slot + 1) * st()] = double2{k.z, k.w};
   }
 };
+using KStore = KStoreT<0>;
 
 __device__ inline bool valid4(double p0, double p1, double p2, double p3) {
   return isfinite(p0) && isfinite(p1) && isfinite(p2) && isfinite(p3) && p1 > 0.0;
@@ -196,8 +201,8 @@ __device__ constexpr double tabD(int s) {
 // compile time); the FP64 loops stay rolled, their right-hand sides being
 // several times larger.
 // REC: write the stage velocities into rec (when non-null) as they form.
-template <int METRIC, int ARITH, bool REC = false>
-__device__ inline StageOut stages(const DevScene &S, const KStore &K, const double p0[4], const d4 vel, double h,
+template <int METRIC, int ARITH, bool REC = false, class KS = KStore>
+__device__ inline StageOut stages(const DevScene &S, const KS &K, const double p0[4], const d4 vel, double h,
                                   bool need_k1, double *rec = nullptr) {
 #pragma clang fp contract(off)
   constexpr bool UNROLL = (ARITH != ARITH_EXACT);
@@ -341,8 +346,8 @@ __device__ inline StageOut stages(const DevScene &S, const KStore &K, const doub
 // of the sums is a fraction of that of the right-hand sides, which
 // fp32_noise bounds.  An invalid stage (r <= 0 or not finite) sends the
 // attempt to FP64 like an error ratio near a decision.
-template <int METRIC, bool REC = false>
-__device__ inline StageOut stages32(const DevScene &S, const KStore &K, const double p0[4], const d4 vel, double h,
+template <int METRIC, bool REC = false, class KS = KStore>
+__device__ inline StageOut stages32(const DevScene &S, const KS &K, const double p0[4], const d4 vel, double h,
                                     bool need_k1, double *rec = nullptr) {
 #pragma clang fp contract(fast)
   StageOut o;
@@ -464,8 +469,8 @@ __device__ inline bool fp32_eligible(const DevScene &S, const Lane &L) {
 // Returns RES_*; `acc` reports acceptance; p0/v0 keep the pre-step state.
 // REC (recording render and adjoint kernels): rec, when non-null, receives
 // the step record of the attempt, whatever its outcome.
-template <int METRIC, int PREC, bool REC = false>
-__device__ inline int attempt(const DevScene &S, Lane &L, const KStore &K, bool &acc, double p0[4], d4 &v0,
+template <int METRIC, int PREC, bool REC = false, class KS = KStore>
+__device__ inline int attempt(const DevScene &S, Lane &L, const KS &K, bool &acc, double p0[4], d4 &v0,
                               bool use32, bool use_fast = false, double *rec = nullptr) {
 #pragma clang fp contract(off)
   constexpr int BASE = (PREC == 2) ? ARITH_FAST : ARITH_EXACT;

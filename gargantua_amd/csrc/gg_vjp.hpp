@@ -42,10 +42,19 @@ struct RhsBar {
   d4 v;              // lam^T da/dv
 };
 
-// The VJP at (r, theta) given st = sin theta, ct = cos theta.
-GG_HD inline RhsBar kerr_rhs_vjp_sc(double m, double a, double r, double st, double ct, d4 v, d4 lam) {
+// The metric algebra of the VJP at (r, theta), given st = sin theta and
+// ct = cos theta: the inverse metric, the metric derivatives and the
+// intermediates the backward sweep reads.  It does not depend on the
+// velocity or the cotangent, so one evaluation serves every product (and
+// the RHS value) at the same point.
+struct KerrPoint {
+  double m, a, r, st, ct;
+  double tm, a2, r2, sin2, sc, ct2, sigma, delta, is, id, is2, isd, dsig, rp, sm, dtt_r, dtt_t, dtp_r, U, dtp_t;
+  double bp, W, s4, dpp_r, Z, dpp_t, Y, gtt, gtp, X, isn, gpp, grr, drr_t, N, id2, drr_r, two_r;
+};
+
+GG_HD inline KerrPoint kerr_point(double m, double a, double r, double st, double ct) {
   GG_CONTRACT_FAST
-  // ---- forward: inverse metric and metric derivatives, intermediates kept ----
   const double tm = 2.0 * m;
   const double a2 = a * a, r2 = r * r, sin2 = st * st, sc = st * ct, ct2 = ct * ct;
   const double sigma = r2 + a2 * ct2;
@@ -79,6 +88,28 @@ GG_HD inline RhsBar kerr_rhs_vjp_sc(double m, double a, double r, double st, dou
   const double drr_r = N * id2;                   // d g_rr / dr
   const double two_r = 2.0 * r;                   // d g_thth / dr
 
+  KerrPoint P;
+  P.m = m; P.a = a; P.r = r; P.st = st; P.ct = ct;
+  P.tm = tm; P.a2 = a2; P.r2 = r2; P.sin2 = sin2; P.sc = sc; P.ct2 = ct2; P.sigma = sigma; P.delta = delta;
+  P.is = is; P.id = id; P.is2 = is2; P.isd = isd; P.dsig = dsig; P.rp = rp; P.sm = sm; P.dtt_r = dtt_r;
+  P.dtt_t = dtt_t; P.dtp_r = dtp_r; P.U = U; P.dtp_t = dtp_t; P.bp = bp; P.W = W; P.s4 = s4; P.dpp_r = dpp_r;
+  P.Z = Z; P.dpp_t = dpp_t; P.Y = Y; P.gtt = gtt; P.gtp = gtp; P.X = X; P.isn = isn; P.gpp = gpp;
+  P.grr = grr; P.drr_t = drr_t; P.N = N; P.id2 = id2; P.drr_r = drr_r; P.two_r = two_r;
+  return P;
+}
+
+// The VJP at a point of kerr_point.
+GG_HD inline RhsBar kerr_rhs_vjp_pt(const KerrPoint &P, d4 v, d4 lam) {
+  GG_CONTRACT_FAST
+  const double m = P.m, a = P.a, r = P.r, st = P.st, ct = P.ct;
+  const double tm = P.tm, a2 = P.a2, r2 = P.r2, sin2 = P.sin2, sc = P.sc, ct2 = P.ct2,
+               sigma = P.sigma, delta = P.delta, is = P.is, id = P.id, is2 = P.is2, isd = P.isd,
+               dsig = P.dsig, rp = P.rp, sm = P.sm, dtt_r = P.dtt_r, dtt_t = P.dtt_t, dtp_r = P.dtp_r,
+               U = P.U, dtp_t = P.dtp_t, bp = P.bp, W = P.W, s4 = P.s4, dpp_r = P.dpp_r,
+               Z = P.Z, dpp_t = P.dpp_t, Y = P.Y, gtt = P.gtt, gtp = P.gtp, X = P.X,
+               isn = P.isn, gpp = P.gpp, grr = P.grr, drr_t = P.drr_t, N = P.N, id2 = P.id2,
+               drr_r = P.drr_r, two_r = P.two_r;
+  (void)m;
   // ---- the contraction with lowered index ----
   const double vt = v.x, vr = v.y, vth = v.z, vph = v.w;
   const double w_t = dtt_r * vt + dtp_r * vph, w_p = dtp_r * vt + dpp_r * vph;
@@ -209,6 +240,28 @@ GG_HD inline RhsBar kerr_rhs_vjp_sc(double m, double a, double r, double st, dou
   o.th = b_st * ct - b_ct * st;                               // st = sin th, ct = cos th
   o.a = b_a;
   return o;
+}
+
+// a^mu = -g^{mu nu} f_nu at a point of kerr_point: the RHS value from the
+// same lowered-index contraction.
+GG_HD inline d4 kerr_rhs_pt(const KerrPoint &P, d4 v) {
+  GG_CONTRACT_FAST
+  const double vt = v.x, vr = v.y, vth = v.z, vph = v.w;
+  const double w_t = P.dtt_r * vt + P.dtp_r * vph, w_p = P.dtp_r * vt + P.dpp_r * vph;
+  const double w_r = P.drr_r * vr, w_th = P.two_r * vth;
+  const double u_t = P.dtt_t * vt + P.dtp_t * vph, u_p = P.dtp_t * vt + P.dpp_t * vph;
+  const double u_r = P.drr_t * vr, u_th = P.dsig * vth;
+  const double wv = w_t * vt + w_p * vph + w_r * vr + w_th * vth;
+  const double uv = u_t * vt + u_p * vph + u_r * vr + u_th * vth;
+  const double f_t = vr * w_t + vth * u_t, f_p = vr * w_p + vth * u_p;
+  const double f_r = vr * w_r + vth * u_r - 0.5 * wv;
+  const double f_th = vr * w_th + vth * u_th - 0.5 * uv;
+  return d4{-(P.gtt * f_t + P.gtp * f_p), -(P.grr * f_r), -(P.is * f_th), -(P.gtp * f_t + P.gpp * f_p)};
+}
+
+// The VJP at (r, theta) given st = sin theta, ct = cos theta.
+GG_HD inline RhsBar kerr_rhs_vjp_sc(double m, double a, double r, double st, double ct, d4 v, d4 lam) {
+  return kerr_rhs_vjp_pt(kerr_point(m, a, r, st, ct), v, lam);
 }
 
 GG_HD inline RhsBar kerr_rhs_vjp(double m, double a, double r, double th, d4 v, d4 lam) {
