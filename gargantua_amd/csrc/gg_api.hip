@@ -551,8 +551,10 @@ int plan_out(DevBuf &stage, void *user, size_t bytes, OutPlan &p) {
 }
 
 // Blocks per CU of a launch over n_rays.  blocks_per_cu = 0 picks by the
-// number of rays: two wavefronts per SIMD (8 per CU) when every lane gets at
-// least 3 rays, one per SIMD otherwise.  With a few rays per lane a render is
+// number of rays: three wavefronts per SIMD (12 per CU; kernels whose
+// registers allow fewer run the extra blocks after the resident ones, which
+// then find the queue empty) when every lane at two per SIMD gets at least 3
+// rays, one per SIMD otherwise.  With a few rays per lane a render is
 // bounded by its longest rays, which a wavefront alone on its SIMD advances
 // faster (512^2 Kerr renders and the differentiable render's recording pass:
 // 4-25% shorter at one wavefront per SIMD; 724^2 and larger: longer).
@@ -560,7 +562,7 @@ uint64_t blocks_per_cu_for(const gg_context *ctx, uint64_t n_rays) {
   if (ctx->launch.blocks_per_cu) return ctx->launch.blocks_per_cu;
   const uint64_t per_block = ctx->launch.block_threads / 64;
   const uint64_t lanes_two = (uint64_t)ctx->num_cus * 8 * 64;
-  const uint64_t waves_per_cu = n_rays >= 3 * lanes_two ? 8 : 4;
+  const uint64_t waves_per_cu = n_rays >= 3 * lanes_two ? 12 : 4;
   return waves_per_cu > per_block ? waves_per_cu / per_block : 1;
 }
 
@@ -1273,9 +1275,10 @@ int gg_autotune(gg_context *ctx, const gg_scene *scenes, uint32_t n, int precisi
   if (candidates && n_candidates) {
     cands.assign(candidates, candidates + n_candidates);
   } else {
+    // 4, 8 or 12 resident wavefronts per CU (one to three per SIMD)
     for (uint32_t bt : {64u, 128u, 256u})
-      for (uint32_t bpc : {2u, 4u, 8u})
-        for (uint32_t ri : {8u, 32u}) cands.push_back(gg_launch{bt, bpc * 256u / bt, ri, 2, 0});
+      for (uint32_t waves : {4u, 8u, 12u})
+        for (uint32_t ri : {8u, 32u}) cands.push_back(gg_launch{bt, waves * 64u / bt, ri, 2, 0});
   }
   const gg_launch saved = ctx->launch;
   std::vector<gg_outputs> none(n);

@@ -4,6 +4,9 @@
 // persistent trace kernel itself is gg_trace_kernel.hpp.
 #include <hip/hip_runtime.h>
 
+// The disk-mode trace kernels fit 168 VGPRs (a few spill slots in the FP64
+// tiers): three wavefronts per SIMD.
+#define GG_MIN_WAVES 3
 #include "gg_trace_kernel.hpp"
 
 namespace gg {

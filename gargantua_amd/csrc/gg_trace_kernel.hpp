@@ -54,11 +54,14 @@ __device__ inline uint32_t find_scene(const DevScene *scenes, uint32_t n, uint64
 // DISK 0 none, 1 volumetric, 2 thin; 3 the Hittable scene alone
 // (trace_ray_with_scene), 4 volumetric disk and scene
 // (trace_ray_with_disk_and_scene).
-// GG_MIN_WAVES: waves per SIMD the register budget must allow (2: 256
-// VGPRs; the run-time compiled user metrics take 1, for 512 VGPRs + AGPRs).
+// GG_MIN_WAVES: waves per SIMD the register budget must allow (3: 168
+// VGPRs, the disk modes; 2: 256, the Hittable scene modes; the run-time
+// compiled user metrics take 1, for 512 VGPRs + AGPRs).
 #ifndef GG_MIN_WAVES
 #define GG_MIN_WAVES 2
 #endif
+// Doubles of LDS per wavefront: five stage slots of 64 lanes x 4 doubles.
+constexpr uint32_t KREG = 5 * 64 * 4;
 template <int METRIC, int DISK, int PREC, int SRC, int BATCH>
 __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T) {
 #pragma clang fp contract(off)
@@ -66,8 +69,10 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
   const uint64_t n = T.n_rays;
   constexpr bool single = (BATCH == 0);
 
+  // LDS: one region of KREG doubles per wavefront (its lanes' stage slots), so
+  // the slot offsets are immediates whatever the block size
   extern __shared__ __align__(16) double lds_k[];
-  const KStore K{reinterpret_cast<double2 *>(lds_k) + threadIdx.x, blockDim.x};
+  const KStoreT<64> K{reinterpret_cast<double2 *>(lds_k + (threadIdx.x >> 6) * KREG) + (threadIdx.x & 63), 64};
   Lane L;
   L.n_rhs = 0; L.n_rhs32 = 0; L.n_rhsf = 0; L.n_full = 0; L.n_refl = 0;
   uint32_t n_att = 0, n_retired = 0;
@@ -169,48 +174,51 @@ __global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T)
                              : key == 1 ? n0 + o1 + __popcll(m1 & below)
                                         : n0 + n1 + o2 + __popcll(m2 & below);
         const uint32_t n_act = n0 + n1 + n2;
-        // The exchange reuses stage slots 1..4 (free between attempts) as
-        // 16 planes of B doubles; two passes of at most 14 fields.
-        double *X = lds_k + 4 * B;
+        // The exchange reuses stage slots 1..4 of every wavefront's region
+        // (free between attempts) as 16 planes of 64 doubles per wavefront:
+        // field f of block lane i at X(f, i); two passes of at most 14 fields.
+        auto X = [&](uint32_t f, uint32_t i) -> double & {
+          return lds_k[(i >> 6) * KREG + 256 + f * 64 + (i & 63)];
+        };
         d4 k1 = K.get(0);
         __syncthreads();
         if (active) {
-          X[0 * B + dst] = L.pos[0]; X[1 * B + dst] = L.pos[1]; X[2 * B + dst] = L.pos[2]; X[3 * B + dst] = L.pos[3];
-          X[4 * B + dst] = L.vel.x; X[5 * B + dst] = L.vel.y; X[6 * B + dst] = L.vel.z; X[7 * B + dst] = L.vel.w;
-          X[8 * B + dst] = L.lambda; X[9 * B + dst] = L.h;
-          X[10 * B + dst] = k1.x; X[11 * B + dst] = k1.y; X[12 * B + dst] = k1.z; X[13 * B + dst] = k1.w;
+          X(0, dst) = L.pos[0]; X(1, dst) = L.pos[1]; X(2, dst) = L.pos[2]; X(3, dst) = L.pos[3];
+          X(4, dst) = L.vel.x; X(5, dst) = L.vel.y; X(6, dst) = L.vel.z; X(7, dst) = L.vel.w;
+          X(8, dst) = L.lambda; X(9, dst) = L.h;
+          X(10, dst) = k1.x; X(11, dst) = k1.y; X(12, dst) = k1.z; X(13, dst) = k1.w;
         }
         __syncthreads();
         const bool now_active = tid < n_act;
         if (now_active) {
-          L.pos[0] = X[0 * B + tid]; L.pos[1] = X[1 * B + tid]; L.pos[2] = X[2 * B + tid]; L.pos[3] = X[3 * B + tid];
-          L.vel = d4{X[4 * B + tid], X[5 * B + tid], X[6 * B + tid], X[7 * B + tid]};
-          L.lambda = X[8 * B + tid]; L.h = X[9 * B + tid];
-          k1 = d4{X[10 * B + tid], X[11 * B + tid], X[12 * B + tid], X[13 * B + tid]};
+          L.pos[0] = X(0, tid); L.pos[1] = X(1, tid); L.pos[2] = X(2, tid); L.pos[3] = X(3, tid);
+          L.vel = d4{X(4, tid), X(5, tid), X(6, tid), X(7, tid)};
+          L.lambda = X(8, tid); L.h = X(9, tid);
+          k1 = d4{X(10, tid), X(11, tid), X(12, tid), X(13, tid)};
         }
         __syncthreads();
         if (active) {
-          X[0 * B + dst] = L.intensity; X[1 * B + dst] = L.transmission; X[2 * B + dst] = L.err_prev;
-          X[3 * B + dst] = __longlong_as_double((long long)ray);
-          X[4 * B + dst] = __longlong_as_double((long long)out_i);
-          X[5 * B + dst] = __longlong_as_double((long long)(((uint64_t)L.attempts << 32) | L.accepted));
-          X[6 * B + dst] = __longlong_as_double(
+          X(0, dst) = L.intensity; X(1, dst) = L.transmission; X(2, dst) = L.err_prev;
+          X(3, dst) = __longlong_as_double((long long)ray);
+          X(4, dst) = __longlong_as_double((long long)out_i);
+          X(5, dst) = __longlong_as_double((long long)(((uint64_t)L.attempts << 32) | L.accepted));
+          X(6, dst) = __longlong_as_double(
               (long long)(((uint64_t)si << 3) | (L.k1_fast ? 4u : 0u) | (L.k1_f32 ? 2u : 0u) | (L.k1_ok ? 1u : 0u)));
-          X[7 * B + dst] = L.b2;
-          if (record) X[8 * B + dst] = __longlong_as_double((long long)(((uint64_t)r_need << 32) | r_chunk));
+          X(7, dst) = L.b2;
+          if (record) X(8, dst) = __longlong_as_double((long long)(((uint64_t)r_need << 32) | r_chunk));
         }
         __syncthreads();
         if (now_active) {
-          L.intensity = X[0 * B + tid]; L.transmission = X[1 * B + tid]; L.err_prev = X[2 * B + tid];
-          ray = (uint64_t)__double_as_longlong(X[3 * B + tid]);
-          out_i = (uint64_t)__double_as_longlong(X[4 * B + tid]);
-          uint64_t aa = (uint64_t)__double_as_longlong(X[5 * B + tid]);
+          L.intensity = X(0, tid); L.transmission = X(1, tid); L.err_prev = X(2, tid);
+          ray = (uint64_t)__double_as_longlong(X(3, tid));
+          out_i = (uint64_t)__double_as_longlong(X(4, tid));
+          uint64_t aa = (uint64_t)__double_as_longlong(X(5, tid));
           L.attempts = (uint32_t)(aa >> 32); L.accepted = (uint32_t)aa;
-          uint64_t sf = (uint64_t)__double_as_longlong(X[6 * B + tid]);
+          uint64_t sf = (uint64_t)__double_as_longlong(X(6, tid));
           si = (uint32_t)(sf >> 3); L.k1_ok = (sf & 1u) != 0; L.k1_f32 = (sf & 2u) != 0; L.k1_fast = (sf & 4u) != 0;
-          L.b2 = X[7 * B + tid];
+          L.b2 = X(7, tid);
           if (record) {
-            const uint64_t cw = (uint64_t)__double_as_longlong(X[8 * B + tid]);
+            const uint64_t cw = (uint64_t)__double_as_longlong(X(8, tid));
             r_chunk = (uint32_t)cw;
             r_need = (cw >> 32) != 0;
           }
