@@ -822,11 +822,13 @@ int compile_user_module(const std::string &user, int mode, int src, std::vector<
   const char *at_expr = "gg::gg_user_metric_at";
   hiprtcAddNameExpression(p, expr);
   hiprtcAddNameExpression(p, at_expr);
-  // The library's own flags (gfx950, no FMA contraction, no packed FP32), and
-  // one wave per SIMD: the nine inlined tensor evaluations of each RHS then
-  // keep their derivatives in VGPRs and AGPRs instead of scratch memory.
+  // The library's own flags (gfx950, no FMA contraction, no packed FP32, no
+  // machine LICM), and one wave per SIMD: the nine inlined tensor evaluations
+  // of each RHS then keep their derivatives in VGPRs and AGPRs instead of
+  // scratch memory.
   const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
-                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-DGG_MIN_WAVES=1"};
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                        "-mllvm", "-disable-machine-licm", "-DGG_MIN_WAVES=1"};
   const hiprtcResult r = hiprtcCompileProgram(p, (int)(sizeof opts / sizeof opts[0]), opts);
   size_t n = 0;
   hiprtcGetProgramLogSize(p, &n);
