@@ -58,7 +58,6 @@ def profiled_traffic(profile: str, kernel: str):
     except (OSError, ValueError, IndexError):
         return None
     return None
-    return None
 
 METRIC = "geodesics/s Kerr a=0.9 1024²: mixed FP32 vs FP64 at ≤1e-6 err; adjoint grads/s"
 
