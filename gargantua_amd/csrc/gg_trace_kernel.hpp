@@ -56,14 +56,16 @@ __device__ inline uint32_t find_scene(const DevScene *scenes, uint32_t n, uint64
 // (trace_ray_with_disk_and_scene).
 // GG_MIN_WAVES: waves per SIMD the register budget must allow (3: 168
 // VGPRs, the disk modes; 2: 256, the Hittable scene modes; the run-time
-// compiled user metrics take 1, for 512 VGPRs + AGPRs).
+// compiled user metrics take 1, for 512 VGPRs + AGPRs).  The recording
+// kernels (SRC 2, the differentiable render's forward pass, whose grids are
+// small enough to run one wavefront per SIMD) take at most 2: no scratch.
 #ifndef GG_MIN_WAVES
 #define GG_MIN_WAVES 2
 #endif
 // Doubles of LDS per wavefront: five stage slots of 64 lanes x 4 doubles.
 constexpr uint32_t KREG = 5 * 64 * 4;
 template <int METRIC, int DISK, int PREC, int SRC, int BATCH>
-__global__ __launch_bounds__(256, GG_MIN_WAVES) void trace_kernel(TraceLaunch T) {
+__global__ __launch_bounds__(256, (SRC == 2 && GG_MIN_WAVES > 2) ? 2 : GG_MIN_WAVES) void trace_kernel(TraceLaunch T) {
 #pragma clang fp contract(off)
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t n = T.n_rays;

@@ -140,3 +140,24 @@ def test_learned_ray_order_in_batches(renderer):
         for a, b in zip(outs, ref):
             assert_same(a, b, "fp64")
     renderer.set_launch(256, 2, 16, 1, 0)
+
+
+def test_default_launch_shape_by_ray_count(renderer):
+    """blocks_per_cu = 0 (the default) launches one wavefront per SIMD below
+    3 rays per lane at two (a 512^2 image: 4 blocks of 64 threads per CU)
+    and three per SIMD above (1024^2: 12 per CU); the FP64 outputs are those
+    of an explicit launch shape, bit for bit."""
+    small = gg.configs.kerr_1024(512, 512)
+    large = gg.configs.kerr_1024(1024, 1024)
+    renderer.set_launch()
+    assert renderer.get_launch()["blocks_per_cu"] == 0
+    a_small = renderer.render(small, "fp64", state=False)
+    a_large = renderer.render(large, "fp64", state=False)
+    gs, gl = a_small["stats"]["grid_blocks"], a_large["stats"]["grid_blocks"]
+    assert a_small["stats"]["block_threads"] == 64 and gs % 4 == 0 and gl == 3 * gs, (gs, gl)
+    renderer.set_launch(64, 8, 64, 1, 0)
+    for sc, a in ((small, a_small), (large, a_large)):
+        b = renderer.render(sc, "fp64", state=False)
+        assert b["stats"]["grid_blocks"] == 2 * gs
+        assert_same(a, b, "fp64")
+    renderer.set_launch(256, 2, 16, 1, 0)
