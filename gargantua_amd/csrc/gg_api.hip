@@ -550,27 +550,28 @@ int plan_out(DevBuf &stage, void *user, size_t bytes, OutPlan &p) {
   return GG_OK;
 }
 
-// Blocks per CU of a launch over n_rays.  blocks_per_cu = 0 picks by the
-// number of rays: three wavefronts per SIMD (12 per CU; kernels whose
-// registers allow fewer run the extra blocks after the resident ones, which
-// then find the queue empty) when every lane at two per SIMD gets at least 3
-// rays, one per SIMD otherwise.  With a few rays per lane a render is
-// bounded by its longest rays, which a wavefront alone on its SIMD advances
-// faster (512^2 Kerr renders and the differentiable render's recording pass:
-// 4-25% shorter at one wavefront per SIMD; 724^2 and larger: longer).
-uint64_t blocks_per_cu_for(const gg_context *ctx, uint64_t n_rays) {
+// Blocks per CU of a launch over n_rays of a kernel built for at most
+// max_waves wavefronts per SIMD (3: the disk modes; 2: the Hittable scene
+// modes and the recording renders; 1: run-time user metrics).
+// blocks_per_cu = 0 picks by the number of rays: max_waves wavefronts per
+// SIMD when every lane at two per SIMD gets at least 3 rays, one per SIMD
+// otherwise.  With a few rays per lane a render is bounded by its longest
+// rays, which a wavefront alone on its SIMD advances faster (512^2 Kerr
+// renders and the differentiable render's recording pass: 4-25% shorter at
+// one wavefront per SIMD; 724^2 and larger: longer).
+uint64_t blocks_per_cu_for(const gg_context *ctx, uint64_t n_rays, uint32_t max_waves) {
   if (ctx->launch.blocks_per_cu) return ctx->launch.blocks_per_cu;
   const uint64_t per_block = ctx->launch.block_threads / 64;
   const uint64_t lanes_two = (uint64_t)ctx->num_cus * 8 * 64;
-  const uint64_t waves_per_cu = n_rays >= 3 * lanes_two ? 12 : 4;
+  const uint64_t waves_per_cu = n_rays >= 3 * lanes_two ? 4 * (uint64_t)max_waves : 4;
   return waves_per_cu > per_block ? waves_per_cu / per_block : 1;
 }
 
-uint32_t grid_for(gg_context *ctx, uint64_t n_rays) {
+uint32_t grid_for(gg_context *ctx, uint64_t n_rays, uint32_t max_waves) {
   uint64_t waves_needed = (n_rays + 63) / 64;
   uint64_t per_block = ctx->launch.block_threads / 64;
   uint64_t blocks_needed = (waves_needed + per_block - 1) / per_block;
-  uint64_t grid = (uint64_t)ctx->num_cus * blocks_per_cu_for(ctx, n_rays);
+  uint64_t grid = (uint64_t)ctx->num_cus * blocks_per_cu_for(ctx, n_rays, max_waves);
   if (blocks_needed < grid) grid = blocks_needed;
   if (grid < 1) grid = 1;
   return (uint32_t)grid;
@@ -1041,7 +1042,9 @@ int render_impl(gg_context *ctx, const gg_scene *scenes, uint32_t n_scenes, int 
   T.rec_cap = ctx->rec_req.cap;
   T.rec_cursor = ctx->rec_req.cursor;
   T.rec_last = ctx->rec_req.last;
-  uint32_t grid = grid_for(ctx, total);
+  // wavefronts per SIMD the launched kernel's registers allow (GG_MIN_WAVES)
+  const uint32_t max_waves = user ? 1u : (mode >= 3 || ctx->rec_req.pool) ? 2u : 3u;
+  uint32_t grid = grid_for(ctx, total, max_waves);
   uint32_t block = ctx->launch.block_threads;
   const bool use_graph = (n_scenes > 1) && ctx->launch.graph == 1 && total > 0;
   if (use_graph) {
@@ -1239,7 +1242,7 @@ int gg_render_batch_noalloc(gg_context *ctx, const gg_scene *scenes, uint32_t n,
   T.n_rays = total;
   T.counters = cnt_d;
   T.refill_interval = ctx->launch.refill_interval;
-  const uint32_t grid = grid_for(ctx, total);
+  const uint32_t grid = grid_for(ctx, total, 3);   // disk modes only (checked above)
   const uint32_t block = ctx->launch.block_threads;
   HIP_TRY(launch_reset(cnt_d, s));       // the counters start at zero (a kernel, not a memset)
   HIP_TRY(hipEventRecord(ctx->ev0, s));
