@@ -27,8 +27,7 @@
 //       must equal the next checkpoint bit for bit).  When K is at least
 //       every ray's accepted count there is one segment per ray, the
 //       re-trace stores every step record, and adjoint_rev_stored_kernel
-//       (the same sweep without recomputation, two wavefronts per SIMD)
-//       reads them.  adjoint_rev_chunked_kernel reads the records the
+//       (the same sweep without recomputation) reads them.  adjoint_rev_chunked_kernel reads the records the
 //       recording render wrote instead (adjoint_seed_kernel computes the
 //       crossing seeds from them).
 //   adjoint_cam_kernel  adds the camera ray's dependence on (a, i) and
@@ -479,8 +478,8 @@ __global__ __launch_bounds__(FWD_BLOCK, 2) void adjoint_fwd_kernel(GradLaunch G)
 // recomputed from their checkpoints.  MODE 1: every step record of every ray
 // is in the last-segment pool (one segment per ray).  MODE 2: the records are
 // the recording render's chunks, walked back through their back-pointers.
-// Modes 1 and 2 only read records (fewer live registers: two wavefronts per
-// SIMD).
+// Every mode runs one wavefront per SIMD (64-thread blocks, 4 per CU), the
+// record of the attempt before the current one loading into AGPRs.
 template <int MODE>
 __device__ __forceinline__ void adjoint_rev_body(const GradLaunch &G) {
 #pragma clang fp contract(off)
