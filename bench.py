@@ -164,14 +164,18 @@ def _roofline_frac(stats, kernel_ms):
 
 
 def _vs_oracle(gpu, ora, band=None):
-    """Class agreement (outside the band too when given), final states within 1e-6, attempt counts."""
-    from tests.parity import state_rel_err
+    """Class agreement (outside the band too when given), final states within 1e-6, attempt counts, PSNR of
+    the disk intensity over the sampled rays."""
+    from tests.parity import psnr, state_rel_err
     eq = gpu["cls"] == ora["cls"]
     err = state_rel_err(gpu, ora)
     out = {"rays": int(len(eq)), "class_agree": float(np.mean(eq)),
            "err_le_1e-6": float(np.mean(err[eq] <= 1e-6)) if eq.any() else 1.0,
            "err_median": float(np.median(err[eq])) if eq.any() else 0.0,
-           "attempts_equal": float(np.mean(gpu["attempts"] == ora["attempts"]))}
+           "attempts_equal": float(np.mean(gpu["attempts"] == ora["attempts"])),
+           "psnr_intensity": psnr(gpu["intensity"], ora["intensity"])}
+    if not math.isfinite(out["psnr_intensity"]):
+        out["psnr_intensity"] = None      # identical intensities
     if band is not None:
         ok = eq & ~band
         out["class_agree_outside_band"] = float(np.mean(eq[~band])) if (~band).any() else 1.0
@@ -263,7 +267,7 @@ def frame_batch64(torch, rend, ora_stride=16):
     t0 = time.perf_counter()
     refs = [ora.render_rays(s, ix.astype(np.uint64), state=False) for s, ix in zip(scenes, idxs)]
     dt = time.perf_counter() - t0
-    cat = lambda parts: {k: np.concatenate([p[k] for p in parts]) for k in ("cls", "r", "theta", "phi", "attempts")}
+    cat = lambda parts: {k: np.concatenate([p[k] for p in parts]) for k in ("cls", "r", "theta", "phi", "attempts", "intensity")}
     ref_all, band_all = cat(refs), np.concatenate(bands)
     worst = {p: min(_vs_oracle(g, r)["err_le_1e-6"] for g, r in zip(sub[p], refs)) for p in sub}
     return {"workload": "64 seeded Kerr scenes (generator seed 2026) in one graph-captured launch",
