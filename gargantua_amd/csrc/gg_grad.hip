@@ -27,9 +27,10 @@
 //       must equal the next checkpoint bit for bit).  When K is at least
 //       every ray's accepted count there is one segment per ray, the
 //       re-trace stores every step record, and adjoint_rev_stored_kernel
-//       (the same sweep without recomputation) reads them.  adjoint_rev_chunked_kernel reads the records the
-//       recording render wrote instead (adjoint_seed_kernel computes the
-//       crossing seeds from them).
+//       (the same sweep without recomputation) reads them.
+//       adjoint_rev_chunked_kernel reads the records the recording render
+//       wrote instead (adjoint_seed_kernel computes the crossing seeds from
+//       them).
 //   adjoint_cam_kernel  adds the camera ray's dependence on (a, i) and
 //       writes dI/d(a, i, q, r_in) per pixel.
 #include <hip/hip_runtime.h>

@@ -45,8 +45,8 @@ struct RhsBar {
 // The metric algebra of the VJP at (r, theta), given st = sin theta and
 // ct = cos theta: the inverse metric, the metric derivatives and the
 // intermediates the backward sweep reads.  It does not depend on the
-// velocity or the cotangent, so one evaluation serves every product (and
-// the RHS value) at the same point.
+// velocity or the cotangent, so one evaluation serves every product at the
+// same point.
 struct KerrPoint {
   double m, a, r, st, ct;
   double tm, a2, r2, sin2, sc, ct2, sigma, delta, is, id, is2, isd, dsig, rp, sm, dtt_r, dtt_t, dtp_r, U, dtp_t;
@@ -240,23 +240,6 @@ GG_HD inline RhsBar kerr_rhs_vjp_pt(const KerrPoint &P, d4 v, d4 lam) {
   o.th = b_st * ct - b_ct * st;                               // st = sin th, ct = cos th
   o.a = b_a;
   return o;
-}
-
-// a^mu = -g^{mu nu} f_nu at a point of kerr_point: the RHS value from the
-// same lowered-index contraction.
-GG_HD inline d4 kerr_rhs_pt(const KerrPoint &P, d4 v) {
-  GG_CONTRACT_FAST
-  const double vt = v.x, vr = v.y, vth = v.z, vph = v.w;
-  const double w_t = P.dtt_r * vt + P.dtp_r * vph, w_p = P.dtp_r * vt + P.dpp_r * vph;
-  const double w_r = P.drr_r * vr, w_th = P.two_r * vth;
-  const double u_t = P.dtt_t * vt + P.dtp_t * vph, u_p = P.dtp_t * vt + P.dpp_t * vph;
-  const double u_r = P.drr_t * vr, u_th = P.dsig * vth;
-  const double wv = w_t * vt + w_p * vph + w_r * vr + w_th * vth;
-  const double uv = u_t * vt + u_p * vph + u_r * vr + u_th * vth;
-  const double f_t = vr * w_t + vth * u_t, f_p = vr * w_p + vth * u_p;
-  const double f_r = vr * w_r + vth * u_r - 0.5 * wv;
-  const double f_th = vr * w_th + vth * u_th - 0.5 * uv;
-  return d4{-(P.gtt * f_t + P.gtp * f_p), -(P.grr * f_r), -(P.is * f_th), -(P.gtp * f_t + P.gpp * f_p)};
 }
 
 // The VJP at (r, theta) given st = sin theta, ct = cos theta.

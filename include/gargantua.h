@@ -234,8 +234,12 @@ typedef struct gg_stats {
 typedef struct gg_launch {
   uint32_t block_threads;     /* 64..256, multiple of 64 (default 64)         */
   uint32_t blocks_per_cu;     /* blocks launched per CU; 0 (default) = by ray
-                                 count: 8 wavefronts per CU when every lane
-                                 gets at least 3 rays, else 4 (one per SIMD) */
+                                 count: when every lane at 8 wavefronts per CU
+                                 gets at least 3 rays, as many wavefronts as
+                                 the kernel's registers allow (12 per CU for
+                                 the disk modes, 8 for the scene modes and
+                                 recording renders, 4 for user metrics),
+                                 else 4 (one per SIMD)                         */
   uint32_t refill_interval;   /* attempts between retired-lane refills (64)   */
   uint32_t graph;             /* gg_render_batch: 1 = capture the launch in a
                                  hipGraph and replay it on later calls with the
